@@ -1,0 +1,285 @@
+/*
+ * micm_hip.h -- C-ABI of the MI355X-native Rosenbrock chemistry solver.
+ *
+ * The boundary replaces the C++-ABI "Driver" seam of the reference's CUDA
+ * backend (plain functions taking POD parameter structs, e.g.
+ * include/micm/cuda/process/cuda_process_set.cuh:29-41) with extern "C"
+ * entry points that take plain pointers and sizes, return an int status and
+ * never throw.  Every per-cell array handed to a kernel entry point lives in
+ * device memory in the reference's vector ordering with a group size of one
+ * wavefront (64 cells):
+ *
+ *     dense  (cell c, variable v):  ((c / 64) * n_vars + v) * 64 + c % 64
+ *     sparse (cell c, non-zero p):  ((c / 64) * nnz    + p) * 64 + c % 64
+ *
+ * which is include/micm/util/vector_matrix.hpp:200 (VectorMatrix<T, 64>) and
+ * include/micm/util/sparse_matrix_vector_ordering_compressed_sparse_row.hpp:79-89
+ * (SparseMatrixVectorOrderingCompressedSparseRow<64>).  Buffers hold
+ * ceil(n_cells / 64) full groups; padding lanes are never read as results.
+ *
+ * Species and reactions use the reference's post-Build orders: species are
+ * permuted by the diagonal Markowitz reordering
+ * (include/micm/solver/solver_builder.inl:78-119) unless the description
+ * disables it, and reactions are stable-sorted by rate-constant type
+ * (solver_builder.inl:274-329).  The getters below report both permutations.
+ */
+#ifndef MICM_HIP_H
+#define MICM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define MICM_HIP_OK 0
+#define MICM_HIP_ERR_INVALID_ARGUMENT (-1)
+#define MICM_HIP_ERR_DEVICE (-2)
+#define MICM_HIP_ERR_OUT_OF_MEMORY (-3)
+#define MICM_HIP_ERR_UNSUPPORTED (-4)
+#define MICM_HIP_ERR_SPECIES (-5) /* unknown / unused species, as MicmException in process_set.hpp:291-309 */
+
+/* Cells per vector group (one CDNA wavefront). */
+#define MICM_HIP_GROUP 64
+
+/* Doubles per reaction in micm_hip_mechanism_desc::rate_params. */
+#define MICM_HIP_RATE_PARAM_STRIDE 24
+
+/* Rate-constant types, numbered in the reference's sort order
+ * (solver_builder.inl:279-324).  Parameter layout per reaction row of
+ * rate_params (unused slots ignored):
+ *   ARRHENIUS    A, B, C, D, E                       (arrhenius_rate_constant.hpp)
+ *   TROE         k0_A, k0_B, k0_C, kinf_A, kinf_B, kinf_C, Fc, N
+ *   TERNARY      same as TROE                        (ternary_chemical_activation_rate_constant.hpp)
+ *   BRANCHED     branch (0 alkoxy, 1 nitrate), X, Y, a0, n
+ *   TUNNELING    A, B, C
+ *   TAYLOR       A, B, C, D, E, n_coefficients, c_0 .. c_15
+ *   REVERSIBLE   A, C, k_r
+ *   USER_DEFINED scaling_factor       (one custom parameter per reaction)
+ *   SURFACE      diffusion_coefficient, molecular_weight, reaction_probability
+ *                                     (two custom parameters per reaction)
+ */
+#define MICM_HIP_RATE_ARRHENIUS 0
+#define MICM_HIP_RATE_TROE 1
+#define MICM_HIP_RATE_TERNARY 2
+#define MICM_HIP_RATE_BRANCHED 3
+#define MICM_HIP_RATE_TUNNELING 4
+#define MICM_HIP_RATE_TAYLOR 5
+#define MICM_HIP_RATE_REVERSIBLE 6
+#define MICM_HIP_RATE_USER_DEFINED 7
+#define MICM_HIP_RATE_SURFACE 8
+#define MICM_HIP_RATE_N_TYPES 9
+
+/* Solver states, numbered as micm::SolverState (solver_result.hpp:13-35). */
+#define MICM_HIP_STATE_NOT_YET_CALLED 0
+#define MICM_HIP_STATE_RUNNING 1
+#define MICM_HIP_STATE_CONVERGED 2
+#define MICM_HIP_STATE_MAX_STEPS 3
+#define MICM_HIP_STATE_STEP_SIZE_TOO_SMALL 4
+#define MICM_HIP_STATE_REPEATEDLY_SINGULAR 5
+#define MICM_HIP_STATE_NAN_DETECTED 6
+#define MICM_HIP_STATE_INF_DETECTED 7
+
+/* Per-cell statistics, in the field order of micm::SolverStats
+ * (solver_result.hpp:37-57); final_time is reported separately. */
+#define MICM_HIP_STAT_FUNCTION_CALLS 0
+#define MICM_HIP_STAT_JACOBIAN_UPDATES 1
+#define MICM_HIP_STAT_STEPS 2
+#define MICM_HIP_STAT_ACCEPTED 3
+#define MICM_HIP_STAT_REJECTED 4
+#define MICM_HIP_STAT_DECOMPOSITIONS 5
+#define MICM_HIP_STAT_SOLVES 6
+#define MICM_HIP_N_STATS 7
+
+/* Tableau presets (rosenbrock_solver_parameters.hpp:152-433). */
+#define MICM_HIP_ROS_TWO_STAGE 2
+#define MICM_HIP_ROS_THREE_STAGE 3
+#define MICM_HIP_ROS_FOUR_STAGE 4
+#define MICM_HIP_ROS_FOUR_STAGE_DA 41
+#define MICM_HIP_ROS_SIX_STAGE_DA 61
+
+/* Integration forms of micm_hip_solve. */
+#define MICM_HIP_SOLVE_LOCKSTEP 0   /* one step size per state, as rosenbrock.inl:12-304 */
+#define MICM_HIP_SOLVE_PLAIN 1      /* per-cell step size, one launch per kernel per stage */
+#define MICM_HIP_SOLVE_PERSISTENT 2 /* per-cell step size, persistent work-queue kernel */
+
+/* Flat mechanism description.  Species are the phase's non-parameterized
+ * species in declaration order; reactions are in the order given to
+ * SolverBuilder::SetReactions. */
+typedef struct micm_hip_mechanism_desc {
+  int64_t n_species;
+  int64_t n_reactions;
+  const int64_t* reactant_offsets; /* [n_reactions + 1] */
+  const int64_t* reactants;        /* species index, or -(p + 1) for parameterized species p */
+  const int64_t* product_offsets;  /* [n_reactions + 1] */
+  const int64_t* products;         /* species index, or -(p + 1) (skipped, process_set.hpp:301) */
+  const double* yields;            /* one per product */
+  const int32_t* rate_types;       /* [n_reactions], MICM_HIP_RATE_* */
+  const double* rate_params;       /* [n_reactions * MICM_HIP_RATE_PARAM_STRIDE] */
+  int64_t n_parameterized;         /* parameterized ("third-body") species */
+  const double* parameterized;     /* [n_parameterized * 4]: c0, c_T, c_P, c_rho (parameterized_function.hpp:36) */
+  const double* absolute_tolerances; /* [n_species] in declaration order, or NULL for 1e-3 (solver_builder.inl:208) */
+  int32_t reorder_state;           /* 1: diagonal Markowitz order (reference default), 0: declaration order */
+  int32_t ignore_unused_species;   /* 1: reference default (solver_builder.hpp:64) */
+} micm_hip_mechanism_desc;
+
+typedef struct micm_hip_mechanism_info {
+  int64_t n_species;
+  int64_t n_reactions;
+  int64_t n_custom_parameters;
+  int64_t nnz_jacobian;   /* including the diagonal (jacobian.hpp:27-30) */
+  int64_t nnz_lower;      /* including the unit diagonal */
+  int64_t nnz_upper;      /* including the diagonal */
+  int64_t n_process_infos;
+  int64_t n_jacobian_terms;
+  int64_t lu_multiply_adds; /* Doolittle L*U products per factorization */
+  int64_t n_reactant_slots;
+  int64_t n_product_slots;
+} micm_hip_mechanism_info;
+
+/* micm::RosenbrockSolverParameters (rosenbrock_solver_parameters.hpp:22-67). */
+typedef struct micm_hip_rosenbrock_parameters {
+  int64_t stages;
+  int64_t max_number_of_steps;
+  double round_off;
+  double factor_min;
+  double factor_max;
+  double rejection_factor_decrease;
+  double safety_factor;
+  double h_min;
+  double h_max;
+  double h_start;
+  int32_t new_function_evaluation[6];
+  double estimator_of_local_order;
+  double a[15];
+  double c[15];
+  double m[6];
+  double e[6];
+  double alpha[6];
+  double gamma[6];
+} micm_hip_rosenbrock_parameters;
+
+typedef struct micm_hip_mechanism micm_hip_mechanism;
+typedef struct micm_hip_state micm_hip_state;
+
+/* Device addresses of a state's arrays (grouped layout above). */
+typedef struct micm_hip_state_pointers {
+  double* variables;         /* n_species */
+  double* conditions;        /* 3: temperature [K], pressure [Pa], air density [mol m-3] */
+  double* custom_parameters; /* n_custom_parameters */
+  double* rate_constants;    /* n_reactions */
+  double* jacobian;          /* nnz_jacobian (holds -J, as the reference) */
+  double* lower;             /* nnz_lower */
+  double* upper;             /* nnz_upper */
+  double* absolute_tolerances; /* [n_species], not grouped */
+  int32_t* cell_state;       /* [n_cells] MICM_HIP_STATE_* of the last solve */
+  int64_t* cell_stats;       /* [n_cells * MICM_HIP_N_STATS] */
+  double* cell_final_time;   /* [n_cells] */
+  int64_t n_cells;
+  int64_t n_groups;
+} micm_hip_state_pointers;
+
+/* ------------------------------------------------------------ errors */
+/* Message of the last failing call on this thread (never NULL). */
+const char* micm_hip_last_error(void);
+
+/* ------------------------------------------------------------ solver parameters */
+/* RosenbrockSolverParameters::*StageRosenbrockParameters (rosenbrock_solver_parameters.hpp:152-433). */
+int micm_hip_rosenbrock_parameters_preset(int32_t preset, micm_hip_rosenbrock_parameters* out);
+
+/* ------------------------------------------------------------ mechanism */
+/* Replaces SolverBuilder::Build's symbolic work (solver_builder.inl:227-548):
+ * reaction sort, species map, ProcessSet tables (process_set.hpp:256-492),
+ * Jacobian pattern (jacobian.hpp:15-32), Doolittle fill and schedule
+ * (lu_decomposition_doolittle.inl:69-299), LinearSolver tables
+ * (linear_solver.inl:166-224) and the rate-constant store
+ * (reaction_rate_store.hpp:221-368); uploads every table to the current device. */
+int micm_hip_mechanism_create(const micm_hip_mechanism_desc* desc, micm_hip_mechanism** out);
+void micm_hip_mechanism_destroy(micm_hip_mechanism* mechanism);
+int micm_hip_mechanism_get_info(const micm_hip_mechanism* mechanism, micm_hip_mechanism_info* out);
+/* state_to_declared[i] = declared species index held in state column i. */
+int micm_hip_mechanism_get_species_order(const micm_hip_mechanism* mechanism, int64_t* state_to_declared);
+/* sorted_to_declared[r] = declared reaction index of rate-constant column r. */
+int micm_hip_mechanism_get_reaction_order(const micm_hip_mechanism* mechanism, int64_t* sorted_to_declared);
+/* custom_to_reaction[p] = sorted reaction index whose rate uses custom parameter p. */
+int micm_hip_mechanism_get_custom_parameter_owner(const micm_hip_mechanism* mechanism, int64_t* custom_to_reaction);
+/* (row, column) of every stored element in storage order; any pointer may be NULL. */
+int micm_hip_mechanism_get_patterns(const micm_hip_mechanism* mechanism,
+                                    int64_t* jacobian_rows, int64_t* jacobian_cols,
+                                    int64_t* lower_rows, int64_t* lower_cols,
+                                    int64_t* upper_rows, int64_t* upper_cols);
+/* Storage index of each diagonal element of the Jacobian (state.inl:313). */
+int micm_hip_mechanism_get_jacobian_diagonal(const micm_hip_mechanism* mechanism, int64_t* diagonal);
+
+/* ------------------------------------------------------------ kernel entry points
+ * All pointers are device pointers in the grouped layout; `stream` is a
+ * hipStream_t (NULL = the legacy default stream).  Launches are asynchronous. */
+
+/* CalculateRateConstantsKernelDriver (cuda_rate_constant_kernel.cuh:68-73) and the
+ * parameterized multipliers (reaction_rate_store.hpp:510-517): conditions hold
+ * 3 values per cell (T, P, air density). */
+int micm_hip_calculate_rate_constants(const micm_hip_mechanism* mechanism, int64_t n_cells,
+                                      const double* conditions, const double* custom_parameters,
+                                      double* rate_constants, void* stream);
+/* AddForcingTermsKernelDriver (cuda_process_set.cuh:29-35): forcing += f(y). */
+int micm_hip_add_forcing_terms(const micm_hip_mechanism* mechanism, int64_t n_cells,
+                               const double* rate_constants, const double* variables,
+                               double* forcing, void* stream);
+/* SubtractJacobianTermsKernelDriver (cuda_process_set.cuh:37-41): jacobian -= J(y). */
+int micm_hip_subtract_jacobian_terms(const micm_hip_mechanism* mechanism, int64_t n_cells,
+                                     const double* rate_constants, const double* variables,
+                                     double* jacobian, void* stream);
+/* AlphaMinusJacobianDriver (cuda_rosenbrock.cuh:18-22): jacobian[diag] += alpha. */
+int micm_hip_alpha_minus_jacobian(const micm_hip_mechanism* mechanism, int64_t n_cells,
+                                  double* jacobian, double alpha, void* stream);
+/* LuDecompositionDoolittle::Decompose (lu_decomposition_doolittle.inl:325-403). */
+int micm_hip_lu_decompose(const micm_hip_mechanism* mechanism, int64_t n_cells,
+                          const double* jacobian, double* lower, double* upper, void* stream);
+/* LinearSolver::Solve (linear_solver.inl:235-294): x <- U^-1 L^-1 x in place. */
+int micm_hip_lu_solve(const micm_hip_mechanism* mechanism, int64_t n_cells,
+                      const double* lower, const double* upper, double* x, void* stream);
+/* Per-cell form of NormalizedError (rosenbrock.inl:353-402):
+ * error[c] = max(sqrt(sum_i (e_i / (atol_i + rtol * max(|y_i|, |ynew_i|)))^2 / N), 1e-10).
+ * absolute_tolerances is a plain [n_species] device array; error is [n_cells]. */
+int micm_hip_normalized_error(const micm_hip_mechanism* mechanism, int64_t n_cells,
+                              const double* variables, const double* new_variables,
+                              const double* errors, const double* absolute_tolerances,
+                              double relative_tolerance, double* error, void* stream);
+
+/* ------------------------------------------------------------ state */
+/* Solver::GetState (solver.hpp:174-195): device arrays for n_cells plus
+ * integrator scratch; tolerances start at the mechanism's defaults with
+ * rtol 1e-6 (state.hpp:42). */
+int micm_hip_state_create(const micm_hip_mechanism* mechanism, int64_t n_cells, micm_hip_state** out);
+void micm_hip_state_destroy(micm_hip_state* state);
+int micm_hip_state_get_pointers(const micm_hip_state* state, micm_hip_state_pointers* out);
+/* Host transfers use cell-major host arrays [n_cells][n_values] (state order). */
+int micm_hip_state_set_variables(micm_hip_state* state, const double* host);
+int micm_hip_state_get_variables(const micm_hip_state* state, double* host);
+int micm_hip_state_set_conditions(micm_hip_state* state, const double* host); /* [n_cells][3] */
+int micm_hip_state_set_custom_parameters(micm_hip_state* state, const double* host);
+int micm_hip_state_get_rate_constants(const micm_hip_state* state, double* host);
+int micm_hip_state_set_rate_constants(micm_hip_state* state, const double* host);
+int micm_hip_state_set_tolerances(micm_hip_state* state, double relative_tolerance,
+                                  const double* absolute_tolerances /* [n_species] or NULL */);
+/* Per-cell result of the last solve: states [n_cells], stats [n_cells][7], final_time [n_cells]; any may be NULL. */
+int micm_hip_state_get_results(const micm_hip_state* state, int32_t* states, int64_t* stats, double* final_time);
+/* Stream used by the state's solves (hipStream_t; NULL = default stream). */
+int micm_hip_state_set_stream(micm_hip_state* state, void* stream);
+
+/* Solver::UpdateStateParameters (solver.hpp:217-247) on the device. */
+int micm_hip_update_state_parameters(const micm_hip_mechanism* mechanism, micm_hip_state* state);
+
+/* Solver::Solve (solver.hpp:133-142): integrate every cell over time_step with
+ * the given form, then clamp negatives to zero (solver.hpp:275-297).  Blocks
+ * until the device has finished; per-cell outcomes via micm_hip_state_get_results.
+ * The returned status only reports API/device failures. */
+int micm_hip_solve(const micm_hip_mechanism* mechanism, micm_hip_state* state, double time_step,
+                   const micm_hip_rosenbrock_parameters* parameters, int32_t form);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MICM_HIP_H */

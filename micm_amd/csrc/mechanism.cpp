@@ -1,0 +1,592 @@
+// mechanism.cpp -- symbolic compilation of a mechanism (host side).
+//
+//   reaction sort by rate-constant type      include/micm/solver/solver_builder.inl:274-329
+//   species map, diagonal Markowitz order    solver_builder.inl:78-119, include/micm/solver/linear_solver.inl:29-124
+//   ProcessSet reactant / product tables     include/micm/process/process_set.hpp:256-422
+//   Jacobian non-zeros and flat ids          process_set.hpp:425-492, include/micm/util/jacobian.hpp:15-32
+//   Doolittle fill-in and schedule           include/micm/solver/lu_decomposition_doolittle.inl:69-299
+//   triangular-solve tables                  linear_solver.inl:166-224
+//   rate-constant store                      include/micm/process/reaction_rate_store.hpp:221-368
+#include "mechanism.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <stdexcept>
+
+namespace micm_hip
+{
+  namespace
+  {
+    constexpr double kBoltzmann = 1.380649e-23;
+    constexpr double kAvogadro = 6.02214076e23;
+    constexpr double kGasConstant = kBoltzmann * kAvogadro;  // include/micm/util/constants.hpp:9-11
+    constexpr double kPi = 3.14159265358979323846;
+
+    bool has(const std::vector<int32_t>& sorted, int32_t x)
+    {
+      return std::binary_search(sorted.begin(), sorted.end(), x);
+    }
+
+    void insert_sorted(std::vector<int32_t>& v, int32_t x)
+    {
+      auto it = std::lower_bound(v.begin(), v.end(), x);
+      if (it == v.end() || *it != x)
+        v.insert(it, x);
+    }
+
+    void erase_sorted(std::vector<int32_t>& v, int32_t x)
+    {
+      auto it = std::lower_bound(v.begin(), v.end(), x);
+      if (it != v.end() && *it == x)
+        v.erase(it);
+    }
+
+    // Minimum-Markowitz-cost pivot order on the directed graph of the non-zero
+    // pattern (edge v->c iff entry (v, c) is non-zero).  The cost
+    // (row_degree - 1) * (column_degree - 1) is evaluated in unsigned 64-bit
+    // arithmetic and ties go to the lowest index, as in the reference.
+    // Returns perm[new] = old.
+    std::vector<int64_t> markowitz_order(int32_t n, const std::vector<std::pair<int32_t, int32_t>>& edges)
+    {
+      std::vector<std::vector<int32_t>> out(n), in(n);
+      for (auto [r, c] : edges)
+      {
+        insert_sorted(out[r], c);
+        insert_sorted(in[c], r);
+      }
+      std::vector<uint64_t> rdeg(n), cdeg(n);
+      for (int32_t v = 0; v < n; ++v)
+      {
+        rdeg[v] = out[v].size();
+        cdeg[v] = in[v].size();
+      }
+      std::vector<char> alive(n, 1);
+      std::vector<int64_t> perm;
+      perm.reserve(n);
+      std::vector<int32_t> cols, rows;
+      for (int32_t step = 0; step < n; ++step)
+      {
+        int32_t pivot = -1;
+        uint64_t best = 0;
+        for (int32_t v = 0; v < n; ++v)
+        {
+          if (!alive[v])
+            continue;
+          const uint64_t cost = (rdeg[v] - 1u) * (cdeg[v] - 1u);
+          if (pivot < 0 || cost < best)
+          {
+            best = cost;
+            pivot = v;
+          }
+        }
+        perm.push_back(pivot);
+        alive[pivot] = 0;
+        cols.clear();
+        rows.clear();
+        for (int32_t c : out[pivot])
+          if (c != pivot && alive[c])
+            cols.push_back(c);
+        for (int32_t r : in[pivot])
+          if (r != pivot && alive[r])
+            rows.push_back(r);
+        for (int32_t r : rows)
+          for (int32_t c : cols)
+            if (!has(out[r], c))
+            {
+              insert_sorted(out[r], c);
+              insert_sorted(in[c], r);
+              ++rdeg[r];
+              ++cdeg[c];
+            }
+        for (int32_t c : cols)
+        {
+          erase_sorted(in[c], pivot);
+          --cdeg[c];
+        }
+        for (int32_t r : rows)
+        {
+          erase_sorted(out[r], pivot);
+          --rdeg[r];
+        }
+      }
+      return perm;
+    }
+
+    // CSR position of (row, col) in a sorted-row pattern.
+    int32_t position(const std::vector<int32_t>& row_ptr, const std::vector<int32_t>& col, int32_t r, int32_t c)
+    {
+      auto b = col.begin() + row_ptr[r], e = col.begin() + row_ptr[r + 1];
+      auto it = std::lower_bound(b, e, c);
+      if (it == e || *it != c)
+        return -1;
+      return static_cast<int32_t>(it - col.begin());
+    }
+
+    void to_csr(const std::vector<std::vector<int32_t>>& rows, std::vector<int32_t>& ptr, std::vector<int32_t>& col)
+    {
+      ptr.assign(1, 0);
+      col.clear();
+      for (const auto& r : rows)
+      {
+        col.insert(col.end(), r.begin(), r.end());
+        ptr.push_back(static_cast<int32_t>(col.size()));
+      }
+    }
+  }  // namespace
+
+  MechanismTables compile_mechanism(const micm_hip_mechanism_desc& d)
+  {
+    if (d.n_species <= 0)
+      throw std::invalid_argument("Provided chemical system contains no species.");
+    if (d.n_reactions < 0 || (d.n_reactions > 0 && (!d.reactant_offsets || !d.product_offsets || !d.rate_types ||
+                                                    !d.rate_params)))
+      throw std::invalid_argument("incomplete mechanism description");
+    if (d.n_species > (1 << 20) || d.n_reactions > (1 << 24))
+      throw std::invalid_argument("mechanism too large");
+    const int32_t n = static_cast<int32_t>(d.n_species);
+    const int64_t R = d.n_reactions;
+    MechanismTables t;
+    t.n_species = n;
+    t.n_reactions = R;
+
+    // ---- validate species references
+    for (int64_t r = 0; r < R; ++r)
+    {
+      if (d.rate_types[r] < 0 || d.rate_types[r] >= MICM_HIP_RATE_N_TYPES)
+        throw std::invalid_argument("unknown rate-constant type");
+      for (int64_t i = d.reactant_offsets[r]; i < d.reactant_offsets[r + 1]; ++i)
+        if (d.reactants[i] >= n || -d.reactants[i] - 1 >= d.n_parameterized)
+          throw std::domain_error("reactant does not exist");
+      for (int64_t i = d.product_offsets[r]; i < d.product_offsets[r + 1]; ++i)
+        if (d.products[i] >= n || -d.products[i] - 1 >= d.n_parameterized)
+          throw std::domain_error("product does not exist");
+    }
+    if (!d.ignore_unused_species)
+    {
+      std::vector<char> used(n, 0);
+      for (int64_t r = 0; r < R; ++r)
+      {
+        for (int64_t i = d.reactant_offsets[r]; i < d.reactant_offsets[r + 1]; ++i)
+          if (d.reactants[i] >= 0)
+            used[d.reactants[i]] = 1;
+        for (int64_t i = d.product_offsets[r]; i < d.product_offsets[r + 1]; ++i)
+          if (d.products[i] >= 0)
+            used[d.products[i]] = 1;
+      }
+      for (int32_t s = 0; s < n; ++s)
+        if (!used[s])
+          throw std::domain_error("Unused species in chemical system");
+    }
+
+    // ---- reactions stable-sorted by type
+    t.sorted_to_declared.resize(R);
+    for (int64_t r = 0; r < R; ++r)
+      t.sorted_to_declared[r] = r;
+    std::stable_sort(t.sorted_to_declared.begin(), t.sorted_to_declared.end(),
+                     [&](int64_t a, int64_t b) { return d.rate_types[a] < d.rate_types[b]; });
+
+    // ---- Jacobian non-zeros in declared indices, then the species order
+    std::vector<std::pair<int32_t, int32_t>> nz;
+    for (int64_t r = 0; r < R; ++r)
+    {
+      for (int64_t i = d.reactant_offsets[r]; i < d.reactant_offsets[r + 1]; ++i)
+      {
+        const int64_t ind = d.reactants[i];
+        if (ind < 0)
+          continue;
+        for (int64_t j = d.reactant_offsets[r]; j < d.reactant_offsets[r + 1]; ++j)
+          if (d.reactants[j] >= 0)
+            nz.emplace_back(static_cast<int32_t>(d.reactants[j]), static_cast<int32_t>(ind));
+        for (int64_t j = d.product_offsets[r]; j < d.product_offsets[r + 1]; ++j)
+          if (d.products[j] >= 0)
+            nz.emplace_back(static_cast<int32_t>(d.products[j]), static_cast<int32_t>(ind));
+      }
+    }
+    std::sort(nz.begin(), nz.end());
+    nz.erase(std::unique(nz.begin(), nz.end()), nz.end());
+    if (d.reorder_state)
+      t.state_to_declared = markowitz_order(n, nz);
+    else
+    {
+      t.state_to_declared.resize(n);
+      for (int32_t i = 0; i < n; ++i)
+        t.state_to_declared[i] = i;
+    }
+    std::vector<int32_t> to_state(n);
+    for (int32_t i = 0; i < n; ++i)
+      to_state[t.state_to_declared[i]] = i;
+    t.absolute_tolerances.assign(n, 1e-3);
+    if (d.absolute_tolerances)
+      for (int32_t i = 0; i < n; ++i)
+        t.absolute_tolerances[i] = d.absolute_tolerances[t.state_to_declared[i]];
+
+    std::vector<std::vector<int32_t>> arow(n);
+    for (auto [r, c] : nz)
+      arow[to_state[r]].push_back(to_state[c]);
+    for (int32_t i = 0; i < n; ++i)
+    {
+      arow[i].push_back(i);
+      std::sort(arow[i].begin(), arow[i].end());
+      arow[i].erase(std::unique(arow[i].begin(), arow[i].end()), arow[i].end());
+    }
+    to_csr(arow, t.jac_row_ptr, t.jac_col);
+
+    // ---- rate-constant store and ProcessSet tables (sorted reactions, state indices)
+    t.multiplier_offsets.push_back(0);
+    t.reactant_offsets.push_back(0);
+    t.product_offsets.push_back(0);
+    int32_t custom = 0;
+    for (int64_t k = 0; k < R; ++k)
+    {
+      const int64_t r = t.sorted_to_declared[k];
+      const int32_t type = d.rate_types[r];
+      t.rate_type.push_back(type);
+      std::array<double, MICM_HIP_RATE_PARAM_STRIDE> p{};
+      for (int j = 0; j < MICM_HIP_RATE_PARAM_STRIDE; ++j)
+        p[j] = d.rate_params[r * MICM_HIP_RATE_PARAM_STRIDE + j];
+      if (type == MICM_HIP_RATE_BRANCHED)
+      {
+        const double k0 = 2.0e-22 * kAvogadro * 1.0e-6 * std::exp(static_cast<double>(static_cast<int64_t>(p[4])));
+        const double air_ref = 2.45e19 / kAvogadro * 1.0e6;
+        const double a = k0 * air_ref;
+        const double b = 0.43 * std::pow(293.0 / 298.0, -8.0);
+        const double A_val = a / (1.0 + a / b) * std::pow(0.41, 1.0 / (1.0 + std::pow(std::log10(a / b), 2.0)));
+        p[5] = k0;
+        p[6] = A_val * (1.0 - p[3]) / p[3];
+      }
+      if (type == MICM_HIP_RATE_SURFACE)
+        p[3] = 8.0 * kGasConstant / (kPi * p[1]);
+      if (type == MICM_HIP_RATE_TAYLOR && (p[5] < 0 || p[5] > 16))
+        throw std::invalid_argument("Taylor series rate constants take at most 16 coefficients");
+      t.rate_params.insert(t.rate_params.end(), p.begin(), p.end());
+      if (type == MICM_HIP_RATE_USER_DEFINED)
+      {
+        t.custom_index.push_back(custom);
+        t.custom_owner.push_back(k);
+        custom += 1;
+      }
+      else if (type == MICM_HIP_RATE_SURFACE)
+      {
+        t.custom_index.push_back(custom);
+        t.custom_owner.push_back(k);
+        t.custom_owner.push_back(k);
+        custom += 2;
+      }
+      else
+        t.custom_index.push_back(-1);
+      for (int64_t i = d.reactant_offsets[r]; i < d.reactant_offsets[r + 1]; ++i)
+      {
+        const int64_t id = d.reactants[i];
+        if (id >= 0)
+          t.reactant_ids.push_back(to_state[id]);
+        else
+          for (int j = 0; j < 4; ++j)
+            t.multipliers.push_back(d.parameterized[4 * (-id - 1) + j]);
+      }
+      t.reactant_offsets.push_back(static_cast<int32_t>(t.reactant_ids.size()));
+      t.multiplier_offsets.push_back(static_cast<int32_t>(t.multipliers.size() / 4));
+      for (int64_t i = d.product_offsets[r]; i < d.product_offsets[r + 1]; ++i)
+        if (d.products[i] >= 0)
+        {
+          t.product_ids.push_back(to_state[d.products[i]]);
+          t.yields.push_back(d.yields[i]);
+        }
+      t.product_offsets.push_back(static_cast<int32_t>(t.product_ids.size()));
+    }
+    t.n_custom = custom;
+
+    // ---- Jacobian terms: ordered by independent species, then by process (stable)
+    std::vector<std::pair<int32_t, int32_t>> columns;  // (independent, process)
+    for (int32_t k = 0; k < (int32_t)R; ++k)
+      for (int32_t i = t.reactant_offsets[k]; i < t.reactant_offsets[k + 1]; ++i)
+        columns.emplace_back(t.reactant_ids[i], k);
+    std::stable_sort(columns.begin(), columns.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    t.term_dep_offsets.push_back(0);
+    t.term_flat_offsets.push_back(0);
+    for (auto [ind, k] : columns)
+    {
+      t.term_process.push_back(k);
+      bool skipped = false;
+      for (int32_t i = t.reactant_offsets[k]; i < t.reactant_offsets[k + 1]; ++i)
+      {
+        const int32_t s = t.reactant_ids[i];
+        if (s == ind && !skipped)
+        {
+          skipped = true;
+          continue;
+        }
+        t.term_dep_ids.push_back(s);
+        t.term_flat.push_back(position(t.jac_row_ptr, t.jac_col, s, ind));
+      }
+      t.term_dep_offsets.push_back(static_cast<int32_t>(t.term_dep_ids.size()));
+      t.term_flat.push_back(position(t.jac_row_ptr, t.jac_col, ind, ind));
+      for (int32_t i = t.product_offsets[k]; i < t.product_offsets[k + 1]; ++i)
+        t.term_flat.push_back(position(t.jac_row_ptr, t.jac_col, t.product_ids[i], ind));
+      t.term_flat_offsets.push_back(static_cast<int32_t>(t.term_flat.size()));
+    }
+
+    // ---- Doolittle fill-in, one row of U and one column of L at a time
+    std::vector<std::vector<int32_t>> acol(n);
+    for (int32_t i = 0; i < n; ++i)
+      for (int32_t c : arow[i])
+        acol[c].push_back(i);
+    std::vector<std::vector<int32_t>> urow(n), ucol(n), lrow(n), lcol(n);  // strictly-lower L in lrow/lcol
+    std::vector<char> mark(n, 0);
+    std::vector<int32_t> touched;
+    for (int32_t i = 0; i < n; ++i)
+    {
+      touched.clear();
+      auto add = [&](int32_t k)
+      {
+        if (!mark[k])
+        {
+          mark[k] = 1;
+          touched.push_back(k);
+        }
+      };
+      add(i);
+      for (int32_t k : arow[i])
+        if (k >= i)
+          add(k);
+      for (int32_t j : lrow[i])
+        for (int32_t k : urow[j])
+          if (k >= i)
+            add(k);
+      std::sort(touched.begin(), touched.end());
+      for (int32_t k : touched)
+      {
+        urow[i].push_back(k);
+        ucol[k].push_back(i);
+        mark[k] = 0;
+      }
+      touched.clear();
+      for (int32_t k : acol[i])
+        if (k > i)
+          add(k);
+      for (int32_t j : ucol[i])
+        if (j < i)
+          for (int32_t k : lcol[j])
+            if (k > i)
+              add(k);
+      std::sort(touched.begin(), touched.end());
+      for (int32_t k : touched)
+      {
+        lcol[i].push_back(k);
+        lrow[k].push_back(i);
+        mark[k] = 0;
+      }
+    }
+    std::vector<std::vector<int32_t>> lrow_full(n);
+    for (int32_t i = 0; i < n; ++i)
+    {
+      lrow_full[i] = lrow[i];
+      lrow_full[i].push_back(i);
+    }
+    to_csr(lrow_full, t.lower_row_ptr, t.lower_col);
+    to_csr(urow, t.upper_row_ptr, t.upper_col);
+    for (int32_t i = 0; i < n; ++i)
+    {
+      t.jac_diag.push_back(position(t.jac_row_ptr, t.jac_col, i, i));
+      t.lower_diag.push_back(position(t.lower_row_ptr, t.lower_col, i, i));
+      t.upper_diag.push_back(position(t.upper_row_ptr, t.upper_col, i, i));
+    }
+
+    // ---- Doolittle schedule
+    t.lu_term_offsets.push_back(0);
+    for (int32_t i = 0; i < n; ++i)
+    {
+      for (int32_t k : urow[i])
+      {
+        for (int32_t j : lrow[i])
+          if (has(urow[j], k))
+          {
+            t.lu_term_l.push_back(position(t.lower_row_ptr, t.lower_col, i, j));
+            t.lu_term_u.push_back(position(t.upper_row_ptr, t.upper_col, j, k));
+          }
+        t.lu_out.push_back(position(t.upper_row_ptr, t.upper_col, i, k));
+        t.lu_a.push_back(position(t.jac_row_ptr, t.jac_col, i, k));
+        t.lu_div.push_back(-1);
+        t.lu_term_offsets.push_back(static_cast<int32_t>(t.lu_term_l.size()));
+      }
+      for (int32_t k : lcol[i])
+      {
+        for (int32_t j : lrow[k])
+        {
+          if (j >= i)
+            break;
+          if (has(urow[j], i))
+          {
+            t.lu_term_l.push_back(position(t.lower_row_ptr, t.lower_col, k, j));
+            t.lu_term_u.push_back(position(t.upper_row_ptr, t.upper_col, j, i));
+          }
+        }
+        t.lu_out.push_back(position(t.lower_row_ptr, t.lower_col, k, i));
+        t.lu_a.push_back(position(t.jac_row_ptr, t.jac_col, k, i));
+        t.lu_div.push_back(t.upper_diag[i]);
+        t.lu_term_offsets.push_back(static_cast<int32_t>(t.lu_term_l.size()));
+      }
+    }
+
+    // ---- substitution tables
+    t.fwd_offsets.push_back(0);
+    t.bwd_offsets.push_back(0);
+    for (int32_t i = 0; i < n; ++i)
+    {
+      for (int32_t j : lrow[i])
+      {
+        t.fwd_mat.push_back(position(t.lower_row_ptr, t.lower_col, i, j));
+        t.fwd_col.push_back(j);
+      }
+      t.fwd_offsets.push_back(static_cast<int32_t>(t.fwd_mat.size()));
+      for (int32_t j : urow[i])
+        if (j > i)
+        {
+          t.bwd_mat.push_back(position(t.upper_row_ptr, t.upper_col, i, j));
+          t.bwd_col.push_back(j);
+        }
+      t.bwd_offsets.push_back(static_cast<int32_t>(t.bwd_mat.size()));
+    }
+    return t;
+  }
+
+  bool rosenbrock_preset(int32_t preset, micm_hip_rosenbrock_parameters& p)
+  {
+    p = micm_hip_rosenbrock_parameters{};
+    p.max_number_of_steps = 1000;
+    p.round_off = std::numeric_limits<double>::epsilon();
+    p.factor_min = 0.2;
+    p.factor_max = 6;
+    p.rejection_factor_decrease = 0.1;
+    p.safety_factor = 0.9;
+    // Tableaus: Hairer & Wanner, Solving ODEs II (2-, 4- and 6-stage), Sandu et al.,
+    // Atmos. Environ. 31 (1997) 3459 (3-stage); values as rosenbrock_solver_parameters.hpp:152-433.
+    switch (preset)
+    {
+      case MICM_HIP_ROS_TWO_STAGE:
+      {
+        const double g = 1.0 + 1.0 / 1.41421356237309504880;
+        p.stages = 2;
+        p.a[0] = 1.0 / g;
+        p.c[0] = -2.0 / g;
+        p.new_function_evaluation[0] = p.new_function_evaluation[1] = 1;
+        p.m[0] = 3.0 / (2.0 * g);
+        p.m[1] = 1.0 / (2.0 * g);
+        p.e[0] = 1.0 / (2.0 * g);
+        p.e[1] = 1.0 / (2.0 * g);
+        p.estimator_of_local_order = 2.0;
+        p.alpha[1] = 1.0;
+        p.gamma[0] = g;
+        p.gamma[1] = -g;
+        return true;
+      }
+      case MICM_HIP_ROS_THREE_STAGE:
+        p.stages = 3;
+        p.a[0] = 1;
+        p.a[1] = 1;
+        p.c[0] = -0.10156171083877702091975600115545e+01;
+        p.c[1] = 0.40759956452537699824805835358067e+01;
+        p.c[2] = 0.92076794298330791242156818474003e+01;
+        p.new_function_evaluation[0] = p.new_function_evaluation[1] = 1;
+        p.m[0] = 0.1e+01;
+        p.m[1] = 0.61697947043828245592553615689730e+01;
+        p.m[2] = -0.42772256543218573326238373806514;
+        p.e[0] = 0.5;
+        p.e[1] = -0.29079558716805469821718236208017e+01;
+        p.e[2] = 0.22354069897811569627360909276199;
+        p.estimator_of_local_order = 3;
+        p.alpha[1] = p.alpha[2] = 0.43586652150845899941601945119356;
+        p.gamma[0] = 0.43586652150845899941601945119356;
+        p.gamma[1] = 0.24291996454816804366592249683314;
+        p.gamma[2] = 0.21851380027664058511513169485832e+01;
+        return true;
+      case MICM_HIP_ROS_FOUR_STAGE:
+        p.stages = 4;
+        p.a[0] = 0.2000000000000000E+01;
+        p.a[1] = 0.1867943637803922E+01;
+        p.a[2] = 0.2344449711399156;
+        p.a[3] = p.a[1];
+        p.a[4] = p.a[2];
+        p.c[0] = -0.7137615036412310E+01;
+        p.c[1] = 0.2580708087951457E+01;
+        p.c[2] = 0.6515950076447975;
+        p.c[3] = -0.2137148994382534E+01;
+        p.c[4] = -0.3214669691237626;
+        p.c[5] = -0.6949742501781779;
+        p.new_function_evaluation[0] = p.new_function_evaluation[1] = p.new_function_evaluation[2] = 1;
+        p.m[0] = 0.2255570073418735E+01;
+        p.m[1] = 0.2870493262186792;
+        p.m[2] = 0.4353179431840180;
+        p.m[3] = 0.1093502252409163E+01;
+        p.e[0] = -0.2815431932141155;
+        p.e[1] = -0.7276199124938920E-01;
+        p.e[2] = -0.1082196201495311;
+        p.e[3] = -0.1093502252409163E+01;
+        p.estimator_of_local_order = 4.0;
+        p.alpha[1] = 0.1145640000000000E+01;
+        p.alpha[2] = p.alpha[3] = 0.6552168638155900;
+        p.gamma[0] = 0.5728200000000000;
+        p.gamma[1] = -0.1769193891319233E+01;
+        p.gamma[2] = 0.7592633437920482;
+        p.gamma[3] = -0.1049021087100450;
+        return true;
+      case MICM_HIP_ROS_FOUR_STAGE_DA:
+        p.stages = 4;
+        p.a[1] = 2.0;
+        p.a[3] = 2.0;
+        p.a[5] = 1.0;
+        p.c[0] = 4.0;
+        p.c[1] = 1.0;
+        p.c[2] = -1.0;
+        p.c[3] = 1.0;
+        p.c[4] = -1.0;
+        p.c[5] = -(8.0 / 3.0);
+        p.new_function_evaluation[0] = p.new_function_evaluation[2] = p.new_function_evaluation[3] = 1;
+        p.m[0] = 2.0;
+        p.m[2] = 1.0;
+        p.m[3] = 1.0;
+        p.e[3] = 1.0;
+        p.estimator_of_local_order = 3.0;
+        p.alpha[2] = p.alpha[3] = 1.0;
+        p.gamma[0] = 0.5;
+        p.gamma[1] = 1.5;
+        return true;
+      case MICM_HIP_ROS_SIX_STAGE_DA:
+      {
+        p.stages = 6;
+        const double al[6] = { 0.000, 0.386, 0.210, 0.630, 1.000, 1.000 };
+        const double ga[6] = { 0.2500000000000000, -0.1043000000000000, 0.1035000000000000, -0.3620000000000023E-01, 0.0, 0.0 };
+        const double a[15] = { 0.1544000000000000E+01, 0.9466785280815826, 0.2557011698983284, 0.3314825187068521E+01,
+                               0.2896124015972201E+01, 0.9986419139977817, 0.1221224509226641E+01, 0.6019134481288629E+01,
+                               0.1253708332932087E+02, -0.6878860361058950, 0.1221224509226641E+01, 0.6019134481288629E+01,
+                               0.1253708332932087E+02, -0.6878860361058950, 1.0 };
+        const double c[15] = { -0.5668800000000000E+01, -0.2430093356833875E+01, -0.2063599157091915,
+                               -0.1073529058151375,     -0.9594562251023355E+01, -0.2047028614809616E+02,
+                               0.7496443313967647E+01,  -0.1024680431464352E+02, -0.3399990352819905E+02,
+                               0.1170890893206160E+02,  0.8083246795921522E+01,  -0.7981132988064893E+01,
+                               -0.3152159432874371E+02, 0.1631930543123136E+02,  -0.6058818238834054E+01 };
+        for (int i = 0; i < 6; ++i)
+        {
+          p.alpha[i] = al[i];
+          p.gamma[i] = ga[i];
+          p.new_function_evaluation[i] = 1;
+        }
+        for (int i = 0; i < 15; ++i)
+        {
+          p.a[i] = a[i];
+          p.c[i] = c[i];
+        }
+        p.m[0] = a[6];
+        p.m[1] = a[7];
+        p.m[2] = a[8];
+        p.m[3] = a[9];
+        p.m[4] = 1.0;
+        p.m[5] = 1.0;
+        p.e[5] = 1.0;
+        p.estimator_of_local_order = 4.0;
+        return true;
+      }
+    }
+    return false;
+  }
+}  // namespace micm_hip

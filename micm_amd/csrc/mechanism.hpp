@@ -1,0 +1,76 @@
+// mechanism.hpp -- host-side compilation of a mechanism into the index tables
+// the HIP kernels consume.  Replaces the symbolic half of the reference's
+// SolverBuilder::Build (include/micm/solver/solver_builder.inl:227-548).
+#pragma once
+
+#include "../../include/micm_hip.h"
+
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace micm_hip
+{
+  // Rate parameters as consumed by the device (per sorted reaction, stride
+  // MICM_HIP_RATE_PARAM_STRIDE).  Derived values live in spare slots:
+  //   BRANCHED  slot 5 = k0, slot 6 = z   (reaction_rate_store.hpp:286-293)
+  //   SURFACE   slot 3 = mean free speed factor 8R/(pi*M) (reaction_rate_store.hpp:328)
+  struct MechanismTables
+  {
+    int64_t n_species = 0, n_reactions = 0, n_custom = 0;
+    std::vector<int64_t> state_to_declared;   // species permutation
+    std::vector<int64_t> sorted_to_declared;  // reaction permutation
+    std::vector<double> absolute_tolerances;  // state order
+
+    // ---- rate constants (sorted reactions)
+    std::vector<int32_t> rate_type;
+    std::vector<double> rate_params;          // n_reactions * stride
+    std::vector<int32_t> custom_index;        // first custom parameter, -1 if none
+    std::vector<int32_t> multiplier_offsets;  // n_reactions + 1
+    std::vector<double> multipliers;          // 4 per parameterized reactant: c0, cT, cP, crho
+    std::vector<int64_t> custom_owner;        // custom parameter -> sorted reaction
+
+    // ---- forcing (ProcessSet, process_set.hpp:280-317)
+    std::vector<int32_t> reactant_offsets, reactant_ids;
+    std::vector<int32_t> product_offsets, product_ids;
+    std::vector<double> yields;
+
+    // ---- Jacobian terms (process_set.hpp:329-391, flat ids :460-492)
+    std::vector<int32_t> term_process;        // one per (independent reactant occurrence, process)
+    std::vector<int32_t> term_dep_offsets;    // dependent reactants of each term
+    std::vector<int32_t> term_dep_ids;
+    std::vector<int32_t> term_flat_offsets;   // deps..., independent, products...
+    std::vector<int32_t> term_flat;
+
+    // ---- sparsity patterns (CSR, row-major sorted)
+    std::vector<int32_t> jac_row_ptr, jac_col;   // with the diagonal (jacobian.hpp:27-30)
+    std::vector<int32_t> lower_row_ptr, lower_col;
+    std::vector<int32_t> upper_row_ptr, upper_col;
+    std::vector<int32_t> jac_diag, lower_diag, upper_diag;
+
+    // ---- Doolittle schedule (lu_decomposition_doolittle.inl:177-299): per output element
+    // in the reference's order (row i: U[i][k] for k ascending, then L[k][i] for k ascending).
+    std::vector<int32_t> lu_out;          // storage index in U (div < 0) or L (div >= 0)
+    std::vector<int32_t> lu_a;            // storage index in the Jacobian, -1 if structurally zero
+    std::vector<int32_t> lu_div;          // U[i][i] storage index for L elements, -1 for U elements
+    std::vector<int32_t> lu_term_offsets; // n_elements + 1
+    std::vector<int32_t> lu_term_l;       // L[i][j] (or L[k][j]) storage index, j ascending
+    std::vector<int32_t> lu_term_u;       // U[j][k] (or U[j][i]) storage index
+
+    // ---- substitution (linear_solver.inl:166-224)
+    std::vector<int32_t> fwd_offsets, fwd_mat, fwd_col;   // row i: L[i][j], j ascending
+    std::vector<int32_t> bwd_offsets, bwd_mat, bwd_col;   // row i (ascending storage): U[i][j], j ascending
+
+    int64_t nnz_jacobian() const { return (int64_t)jac_col.size(); }
+    int64_t nnz_lower() const { return (int64_t)lower_col.size(); }
+    int64_t nnz_upper() const { return (int64_t)upper_col.size(); }
+    int64_t lu_multiply_adds() const { return (int64_t)lu_term_l.size(); }
+  };
+
+  // Throws std::invalid_argument (bad description) or std::domain_error (species errors).
+  MechanismTables compile_mechanism(const micm_hip_mechanism_desc& desc);
+
+  // RosenbrockSolverParameters presets (rosenbrock_solver_parameters.hpp:152-433).
+  bool rosenbrock_preset(int32_t preset, micm_hip_rosenbrock_parameters& out);
+}  // namespace micm_hip

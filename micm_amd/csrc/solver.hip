@@ -1,0 +1,1309 @@
+// solver.hip -- HIP kernels of the Rosenbrock path and the C-ABI of include/micm_hip.h.
+//
+// Layout: one lane = one grid cell; per-cell arrays in the grouped layout
+// ((cell / 64) * n_values + v) * 64 + cell % 64.  Kernels launch 256-thread
+// workgroups (4 wavefronts = 4 cell groups).
+//
+// Integration forms (micm_hip_solve):
+//   LOCKSTEP   -- every cell of the state shares one step size, as the reference's
+//                 multi-cell State (rosenbrock.inl:12-304); the normalized error is
+//                 the reference's single running sum over cells and species.
+//   PLAIN      -- per-cell step size; one launch per kernel per stage, host loop.
+//   PERSISTENT -- per-cell step size; one persistent launch whose wavefronts take
+//                 cells from a work queue and carry each to the end of its step.
+#include "device.hpp"
+#include "mechanism.hpp"
+#include "path.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace micm_hip;
+
+namespace
+{
+  thread_local std::string g_last_error = "";
+
+  int fail(int code, const std::string& msg)
+  {
+    g_last_error = msg;
+    return code;
+  }
+
+#define HIP_TRY(expr)                                                                                     \
+  do                                                                                                      \
+  {                                                                                                       \
+    hipError_t _e = (expr);                                                                               \
+    if (_e != hipSuccess)                                                                                 \
+      return fail(MICM_HIP_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));                \
+  } while (0)
+
+  constexpr int kBlock = 256;
+
+  inline int64_t n_groups_of(int64_t n_cells)
+  {
+    return (n_cells + kGroup - 1) / kGroup;
+  }
+
+  inline unsigned grid_for(int64_t n_cells)
+  {
+    return static_cast<unsigned>((n_groups_of(n_cells) * kGroup + kBlock - 1) / kBlock);
+  }
+
+  // ------------------------------------------------------------------ kernel-level entry points
+
+  __global__ void __launch_bounds__(kBlock) k_rate_constants(DevMech m, int64_t n_cells, const double* __restrict__ cond,
+                                                             const double* __restrict__ custom, double* __restrict__ k)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n_cells)
+      return;
+    const size_t g = c / kGroup;
+    const int l = c % kGroup;
+    Lane cd = lane_view(const_cast<double*>(cond), g, 3, l);
+    Lane cu = lane_view(const_cast<double*>(custom), g, m.n_custom, l);
+    Lane kk = lane_view(k, g, m.n_reactions, l);
+    const double T = cd[0], P = cd[1], rho = cd[2];
+    for (int r = 0; r < m.n_reactions; ++r)
+      kk[r] = rate_constant(m, r, T, P, rho, cu);
+  }
+
+  __global__ void __launch_bounds__(kBlock) k_add_forcing(DevMech m, int64_t n_cells, const double* __restrict__ k,
+                                                          const double* __restrict__ y, double* __restrict__ f)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n_cells)
+      return;
+    const size_t g = c / kGroup;
+    const int l = c % kGroup;
+    add_forcing(m, lane_view(const_cast<double*>(k), g, m.n_reactions, l), lane_view(const_cast<double*>(y), g, m.n_species, l),
+                lane_view(f, g, m.n_species, l));
+  }
+
+  __global__ void __launch_bounds__(kBlock) k_subtract_jacobian(DevMech m, int64_t n_cells, const double* __restrict__ k,
+                                                                const double* __restrict__ y, double* __restrict__ jac)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n_cells)
+      return;
+    const size_t g = c / kGroup;
+    const int l = c % kGroup;
+    subtract_jacobian(m, lane_view(const_cast<double*>(k), g, m.n_reactions, l),
+                      lane_view(const_cast<double*>(y), g, m.n_species, l), lane_view(jac, g, m.nnz_jac, l));
+  }
+
+  __global__ void __launch_bounds__(kBlock) k_alpha_minus_jacobian(DevMech m, int64_t n_cells, double* __restrict__ jac,
+                                                                   double alpha)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n_cells)
+      return;
+    Lane J = lane_view(jac, c / kGroup, m.nnz_jac, c % kGroup);
+    for (int i = 0; i < m.n_species; ++i)
+      J[m.jac_diag[i]] += alpha * 1.0;
+  }
+
+  __global__ void __launch_bounds__(kBlock) k_lu_decompose(DevMech m, int64_t n_cells, const double* __restrict__ jac,
+                                                           double* __restrict__ lower, double* __restrict__ upper)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n_cells)
+      return;
+    const size_t g = c / kGroup;
+    const int l = c % kGroup;
+    lu_decompose(m, lane_view(const_cast<double*>(jac), g, m.nnz_jac, l), lane_view(lower, g, m.nnz_lower, l),
+                 lane_view(upper, g, m.nnz_upper, l));
+  }
+
+  __global__ void __launch_bounds__(kBlock) k_lu_solve(DevMech m, int64_t n_cells, const double* __restrict__ lower,
+                                                       const double* __restrict__ upper, double* __restrict__ x)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n_cells)
+      return;
+    const size_t g = c / kGroup;
+    const int l = c % kGroup;
+    lu_solve<false>(m, lane_view(const_cast<double*>(lower), g, m.nnz_lower, l),
+                    lane_view(const_cast<double*>(upper), g, m.nnz_upper, l), lane_view(x, g, m.n_species, l));
+  }
+
+  __global__ void __launch_bounds__(kBlock) k_normalized_error(int n_species, int64_t n_cells, const double* __restrict__ y,
+                                                               const double* __restrict__ ynew,
+                                                               const double* __restrict__ yerr,
+                                                               const double* __restrict__ atol, double rtol,
+                                                               double* __restrict__ err)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n_cells)
+      return;
+    const size_t g = c / kGroup;
+    const int l = c % kGroup;
+    const double s = error_sum(n_species, lane_view(const_cast<double*>(y), g, n_species, l),
+                               lane_view(const_cast<double*>(ynew), g, n_species, l),
+                               lane_view(const_cast<double*>(yerr), g, n_species, l), atol, rtol);
+    err[c] = smax(sqrt(s / static_cast<double>(n_species > 0 ? n_species : 1)), 1.0e-10);
+  }
+
+  // ------------------------------------------------------------------ solver state
+
+  struct Work
+  {
+    double* y;       // N
+    double* k;       // R
+    double* jac;     // nnzJ
+    double* lower;   // nnzL
+    double* upper;   // nnzU
+    double* f0;      // N
+    double* ynew;    // N
+    double* yerr;    // N
+    double* K;       // stage vectors, N each, stage j at K + j * k_stride
+    size_t k_stride;
+  };
+
+  // The stage vectors of one lane: stage j starts k_stride elements after stage j-1.
+  struct Stages
+  {
+    double* p;
+    size_t stride;
+    __device__ __forceinline__ Lane operator[](int j) const
+    {
+      return Lane{ p + static_cast<size_t>(j) * stride };
+    }
+  };
+
+  __device__ __forceinline__ Stages stage_view(const Work& w, size_t group, int n, int lane)
+  {
+    return Stages{ lane_view(w.K, group, n, lane).p, w.k_stride };
+  }
+
+  __device__ __forceinline__ void load_control(const CellControl* ctl, int64_t c, CellControl& out)
+  {
+    out = ctl[c];
+  }
+
+  // Stage s of one attempt for one lane (rosenbrock.inl:140-195).
+  __device__ inline void run_stage(const DevMech& m, const DevParams& p, int s, const Lane& y, const Lane& k,
+                                   const Lane& f0, const Lane& ynew, const Stages& K, const Lane& L, const Lane& U,
+                                   CellControl& c)
+  {
+    const int n = m.n_species;
+    const int sc = s * (s - 1) / 2;
+    const Lane Ks = K[s];
+    if (s == 0)
+    {
+      for (int i = 0; i < n; ++i)
+        Ks[i] = f0[i];
+    }
+    else if (p.new_f[s])
+    {
+      for (int i = 0; i < n; ++i)
+      {
+        double v = y[i];
+        for (int j = 0; j < s; ++j)
+          v += p.a[sc + j] * K[j][i];
+        ynew[i] = v;
+      }
+      for (int i = 0; i < n; ++i)
+        Ks[i] = 0.0;
+      add_forcing(m, k, ynew, Ks);
+      c.stats[MICM_HIP_STAT_FUNCTION_CALLS] += 1;
+    }
+    if (s + 1 < p.stages && !p.new_f[s + 1])
+    {
+      const Lane Kn = K[s + 1];
+      for (int i = 0; i < n; ++i)
+        Kn[i] = Ks[i];
+    }
+    if (s > 0)
+    {
+      double coef[6];
+      for (int j = 0; j < s; ++j)
+        coef[j] = p.c[sc + j] / c.H;
+      for (int i = 0; i < n; ++i)
+      {
+        double v = Ks[i];
+        for (int j = 0; j < s; ++j)
+          v += coef[j] * K[j][i];
+        Ks[i] = v;
+      }
+    }
+    lu_solve<true>(m, L, U, Ks);
+    c.stats[MICM_HIP_STAT_SOLVES] += 1;
+  }
+
+  // Ynew = Y + sum m_s K_s, Yerr = sum e_s K_s; returns the error sum over species.
+  __device__ inline double combine_and_error(const DevMech& m, const DevParams& p, const Lane& y, const Lane& ynew,
+                                             const Lane& yerr, const Stages& K, const double* atol)
+  {
+    const int n = m.n_species;
+    for (int i = 0; i < n; ++i)
+    {
+      double v = y[i], e = 0.0;
+      for (int s = 0; s < p.stages; ++s)
+        v += p.m[s] * K[s][i];
+      for (int s = 0; s < p.stages; ++s)
+        e += p.e[s] * K[s][i];
+      ynew[i] = v;
+      yerr[i] = e;
+    }
+    return error_sum(n, y, ynew, yerr, atol, p.rtol);
+  }
+
+  // Start of an attempt: a new step (forcing + Jacobian) if due, then alpha and LU.
+  // Returns false if the cell finished at the loop-top checks.
+  __device__ inline bool factor_attempt(const DevMech& m, const DevParams& p, const Lane& y, const Lane& k,
+                                        const Lane& f0, const Lane& J, const Lane& L, const Lane& U, CellControl& c)
+  {
+    if (c.flags & kNewStep)
+    {
+      if (!begin_step(p, c))
+        return false;
+      for (int i = 0; i < m.n_species; ++i)
+        f0[i] = 0.0;
+      add_forcing(m, k, y, f0);
+      c.stats[MICM_HIP_STAT_FUNCTION_CALLS] += 1;
+      for (int i = 0; i < m.nnz_jac; ++i)
+        J[i] = 0.0;
+      subtract_jacobian(m, k, y, J);
+      c.stats[MICM_HIP_STAT_JACOBIAN_UPDATES] += 1;
+      c.last_alpha = 0.0;
+      c.flags &= ~kNewStep;
+    }
+    const double alpha = next_alpha(p, c);
+    for (int i = 0; i < m.n_species; ++i)
+      J[m.jac_diag[i]] += alpha * 1.0;
+    lu_decompose(m, J, L, U);
+    c.stats[MICM_HIP_STAT_DECOMPOSITIONS] += 1;
+    return true;
+  }
+
+  // ---- plain form: one kernel per phase over all cells
+
+  __global__ void __launch_bounds__(kBlock) k_plain_init(DevParams p, int64_t n_cells, CellControl* ctl)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n_cells)
+      return;
+    CellControl cc;
+    control_init(p, cc);
+    ctl[c] = cc;
+  }
+
+  __global__ void __launch_bounds__(kBlock) k_plain_factor(DevMech m, DevParams p, int64_t n_cells, Work w, CellControl* ctl)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n_cells)
+      return;
+    CellControl cc = ctl[c];
+    if (cc.status != MICM_HIP_STATE_RUNNING)
+      return;
+    const size_t g = c / kGroup;
+    const int l = c % kGroup;
+    factor_attempt(m, p, lane_view(w.y, g, m.n_species, l), lane_view(w.k, g, m.n_reactions, l),
+                   lane_view(w.f0, g, m.n_species, l), lane_view(w.jac, g, m.nnz_jac, l),
+                   lane_view(w.lower, g, m.nnz_lower, l), lane_view(w.upper, g, m.nnz_upper, l), cc);
+    ctl[c] = cc;
+  }
+
+  __global__ void __launch_bounds__(kBlock) k_plain_stage(DevMech m, DevParams p, int64_t n_cells, Work w, CellControl* ctl,
+                                                          int s)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n_cells)
+      return;
+    CellControl cc = ctl[c];
+    if (cc.status != MICM_HIP_STATE_RUNNING)
+      return;
+    const size_t g = c / kGroup;
+    const int l = c % kGroup;
+    const int n = m.n_species;
+    const Stages K = stage_view(w, g, n, l);
+    run_stage(m, p, s, lane_view(w.y, g, n, l), lane_view(w.k, g, m.n_reactions, l), lane_view(w.f0, g, n, l),
+              lane_view(w.ynew, g, n, l), K, lane_view(w.lower, g, m.nnz_lower, l), lane_view(w.upper, g, m.nnz_upper, l),
+              cc);
+    ctl[c].stats[MICM_HIP_STAT_FUNCTION_CALLS] = cc.stats[MICM_HIP_STAT_FUNCTION_CALLS];
+    ctl[c].stats[MICM_HIP_STAT_SOLVES] = cc.stats[MICM_HIP_STAT_SOLVES];
+  }
+
+  // Combination, error and accept/reject.  lockstep: only the error terms are
+  // formed here; k_lockstep_error and k_lockstep_control finish the attempt.
+  __global__ void __launch_bounds__(kBlock) k_plain_finish(DevMech m, DevParams p, int64_t n_cells, Work w, CellControl* ctl,
+                                                           const double* __restrict__ atol, unsigned int* active,
+                                                           int lockstep)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    bool running = false;
+    if (c < n_cells)
+    {
+      CellControl cc = ctl[c];
+      if (cc.status == MICM_HIP_STATE_RUNNING)
+      {
+        const size_t g = c / kGroup;
+        const int l = c % kGroup;
+        const int n = m.n_species;
+        const Stages K = stage_view(w, g, n, l);
+        Lane y = lane_view(w.y, g, n, l), ynew = lane_view(w.ynew, g, n, l);
+        const double s = combine_and_error(m, p, y, ynew, lane_view(w.yerr, g, n, l), K, atol);
+        if (!lockstep)
+        {
+          const double error = smax(sqrt(s / static_cast<double>(n > 0 ? n : 1)), 1.0e-10);
+          if (control_update(p, cc, error))
+            for (int i = 0; i < n; ++i)
+              y[i] = ynew[i];
+          ctl[c] = cc;
+          running = cc.status == MICM_HIP_STATE_RUNNING;
+        }
+      }
+    }
+    if (!lockstep)
+    {
+      const unsigned long long mask = __ballot(running);
+      if ((threadIdx.x % kGroup) == 0 && mask)
+        atomicAdd(active, static_cast<unsigned int>(__popcll(mask)));
+    }
+  }
+
+  // Lockstep error: the reference's single running sum over cells (outer) and
+  // species (inner) of one State (rosenbrock.inl:372-401), by one wavefront.
+  __global__ void __launch_bounds__(kGroup) k_lockstep_error(DevMech m, DevParams p, int64_t n_cells, Work w,
+                                                             const CellControl* ctl, const double* __restrict__ atol,
+                                                             double* error_out)
+  {
+    __shared__ double q[kGroup];
+    const int l = threadIdx.x;
+    const int n = m.n_species;
+    double acc = 0.0;
+    if (ctl[0].status != MICM_HIP_STATE_RUNNING)
+      return;
+    for (int64_t c = 0; c < n_cells; ++c)
+    {
+      const size_t g = c / kGroup;
+      const int cl = c % kGroup;
+      Lane y = lane_view(w.y, g, n, cl), ynew = lane_view(w.ynew, g, n, cl), yerr = lane_view(w.yerr, g, n, cl);
+      for (int i0 = 0; i0 < n; i0 += kGroup)
+      {
+        const int i = i0 + l;
+        if (i < n)
+        {
+          const double a = fabs(y[i]), b = fabs(ynew[i]);
+          const double ymax = a > b ? a : b;
+          const double v = yerr[i] / (atol[i] + p.rtol * ymax);
+          q[l] = v * v;
+        }
+        __syncthreads();
+        if (l == 0)
+        {
+          const int cnt = min(kGroup, n - i0);
+          for (int j = 0; j < cnt; ++j)
+            acc += q[j];
+        }
+        __syncthreads();
+      }
+    }
+    if (l == 0)
+    {
+      const double N = static_cast<double>(n_cells * n > 0 ? n_cells * n : 1);
+      *error_out = smax(sqrt(acc / N), 1.0e-10);
+    }
+  }
+
+  __global__ void __launch_bounds__(kBlock) k_lockstep_control(DevMech m, DevParams p, int64_t n_cells, Work w,
+                                                               CellControl* ctl, const double* error, unsigned int* active)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    bool running = false;
+    if (c < n_cells)
+    {
+      CellControl cc = ctl[c];
+      if (cc.status == MICM_HIP_STATE_RUNNING)
+      {
+        const size_t g = c / kGroup;
+        const int l = c % kGroup;
+        Lane y = lane_view(w.y, g, m.n_species, l), ynew = lane_view(w.ynew, g, m.n_species, l);
+        if (control_update(p, cc, *error))
+          for (int i = 0; i < m.n_species; ++i)
+            y[i] = ynew[i];
+        ctl[c] = cc;
+        running = cc.status == MICM_HIP_STATE_RUNNING;
+      }
+    }
+    const unsigned long long mask = __ballot(running);
+    if ((threadIdx.x % kGroup) == 0 && mask)
+      atomicAdd(active, static_cast<unsigned int>(__popcll(mask)));
+  }
+
+  // Lockstep loop-top: all cells share one control state, so a cell that stops at the
+  // loop-top checks stops them all; the factor kernel handles it per cell identically.
+
+  // Post-solve clamp (solver.hpp:275-297) and per-cell results.
+  __global__ void __launch_bounds__(kBlock) k_finalize(int n_species, int64_t n_cells, double* y, CellControl* ctl,
+                                                       int32_t* status, int64_t* stats, double* final_time)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n_cells)
+      return;
+    CellControl cc = ctl[c];
+    if (cc.status == MICM_HIP_STATE_RUNNING)
+      cc.status = MICM_HIP_STATE_CONVERGED;
+    status[c] = cc.status;
+    for (int i = 0; i < 7; ++i)
+      stats[c * 7 + i] = cc.stats[i];
+    final_time[c] = cc.t;
+    Lane yy = lane_view(y, c / kGroup, n_species, c % kGroup);
+    for (int i = 0; i < n_species; ++i)
+      yy[i] = smax(yy[i], 0.0);
+  }
+
+  __global__ void k_zero_u32(unsigned int* p)
+  {
+    *p = 0u;
+  }
+
+  __global__ void k_zero_i32(int* p)
+  {
+    *p = 0;
+  }
+
+  // ---- persistent form
+
+  struct Queue
+  {
+    int* next;  // next unclaimed position in `order`
+    const int32_t* order;  // cell processing order (may be null: identity)
+  };
+
+  // Each wavefront owns one scratch slot (grouped layout, group = slot).  Lanes take
+  // cells from the queue, copy the cell's rate constants and concentrations into the
+  // slot, and run attempts until the cell finishes; the result is clamped and written
+  // back, and the lane takes the next cell.  The loop ends when the queue is drained
+  // and every lane is idle, so every wavefront exits.
+  __global__ void __launch_bounds__(kBlock) k_persistent(DevMech m, DevParams p, int64_t n_cells, Queue q, Work slot,
+                                                         double* __restrict__ y_state, const double* __restrict__ k_state,
+                                                         const double* __restrict__ atol, int32_t* status_out,
+                                                         int64_t* stats_out, double* final_time_out)
+  {
+    const int l = threadIdx.x % kGroup;
+    const size_t s = static_cast<size_t>(blockIdx.x) * (kBlock / kGroup) + threadIdx.x / kGroup;
+    const int n = m.n_species;
+    Lane y = lane_view(slot.y, s, n, l), k = lane_view(slot.k, s, m.n_reactions, l);
+    Lane f0 = lane_view(slot.f0, s, n, l), ynew = lane_view(slot.ynew, s, n, l), yerr = lane_view(slot.yerr, s, n, l);
+    Lane J = lane_view(slot.jac, s, m.nnz_jac, l), L = lane_view(slot.lower, s, m.nnz_lower, l),
+         U = lane_view(slot.upper, s, m.nnz_upper, l);
+    const Stages K = stage_view(slot, s, n, l);
+
+    int64_t cell = -1;
+    bool drained = false;
+    CellControl cc;
+    while (true)
+    {
+      // refill idle lanes with one atomic per wavefront
+      const bool want = cell < 0 && !drained;
+      const unsigned long long mask = __ballot(want);
+      if (mask)
+      {
+        const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+        int base = 0;
+        if (l == leader)
+          base = atomicAdd(q.next, __popcll(mask));
+        base = __shfl(base, leader);
+        if (want)
+        {
+          const int pos = base + __popcll(mask & ((1ull << l) - 1ull));
+          if (pos < n_cells)
+          {
+            cell = q.order ? q.order[pos] : pos;
+            Lane ys = lane_view(y_state, cell / kGroup, n, cell % kGroup);
+            Lane ks = lane_view(const_cast<double*>(k_state), cell / kGroup, m.n_reactions, cell % kGroup);
+            for (int i = 0; i < n; ++i)
+              y[i] = ys[i];
+            for (int r = 0; r < m.n_reactions; ++r)
+              k[r] = ks[r];
+            control_init(p, cc);
+          }
+          else
+            drained = true;
+        }
+      }
+      if (!__any(cell >= 0))
+        break;
+      if (cell >= 0)
+      {
+        bool running = factor_attempt(m, p, y, k, f0, J, L, U, cc);
+        if (running)
+        {
+          for (int st = 0; st < p.stages; ++st)
+            run_stage(m, p, st, y, k, f0, ynew, K, L, U, cc);
+          const double e = combine_and_error(m, p, y, ynew, yerr, K, atol);
+          const double error = smax(sqrt(e / static_cast<double>(n > 0 ? n : 1)), 1.0e-10);
+          if (control_update(p, cc, error))
+            for (int i = 0; i < n; ++i)
+              y[i] = ynew[i];
+          running = cc.status == MICM_HIP_STATE_RUNNING;
+        }
+        if (!running)
+        {
+          Lane ys = lane_view(y_state, cell / kGroup, n, cell % kGroup);
+          for (int i = 0; i < n; ++i)
+            ys[i] = smax(y[i], 0.0);
+          status_out[cell] = cc.status;
+          for (int i = 0; i < 7; ++i)
+            stats_out[cell * 7 + i] = cc.stats[i];
+          final_time_out[cell] = cc.t;
+          cell = -1;
+        }
+      }
+    }
+  }
+}  // namespace
+
+// ====================================================================== C-ABI
+
+struct micm_hip_mechanism
+{
+  MechanismTables t;
+  DevMech dev{};
+  void* blob = nullptr;  // one device allocation for every table
+  int device = 0;
+};
+
+struct micm_hip_state
+{
+  const micm_hip_mechanism* mech = nullptr;
+  int64_t n_cells = 0, n_groups = 0;
+  hipStream_t stream = nullptr;
+  double rtol = 1e-6;
+  // device arrays
+  double *y = nullptr, *cond = nullptr, *custom = nullptr, *k = nullptr, *jac = nullptr, *lower = nullptr,
+         *upper = nullptr, *atol = nullptr;
+  double *f0 = nullptr, *ynew = nullptr, *yerr = nullptr, *K = nullptr;
+  CellControl* ctl = nullptr;
+  int32_t* status = nullptr;
+  int64_t* stats = nullptr;
+  double* final_time = nullptr;
+  unsigned int* active = nullptr;
+  double* lockstep_error = nullptr;
+  int* queue_next = nullptr;
+  unsigned int* host_active = nullptr;  // pinned
+  // persistent scratch
+  int64_t n_slots = 0;
+  double* slot_blob = nullptr;
+  int32_t* order = nullptr;  // persistent processing order
+  bool has_order = false;
+};
+
+namespace
+{
+  template<class T>
+  int upload(void* blob, size_t& off, const std::vector<T>& v, const T*& dst, std::vector<char>& staging)
+  {
+    off = (off + 255) & ~size_t(255);
+    const size_t bytes = v.size() * sizeof(T);
+    if (staging.size() < off + bytes)
+      staging.resize(off + bytes);
+    if (bytes)
+      std::memcpy(staging.data() + off, v.data(), bytes);
+    dst = reinterpret_cast<const T*>(static_cast<char*>(blob) + off);
+    off += bytes;
+    return 0;
+  }
+
+  template<class T>
+  size_t padded_bytes(const std::vector<T>& v)
+  {
+    return ((v.size() * sizeof(T) + 255) & ~size_t(255)) + 256;
+  }
+
+  int check_mech(const micm_hip_mechanism* m)
+  {
+    if (!m)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null mechanism");
+    return MICM_HIP_OK;
+  }
+
+  int ensure_uploaded(const micm_hip_mechanism* m);
+
+  // A mechanism whose tables are on the device.
+  int check_device_mech(const micm_hip_mechanism* m)
+  {
+    if (int rc = check_mech(m))
+      return rc;
+    return ensure_uploaded(m);
+  }
+
+  DevParams make_params(const micm_hip_rosenbrock_parameters& in, double dt, double rtol)
+  {
+    DevParams p{};
+    p.stages = static_cast<int32_t>(in.stages);
+    p.max_steps = in.max_number_of_steps;
+    p.round_off = in.round_off;
+    p.factor_min = in.factor_min;
+    p.factor_max = in.factor_max;
+    p.reject_decrease = in.rejection_factor_decrease;
+    p.safety = in.safety_factor;
+    // rosenbrock.inl:33-35
+    p.h_min = in.h_min == 0.0 ? 1.0e-15 * dt : in.h_min;
+    p.h_max = in.h_max == 0.0 ? dt : std::min(dt, in.h_max);
+    p.h_start = in.h_start == 0.0 ? 1.0e-6 * dt : std::min(p.h_max, in.h_start);
+    p.inv_elo = 1 / in.estimator_of_local_order;
+    p.gamma0 = in.gamma[0];
+    p.dt = dt;
+    p.rtol = rtol;
+    for (int i = 0; i < 6; ++i)
+    {
+      p.new_f[i] = in.new_function_evaluation[i];
+      p.m[i] = in.m[i];
+      p.e[i] = in.e[i];
+    }
+    for (int i = 0; i < 15; ++i)
+    {
+      p.a[i] = in.a[i];
+      p.c[i] = in.c[i];
+    }
+    return p;
+  }
+
+  Work state_work(micm_hip_state* s)
+  {
+    const auto& m = s->mech->dev;
+    Work w{};
+    w.y = s->y;
+    w.k = s->k;
+    w.jac = s->jac;
+    w.lower = s->lower;
+    w.upper = s->upper;
+    w.f0 = s->f0;
+    w.ynew = s->ynew;
+    w.yerr = s->yerr;
+    w.K = s->K;
+    w.k_stride = static_cast<size_t>(s->n_groups) * m.n_species * kGroup;
+    return w;
+  }
+
+  void host_to_grouped(const double* host, int64_t n_cells, int64_t n_vals, std::vector<double>& out)
+  {
+    const int64_t ng = n_groups_of(n_cells);
+    out.assign(static_cast<size_t>(ng) * n_vals * kGroup, 0.0);
+    for (int64_t c = 0; c < n_cells; ++c)
+      for (int64_t v = 0; v < n_vals; ++v)
+        out[(static_cast<size_t>(c / kGroup) * n_vals + v) * kGroup + c % kGroup] = host[c * n_vals + v];
+  }
+
+  void grouped_to_host(const std::vector<double>& in, int64_t n_cells, int64_t n_vals, double* host)
+  {
+    for (int64_t c = 0; c < n_cells; ++c)
+      for (int64_t v = 0; v < n_vals; ++v)
+        host[c * n_vals + v] = in[(static_cast<size_t>(c / kGroup) * n_vals + v) * kGroup + c % kGroup];
+  }
+
+  int set_grouped(micm_hip_state* s, double* dst, const double* host, int64_t n_vals)
+  {
+    if (!s || (!host && n_vals))
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null argument");
+    if (n_vals == 0)
+      return MICM_HIP_OK;
+    std::vector<double> buf;
+    host_to_grouped(host, s->n_cells, n_vals, buf);
+    HIP_TRY(hipMemcpyAsync(dst, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MICM_HIP_OK;
+  }
+
+  int get_grouped(const micm_hip_state* s, const double* src, double* host, int64_t n_vals)
+  {
+    if (!s || (!host && n_vals))
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null argument");
+    if (n_vals == 0)
+      return MICM_HIP_OK;
+    std::vector<double> buf(static_cast<size_t>(s->n_groups) * n_vals * kGroup);
+    HIP_TRY(hipMemcpyAsync(buf.data(), src, buf.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    grouped_to_host(buf, s->n_cells, n_vals, host);
+    return MICM_HIP_OK;
+  }
+
+  int ensure_slots(micm_hip_state* s, int64_t slots, int stages)
+  {
+    (void)stages;
+    if (s->n_slots >= slots)
+      return MICM_HIP_OK;
+    if (s->slot_blob)
+      HIP_TRY(hipFree(s->slot_blob));
+    s->slot_blob = nullptr;
+    const auto& m = s->mech->dev;
+    const size_t per = static_cast<size_t>(m.n_reactions + m.nnz_jac + m.nnz_lower + m.nnz_upper + 10 * m.n_species);
+    HIP_TRY(hipMalloc(&s->slot_blob, per * kGroup * sizeof(double) * slots));
+    s->n_slots = slots;
+    return MICM_HIP_OK;
+  }
+
+  Work slot_work(micm_hip_state* s)
+  {
+    const auto& m = s->mech->dev;
+    const size_t S = static_cast<size_t>(s->n_slots) * kGroup;
+    double* p = s->slot_blob;
+    Work w{};
+    auto take = [&](int64_t n) {
+      double* r = p;
+      p += S * n;
+      return r;
+    };
+    w.y = take(m.n_species);
+    w.k = take(m.n_reactions);
+    w.jac = take(m.nnz_jac);
+    w.lower = take(m.nnz_lower);
+    w.upper = take(m.nnz_upper);
+    w.f0 = take(m.n_species);
+    w.ynew = take(m.n_species);
+    w.yerr = take(m.n_species);
+    w.K = take(6 * static_cast<int64_t>(m.n_species));
+    w.k_stride = S * m.n_species;
+    return w;
+  }
+
+  // Copies every table of a compiled mechanism into one device allocation.  Runs on
+  // the first call that needs the device, so a mechanism can be compiled and
+  // inspected on a host without a GPU.
+  int ensure_uploaded(const micm_hip_mechanism* cm)
+  {
+    auto* m = const_cast<micm_hip_mechanism*>(cm);
+    if (m->blob)
+      return MICM_HIP_OK;
+    const MechanismTables& t = m->t;
+    HIP_TRY(hipGetDevice(&m->device));
+    size_t bytes = 0;
+    bytes += padded_bytes(t.rate_type) + padded_bytes(t.rate_params) + padded_bytes(t.custom_index) +
+             padded_bytes(t.multiplier_offsets) + padded_bytes(t.multipliers) + padded_bytes(t.reactant_offsets) +
+             padded_bytes(t.reactant_ids) + padded_bytes(t.product_offsets) + padded_bytes(t.product_ids) +
+             padded_bytes(t.yields) + padded_bytes(t.term_process) + padded_bytes(t.term_dep_offsets) +
+             padded_bytes(t.term_dep_ids) + padded_bytes(t.term_flat_offsets) + padded_bytes(t.term_flat) +
+             padded_bytes(t.jac_diag) + padded_bytes(t.lu_out) + padded_bytes(t.lu_a) + padded_bytes(t.lu_div) +
+             padded_bytes(t.lu_term_offsets) + padded_bytes(t.lu_term_l) + padded_bytes(t.lu_term_u) +
+             padded_bytes(t.lower_diag) + padded_bytes(t.upper_diag) + padded_bytes(t.fwd_offsets) +
+             padded_bytes(t.fwd_mat) + padded_bytes(t.fwd_col) + padded_bytes(t.bwd_offsets) + padded_bytes(t.bwd_mat) +
+             padded_bytes(t.bwd_col) + 256;
+    HIP_TRY(hipMalloc(&m->blob, bytes));
+    std::vector<char> staging;
+    size_t off = 0;
+    DevMech& d = m->dev;
+    d.n_species = static_cast<int32_t>(t.n_species);
+    d.n_reactions = static_cast<int32_t>(t.n_reactions);
+    d.n_custom = static_cast<int32_t>(t.n_custom);
+    d.nnz_jac = static_cast<int32_t>(t.nnz_jacobian());
+    d.nnz_lower = static_cast<int32_t>(t.nnz_lower());
+    d.nnz_upper = static_cast<int32_t>(t.nnz_upper());
+    d.n_terms = static_cast<int32_t>(t.term_process.size());
+    d.n_lu_elements = static_cast<int32_t>(t.lu_out.size());
+    upload(m->blob, off, t.rate_type, d.rate_type, staging);
+    upload(m->blob, off, t.rate_params, d.rate_params, staging);
+    upload(m->blob, off, t.custom_index, d.custom_index, staging);
+    upload(m->blob, off, t.multiplier_offsets, d.multiplier_offsets, staging);
+    upload(m->blob, off, t.multipliers, d.multipliers, staging);
+    upload(m->blob, off, t.reactant_offsets, d.reactant_offsets, staging);
+    upload(m->blob, off, t.reactant_ids, d.reactant_ids, staging);
+    upload(m->blob, off, t.product_offsets, d.product_offsets, staging);
+    upload(m->blob, off, t.product_ids, d.product_ids, staging);
+    upload(m->blob, off, t.yields, d.yields, staging);
+    upload(m->blob, off, t.term_process, d.term_process, staging);
+    upload(m->blob, off, t.term_dep_offsets, d.term_dep_offsets, staging);
+    upload(m->blob, off, t.term_dep_ids, d.term_dep_ids, staging);
+    upload(m->blob, off, t.term_flat_offsets, d.term_flat_offsets, staging);
+    upload(m->blob, off, t.term_flat, d.term_flat, staging);
+    upload(m->blob, off, t.jac_diag, d.jac_diag, staging);
+    upload(m->blob, off, t.lu_out, d.lu_out, staging);
+    upload(m->blob, off, t.lu_a, d.lu_a, staging);
+    upload(m->blob, off, t.lu_div, d.lu_div, staging);
+    upload(m->blob, off, t.lu_term_offsets, d.lu_term_offsets, staging);
+    upload(m->blob, off, t.lu_term_l, d.lu_term_l, staging);
+    upload(m->blob, off, t.lu_term_u, d.lu_term_u, staging);
+    upload(m->blob, off, t.lower_diag, d.lower_diag, staging);
+    upload(m->blob, off, t.upper_diag, d.upper_diag, staging);
+    upload(m->blob, off, t.fwd_offsets, d.fwd_offsets, staging);
+    upload(m->blob, off, t.fwd_mat, d.fwd_mat, staging);
+    upload(m->blob, off, t.fwd_col, d.fwd_col, staging);
+    upload(m->blob, off, t.bwd_offsets, d.bwd_offsets, staging);
+    upload(m->blob, off, t.bwd_mat, d.bwd_mat, staging);
+    upload(m->blob, off, t.bwd_col, d.bwd_col, staging);
+    if (off > bytes)
+      return fail(MICM_HIP_ERR_DEVICE, "internal table sizing error");
+    HIP_TRY(hipMemcpy(m->blob, staging.data(), off, hipMemcpyHostToDevice));
+    return MICM_HIP_OK;
+  }
+}  // namespace
+
+extern "C"
+{
+  const char* micm_hip_last_error(void)
+  {
+    return g_last_error.c_str();
+  }
+
+  int micm_hip_rosenbrock_parameters_preset(int32_t preset, micm_hip_rosenbrock_parameters* out)
+  {
+    if (!out)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null output");
+    if (!rosenbrock_preset(preset, *out))
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "unknown Rosenbrock preset " + std::to_string(preset));
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_mechanism_create(const micm_hip_mechanism_desc* desc, micm_hip_mechanism** out)
+  {
+    if (!desc || !out)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    auto m = std::make_unique<micm_hip_mechanism>();
+    try
+    {
+      m->t = compile_mechanism(*desc);
+    }
+    catch (const std::domain_error& e)
+    {
+      return fail(MICM_HIP_ERR_SPECIES, e.what());
+    }
+    catch (const std::exception& e)
+    {
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, e.what());
+    }
+    {
+      const MechanismTables& t = m->t;
+      DevMech& d = m->dev;
+      d.n_species = static_cast<int32_t>(t.n_species);
+      d.n_reactions = static_cast<int32_t>(t.n_reactions);
+      d.n_custom = static_cast<int32_t>(t.n_custom);
+      d.nnz_jac = static_cast<int32_t>(t.nnz_jacobian());
+      d.nnz_lower = static_cast<int32_t>(t.nnz_lower());
+      d.nnz_upper = static_cast<int32_t>(t.nnz_upper());
+      d.n_terms = static_cast<int32_t>(t.term_process.size());
+      d.n_lu_elements = static_cast<int32_t>(t.lu_out.size());
+    }
+    *out = m.release();
+    return MICM_HIP_OK;
+  }
+
+  void micm_hip_mechanism_destroy(micm_hip_mechanism* m)
+  {
+    if (!m)
+      return;
+    if (m->blob)
+      (void)hipFree(m->blob);
+    delete m;
+  }
+
+  int micm_hip_mechanism_get_info(const micm_hip_mechanism* m, micm_hip_mechanism_info* out)
+  {
+    if (int rc = check_mech(m))
+      return rc;
+    if (!out)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null output");
+    const auto& t = m->t;
+    out->n_species = t.n_species;
+    out->n_reactions = t.n_reactions;
+    out->n_custom_parameters = t.n_custom;
+    out->nnz_jacobian = t.nnz_jacobian();
+    out->nnz_lower = t.nnz_lower();
+    out->nnz_upper = t.nnz_upper();
+    out->n_process_infos = static_cast<int64_t>(t.term_process.size());
+    out->n_jacobian_terms = static_cast<int64_t>(t.term_flat.size());
+    out->lu_multiply_adds = t.lu_multiply_adds();
+    out->n_reactant_slots = static_cast<int64_t>(t.reactant_ids.size());
+    out->n_product_slots = static_cast<int64_t>(t.product_ids.size());
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_mechanism_get_species_order(const micm_hip_mechanism* m, int64_t* out)
+  {
+    if (int rc = check_mech(m))
+      return rc;
+    if (!out)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null output");
+    std::copy(m->t.state_to_declared.begin(), m->t.state_to_declared.end(), out);
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_mechanism_get_reaction_order(const micm_hip_mechanism* m, int64_t* out)
+  {
+    if (int rc = check_mech(m))
+      return rc;
+    if (!out)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null output");
+    std::copy(m->t.sorted_to_declared.begin(), m->t.sorted_to_declared.end(), out);
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_mechanism_get_custom_parameter_owner(const micm_hip_mechanism* m, int64_t* out)
+  {
+    if (int rc = check_mech(m))
+      return rc;
+    if (!out && !m->t.custom_owner.empty())
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null output");
+    std::copy(m->t.custom_owner.begin(), m->t.custom_owner.end(), out);
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_mechanism_get_patterns(const micm_hip_mechanism* m, int64_t* jr, int64_t* jc, int64_t* lr, int64_t* lc,
+                                      int64_t* ur, int64_t* uc)
+  {
+    if (int rc = check_mech(m))
+      return rc;
+    auto dump = [](const std::vector<int32_t>& ptr, const std::vector<int32_t>& col, int64_t* rows, int64_t* cols)
+    {
+      for (size_t i = 0; i + 1 < ptr.size(); ++i)
+        for (int32_t p = ptr[i]; p < ptr[i + 1]; ++p)
+        {
+          if (rows)
+            rows[p] = static_cast<int64_t>(i);
+          if (cols)
+            cols[p] = col[p];
+        }
+    };
+    dump(m->t.jac_row_ptr, m->t.jac_col, jr, jc);
+    dump(m->t.lower_row_ptr, m->t.lower_col, lr, lc);
+    dump(m->t.upper_row_ptr, m->t.upper_col, ur, uc);
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_mechanism_get_jacobian_diagonal(const micm_hip_mechanism* m, int64_t* out)
+  {
+    if (int rc = check_mech(m))
+      return rc;
+    if (!out)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null output");
+    std::copy(m->t.jac_diag.begin(), m->t.jac_diag.end(), out);
+    return MICM_HIP_OK;
+  }
+
+  // ---- kernel entry points
+
+#define KERNEL_PRELUDE()                                                   \
+  if (int rc = check_device_mech(m))                                       \
+    return rc;                                                             \
+  if (n_cells < 0)                                                         \
+    return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "negative cell count");     \
+  if (n_cells == 0)                                                        \
+    return MICM_HIP_OK;                                                    \
+  hipStream_t st = static_cast<hipStream_t>(stream)
+
+  int micm_hip_calculate_rate_constants(const micm_hip_mechanism* m, int64_t n_cells, const double* cond,
+                                        const double* custom, double* k, void* stream)
+  {
+    KERNEL_PRELUDE();
+    if (!cond || !k || (m->dev.n_custom > 0 && !custom))
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null device pointer");
+    hipLaunchKernelGGL(k_rate_constants, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, cond, custom, k);
+    HIP_TRY(hipGetLastError());
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_add_forcing_terms(const micm_hip_mechanism* m, int64_t n_cells, const double* k, const double* y,
+                                 double* f, void* stream)
+  {
+    KERNEL_PRELUDE();
+    if (!k || !y || !f)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null device pointer");
+    hipLaunchKernelGGL(k_add_forcing, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, k, y, f);
+    HIP_TRY(hipGetLastError());
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_subtract_jacobian_terms(const micm_hip_mechanism* m, int64_t n_cells, const double* k, const double* y,
+                                       double* jac, void* stream)
+  {
+    KERNEL_PRELUDE();
+    if (!k || !y || !jac)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null device pointer");
+    hipLaunchKernelGGL(k_subtract_jacobian, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, k, y, jac);
+    HIP_TRY(hipGetLastError());
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_alpha_minus_jacobian(const micm_hip_mechanism* m, int64_t n_cells, double* jac, double alpha, void* stream)
+  {
+    KERNEL_PRELUDE();
+    if (!jac)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null device pointer");
+    hipLaunchKernelGGL(k_alpha_minus_jacobian, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, jac, alpha);
+    HIP_TRY(hipGetLastError());
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_lu_decompose(const micm_hip_mechanism* m, int64_t n_cells, const double* jac, double* lower, double* upper,
+                            void* stream)
+  {
+    KERNEL_PRELUDE();
+    if (!jac || !lower || !upper)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null device pointer");
+    hipLaunchKernelGGL(k_lu_decompose, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, jac, lower, upper);
+    HIP_TRY(hipGetLastError());
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_lu_solve(const micm_hip_mechanism* m, int64_t n_cells, const double* lower, const double* upper, double* x,
+                        void* stream)
+  {
+    KERNEL_PRELUDE();
+    if (!lower || !upper || !x)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null device pointer");
+    hipLaunchKernelGGL(k_lu_solve, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, lower, upper, x);
+    HIP_TRY(hipGetLastError());
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_normalized_error(const micm_hip_mechanism* m, int64_t n_cells, const double* y, const double* ynew,
+                                const double* yerr, const double* atol, double rtol, double* error, void* stream)
+  {
+    KERNEL_PRELUDE();
+    if (!y || !ynew || !yerr || !atol || !error)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null device pointer");
+    hipLaunchKernelGGL(k_normalized_error, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev.n_species, n_cells, y,
+                       ynew, yerr, atol, rtol, error);
+    HIP_TRY(hipGetLastError());
+    return MICM_HIP_OK;
+  }
+
+  // ---- state
+
+  int micm_hip_state_create(const micm_hip_mechanism* m, int64_t n_cells, micm_hip_state** out)
+  {
+    if (int rc = check_device_mech(m))
+      return rc;
+    if (!out || n_cells <= 0)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "state needs at least one cell");
+    *out = nullptr;
+    auto s = std::make_unique<micm_hip_state>();
+    s->mech = m;
+    s->n_cells = n_cells;
+    s->n_groups = n_groups_of(n_cells);
+    const auto& d = m->dev;
+    const size_t G = static_cast<size_t>(s->n_groups) * kGroup;
+    auto alloc = [&](double** p, size_t n) -> int {
+      HIP_TRY(hipMalloc(p, std::max<size_t>(n, 1) * sizeof(double)));
+      HIP_TRY(hipMemset(*p, 0, std::max<size_t>(n, 1) * sizeof(double)));
+      return MICM_HIP_OK;
+    };
+    int rc = 0;
+    if ((rc = alloc(&s->y, G * d.n_species)) || (rc = alloc(&s->cond, G * 3)) || (rc = alloc(&s->custom, G * d.n_custom)) ||
+        (rc = alloc(&s->k, G * d.n_reactions)) || (rc = alloc(&s->jac, G * d.nnz_jac)) ||
+        (rc = alloc(&s->lower, G * d.nnz_lower)) || (rc = alloc(&s->upper, G * d.nnz_upper)) ||
+        (rc = alloc(&s->atol, d.n_species)) || (rc = alloc(&s->f0, G * d.n_species)) ||
+        (rc = alloc(&s->ynew, G * d.n_species)) || (rc = alloc(&s->yerr, G * d.n_species)) ||
+        (rc = alloc(&s->K, G * d.n_species * 6)) || (rc = alloc(&s->final_time, n_cells)) ||
+        (rc = alloc(&s->lockstep_error, 1)))
+    {
+      micm_hip_state_destroy(s.release());
+      return rc;
+    }
+    HIP_TRY(hipMalloc(&s->ctl, sizeof(CellControl) * n_cells));
+    HIP_TRY(hipMalloc(&s->status, sizeof(int32_t) * n_cells));
+    HIP_TRY(hipMemset(s->status, 0, sizeof(int32_t) * n_cells));
+    HIP_TRY(hipMalloc(&s->stats, sizeof(int64_t) * 7 * n_cells));
+    HIP_TRY(hipMemset(s->stats, 0, sizeof(int64_t) * 7 * n_cells));
+    HIP_TRY(hipMalloc(&s->active, sizeof(unsigned int)));
+    HIP_TRY(hipMalloc(&s->queue_next, sizeof(int)));
+    HIP_TRY(hipMalloc(&s->order, sizeof(int32_t) * n_cells));
+    HIP_TRY(hipHostMalloc(&s->host_active, sizeof(unsigned int)));
+    HIP_TRY(hipMemcpy(s->atol, m->t.absolute_tolerances.data(), sizeof(double) * d.n_species, hipMemcpyHostToDevice));
+    *out = s.release();
+    return MICM_HIP_OK;
+  }
+
+  void micm_hip_state_destroy(micm_hip_state* s)
+  {
+    if (!s)
+      return;
+    for (double* p : { s->y, s->cond, s->custom, s->k, s->jac, s->lower, s->upper, s->atol, s->f0, s->ynew, s->yerr, s->K,
+                       s->final_time, s->lockstep_error, s->slot_blob })
+      if (p)
+        (void)hipFree(p);
+    if (s->ctl)
+      (void)hipFree(s->ctl);
+    if (s->status)
+      (void)hipFree(s->status);
+    if (s->stats)
+      (void)hipFree(s->stats);
+    if (s->active)
+      (void)hipFree(s->active);
+    if (s->queue_next)
+      (void)hipFree(s->queue_next);
+    if (s->order)
+      (void)hipFree(s->order);
+    if (s->host_active)
+      (void)hipHostFree(s->host_active);
+    delete s;
+  }
+
+  int micm_hip_state_get_pointers(const micm_hip_state* s, micm_hip_state_pointers* out)
+  {
+    if (!s || !out)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null argument");
+    out->variables = s->y;
+    out->conditions = s->cond;
+    out->custom_parameters = s->custom;
+    out->rate_constants = s->k;
+    out->jacobian = s->jac;
+    out->lower = s->lower;
+    out->upper = s->upper;
+    out->absolute_tolerances = s->atol;
+    out->cell_state = s->status;
+    out->cell_stats = s->stats;
+    out->cell_final_time = s->final_time;
+    out->n_cells = s->n_cells;
+    out->n_groups = s->n_groups;
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_state_set_variables(micm_hip_state* s, const double* host)
+  {
+    return set_grouped(s, s ? s->y : nullptr, host, s ? s->mech->dev.n_species : 0);
+  }
+  int micm_hip_state_get_variables(const micm_hip_state* s, double* host)
+  {
+    return get_grouped(s, s ? s->y : nullptr, host, s ? s->mech->dev.n_species : 0);
+  }
+  int micm_hip_state_set_conditions(micm_hip_state* s, const double* host)
+  {
+    return set_grouped(s, s ? s->cond : nullptr, host, 3);
+  }
+  int micm_hip_state_set_custom_parameters(micm_hip_state* s, const double* host)
+  {
+    return set_grouped(s, s ? s->custom : nullptr, host, s ? s->mech->dev.n_custom : 0);
+  }
+  int micm_hip_state_get_rate_constants(const micm_hip_state* s, double* host)
+  {
+    return get_grouped(s, s ? s->k : nullptr, host, s ? s->mech->dev.n_reactions : 0);
+  }
+  int micm_hip_state_set_rate_constants(micm_hip_state* s, const double* host)
+  {
+    return set_grouped(s, s ? s->k : nullptr, host, s ? s->mech->dev.n_reactions : 0);
+  }
+
+  int micm_hip_state_set_tolerances(micm_hip_state* s, double rtol, const double* atol)
+  {
+    if (!s)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null state");
+    if (!(rtol > 0))
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "relative tolerance must be positive");
+    s->rtol = rtol;
+    if (atol)
+    {
+      HIP_TRY(hipMemcpyAsync(s->atol, atol, sizeof(double) * s->mech->dev.n_species, hipMemcpyHostToDevice, s->stream));
+      HIP_TRY(hipStreamSynchronize(s->stream));
+    }
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_state_get_results(const micm_hip_state* s, int32_t* states, int64_t* stats, double* final_time)
+  {
+    if (!s)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null state");
+    if (states)
+      HIP_TRY(hipMemcpyAsync(states, s->status, sizeof(int32_t) * s->n_cells, hipMemcpyDeviceToHost, s->stream));
+    if (stats)
+      HIP_TRY(hipMemcpyAsync(stats, s->stats, sizeof(int64_t) * 7 * s->n_cells, hipMemcpyDeviceToHost, s->stream));
+    if (final_time)
+      HIP_TRY(hipMemcpyAsync(final_time, s->final_time, sizeof(double) * s->n_cells, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_state_set_stream(micm_hip_state* s, void* stream)
+  {
+    if (!s)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null state");
+    s->stream = static_cast<hipStream_t>(stream);
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_update_state_parameters(const micm_hip_mechanism* m, micm_hip_state* s)
+  {
+    if (int rc = check_mech(m))
+      return rc;
+    if (!s || s->mech != m)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "state does not belong to this mechanism");
+    hipLaunchKernelGGL(k_rate_constants, dim3(grid_for(s->n_cells)), dim3(kBlock), 0, s->stream, m->dev, s->n_cells,
+                       s->cond, s->custom, s->k);
+    HIP_TRY(hipGetLastError());
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_solve(const micm_hip_mechanism* m, micm_hip_state* s, double time_step,
+                     const micm_hip_rosenbrock_parameters* params, int32_t form)
+  {
+    if (int rc = check_mech(m))
+      return rc;
+    if (!s || s->mech != m || !params)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "bad state or parameters");
+    if (params->stages < 1 || params->stages > 6)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "stages must be 1..6");
+    const DevParams p = make_params(*params, time_step, s->rtol);
+    const DevMech& d = m->dev;
+    hipStream_t st = s->stream;
+    const unsigned grid = grid_for(s->n_cells);
+    if (form == MICM_HIP_SOLVE_PLAIN || form == MICM_HIP_SOLVE_LOCKSTEP)
+    {
+      const bool lockstep = form == MICM_HIP_SOLVE_LOCKSTEP;
+      Work w = state_work(s);
+      hipLaunchKernelGGL(k_plain_init, dim3(grid), dim3(kBlock), 0, st, p, s->n_cells, s->ctl);
+      HIP_TRY(hipGetLastError());
+      // every attempt either finishes a cell or advances its step counter, and a cell
+      // stops after max_number_of_steps + 1 attempts, so this bound is never reached
+      const int64_t max_iterations = (params->max_number_of_steps + 2) * 64;
+      for (int64_t it = 0; it < max_iterations; ++it)
+      {
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, s->active);
+        hipLaunchKernelGGL(k_plain_factor, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl);
+        for (int stage = 0; stage < p.stages; ++stage)
+          hipLaunchKernelGGL(k_plain_stage, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl, stage);
+        hipLaunchKernelGGL(k_plain_finish, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl, s->atol,
+                           s->active, lockstep ? 1 : 0);
+        if (lockstep)
+        {
+          hipLaunchKernelGGL(k_lockstep_error, dim3(1), dim3(kGroup), 0, st, d, p, s->n_cells, w, s->ctl, s->atol,
+                             s->lockstep_error);
+          hipLaunchKernelGGL(k_lockstep_control, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl,
+                             s->lockstep_error, s->active);
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(s->host_active, s->active, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (*s->host_active == 0)
+          break;
+      }
+      hipLaunchKernelGGL(k_finalize, dim3(grid), dim3(kBlock), 0, st, d.n_species, s->n_cells, s->y, s->ctl, s->status,
+                         s->stats, s->final_time);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipStreamSynchronize(st));
+      return MICM_HIP_OK;
+    }
+    if (form == MICM_HIP_SOLVE_PERSISTENT)
+    {
+      int dev = 0, cus = 0;
+      HIP_TRY(hipGetDevice(&dev));
+      HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      int blocks_per_cu = 0;
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, k_persistent, kBlock, 0));
+      if (blocks_per_cu < 1)
+        blocks_per_cu = 1;
+      int64_t blocks = static_cast<int64_t>(cus) * blocks_per_cu;
+      const int64_t needed = (s->n_groups + (kBlock / kGroup) - 1) / (kBlock / kGroup);
+      blocks = std::max<int64_t>(1, std::min(blocks, needed));
+      if (int rc = ensure_slots(s, blocks * (kBlock / kGroup), p.stages))
+        return rc;
+      Work w = slot_work(s);
+      hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, s->queue_next);
+      Queue q{ s->queue_next, s->has_order ? s->order : nullptr };
+      hipLaunchKernelGGL(k_persistent, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, st, d, p, s->n_cells, q, w,
+                         s->y, s->k, s->atol, s->status, s->stats, s->final_time);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipStreamSynchronize(st));
+      return MICM_HIP_OK;
+    }
+    return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "unknown solve form");
+  }
+}

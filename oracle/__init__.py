@@ -1,0 +1,190 @@
+"""TEST INFRASTRUCTURE ONLY -- the oracle side.
+
+Two CPU solvers with one ctypes interface:
+
+* `Oracle`     -- liboracle.so, this project's CPU restatement of the
+                  reference's Rosenbrock path (oracle/micm_oracle.cpp).  It is
+                  the checker on the GPU box and the "port" CPU baseline.
+* `Reference`  -- oracle/_ref/libmicm_ref.so, the reference's own header-only
+                  solver compiled from its sources (oracle/ref_driver.cpp).  It
+                  exists only where the reference headers were available at
+                  build time and pins the restatement.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may use
+this package, and only as the checker or the CPU baseline.  Every per-cell
+array is cell-major in the solver's state order ([cell][state index]).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libmicm_ref.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp) if a is not None else _dp()
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip) if a is not None else _ip()
+
+
+def _i32(a):
+    return a.ctypes.data_as(_i32p) if a is not None else _i32p()
+
+
+def reference_available():
+    return os.path.exists(REF_LIB)
+
+
+class _Base:
+    prefix = None
+    path = None
+
+    _libs = {}
+
+    @classmethod
+    def _lib(cls):
+        if cls.path not in cls._libs:
+            if not os.path.exists(cls.path):
+                raise FileNotFoundError(f"{cls.path} is not built (python -c 'import __graft_entry__ as g; g.build()')")
+            lib = ctypes.CDLL(cls.path)
+            p = cls.prefix
+            getattr(lib, p + "solver_create").restype = _vp
+            getattr(lib, p + "solver_create").argtypes = [_vp, ctypes.c_int32]
+            getattr(lib, p + "solver_destroy").argtypes = [_vp]
+            getattr(lib, p + "last_error").restype = ctypes.c_char_p
+            sig = {
+                "sizes": [_vp, _ip],
+                "species_order": [_vp, _ip],
+                "patterns": [_vp, _ip, _ip, _ip, _ip, _ip, _ip],
+                "rate_constants": [_vp, ctypes.c_int64, _dp, _dp, _dp],
+                "forcing": [_vp, ctypes.c_int64, _dp, _dp, _dp],
+                "jacobian": [_vp, ctypes.c_int64, _dp, _dp, _dp],
+                "lu": [_vp, ctypes.c_int64, _dp, ctypes.c_double, _dp, _dp, _dp],
+                "lu_solve": [_vp, ctypes.c_int64, _dp, _dp, _dp],
+                "normalized_error": [_vp, ctypes.c_int64, _dp, _dp, _dp, ctypes.c_double, _dp, _dp],
+                "solve": [_vp, ctypes.c_int64, ctypes.c_int32, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int64,
+                          ctypes.c_double, _dp, _i32p, _ip, _dp],
+            }
+            for name, args in sig.items():
+                f = getattr(lib, p + name)
+                f.argtypes = args
+                f.restype = ctypes.c_int
+            cls._libs[cls.path] = lib
+        return cls._libs[cls.path]
+
+    def __init__(self, desc, tableau=3):
+        self.desc = desc
+        self.lib = self._lib()
+        self.h = getattr(self.lib, self.prefix + "solver_create")(ctypes.byref(desc.as_c()), int(tableau))
+        if not self.h:
+            raise RuntimeError(getattr(self.lib, self.prefix + "last_error")().decode())
+        sizes = np.zeros(6, dtype=np.int64)
+        self._call("sizes", _i(sizes))
+        self.n_species, self.n_reactions, self.n_custom, self.nnz_jacobian, self.nnz_lower, self.nnz_upper = \
+            (int(x) for x in sizes)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            getattr(self.lib, self.prefix + "solver_destroy")(self.h)
+            self.h = None
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, self.prefix + name)(self.h, *args)
+        if rc != 0:
+            raise RuntimeError(f"{self.prefix}{name}: " + getattr(self.lib, self.prefix + "last_error")().decode())
+
+    # ---------------------------------------------------------------- symbolic
+    def species_order(self):
+        """state_to_declared[i] = declared species index held in state column i."""
+        out = np.zeros(self.n_species, dtype=np.int64)
+        self._call("species_order", _i(out))
+        return out
+
+    def patterns(self):
+        arrs = [np.zeros(n, dtype=np.int64) for n in (self.nnz_jacobian, self.nnz_jacobian, self.nnz_lower,
+                                                     self.nnz_lower, self.nnz_upper, self.nnz_upper)]
+        self._call("patterns", *[_i(a) for a in arrs])
+        return {"jacobian": (arrs[0], arrs[1]), "lower": (arrs[2], arrs[3]), "upper": (arrs[4], arrs[5])}
+
+    # ---------------------------------------------------------------- kernels
+    def rate_constants(self, conditions, custom):
+        conditions = np.ascontiguousarray(conditions, dtype=np.float64)
+        n = conditions.shape[0]
+        custom = np.ascontiguousarray(custom, dtype=np.float64).reshape(n, -1)
+        k = np.zeros((n, self.n_reactions))
+        self._call("rate_constants", n, _d(conditions), _d(custom), _d(k))
+        return k
+
+    def forcing(self, k, y):
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        f = np.zeros_like(y)
+        self._call("forcing", y.shape[0], _d(k), _d(y), _d(f))
+        return f
+
+    def jacobian(self, k, y):
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        jac = np.zeros((y.shape[0], self.nnz_jacobian))
+        self._call("jacobian", y.shape[0], _d(k), _d(y), _d(jac))
+        return jac
+
+    def lu(self, jac, alpha):
+        """Returns (jacobian with alpha on its diagonal, L, U)."""
+        jac = np.ascontiguousarray(jac, dtype=np.float64)
+        n = jac.shape[0]
+        jout = np.zeros_like(jac)
+        lo = np.zeros((n, self.nnz_lower))
+        up = np.zeros((n, self.nnz_upper))
+        self._call("lu", n, _d(jac), float(alpha), _d(jout), _d(lo), _d(up))
+        return jout, lo, up
+
+    def lu_solve(self, lower, upper, b):
+        x = np.array(b, dtype=np.float64, copy=True, order="C")
+        self._call("lu_solve", x.shape[0], _d(np.ascontiguousarray(lower)), _d(np.ascontiguousarray(upper)), _d(x))
+        return x
+
+    def normalized_error(self, y, ynew, yerr, rtol, atol):
+        """The reference's norm over all cells of one State."""
+        y, ynew, yerr = (np.ascontiguousarray(a, dtype=np.float64) for a in (y, ynew, yerr))
+        atol = np.ascontiguousarray(atol, dtype=np.float64)
+        e = np.zeros(1)
+        self._call("normalized_error", y.shape[0], _d(y), _d(ynew), _d(yerr), float(rtol), _d(atol), _d(e))
+        return float(e[0])
+
+    def solve(self, conditions, custom, y, dt, n_solves=1, lockstep=False, rtol=1e-6, atol=None):
+        """n_solves x (UpdateStateParameters, Solve(dt)); per-cell States unless lockstep.
+        Returns (y_final, states, stats summed over solves, final_time of the last solve)."""
+        conditions = np.ascontiguousarray(conditions, dtype=np.float64)
+        n = conditions.shape[0]
+        custom = np.ascontiguousarray(custom, dtype=np.float64).reshape(n, -1)
+        y = np.array(y, dtype=np.float64, copy=True, order="C")
+        states = np.zeros(n, dtype=np.int32)
+        stats = np.zeros((n, 7), dtype=np.int64)
+        ft = np.zeros(n)
+        atol = None if atol is None else np.ascontiguousarray(atol, dtype=np.float64)
+        self._call("solve", n, int(bool(lockstep)), _d(conditions), _d(custom), _d(y), float(dt), int(n_solves),
+                   float(rtol), _d(atol), _i32(states), _i(stats), _d(ft))
+        return y, states, stats, ft
+
+
+class Reference(_Base):
+    """The reference's compiled CPU solver (needs oracle/_ref)."""
+    prefix = "ref_"
+    path = REF_LIB
+
+
+class Oracle(_Base):
+    """The CPU restatement."""
+    prefix = "oracle_"
+    path = ORACLE_LIB

@@ -1,0 +1,25 @@
+"""Mechanisms used by the tests besides the synthetic ones."""
+from micm_amd._desc import (ARRHENIUS, BRANCHED, REVERSIBLE, SURFACE, TAYLOR, TERNARY, TROE, TUNNELING,
+                            USER_DEFINED, MechanismDesc)
+
+
+def all_rate_types():
+    """Eight species, one reaction of every rate-constant type, a third body M
+    (c_rho = 1) on two reactions and a second parameterized species, declared in
+    an order that the type sort has to rearrange."""
+    M = -1          # third body
+    Q = -2          # parameterized: 0.5 + 1e-3 T
+    rx = [
+        ([0], [(1, 1.0)], USER_DEFINED, [2.5]),
+        ([1, 2], [(3, 0.7), (4, 0.3)], ARRHENIUS, [3.2e3, 1.1, -250.0, 280.0, 2.0e-6]),
+        ([2, M], [(5, 1.0)], TROE, [6.5e4, -2.6, 40.0, 3.1e6, 0.3, -15.0, 0.55, 1.2]),
+        ([3, 4], [(6, 1.0)], TERNARY, [1.2e5, -1.5, 0.0, 7.0e6, -0.4, 0.0, 0.6, 1.0]),
+        ([4], [(7, 1.0)], BRANCHED, [0.0, 1.1e-2, 350.0, 0.15, 6]),
+        ([5], [(7, 1.0)], BRANCHED, [1.0, 2.3e-2, 120.0, 0.4, 9]),
+        ([6], [(0, 0.5)], TUNNELING, [4.0e-3, 150.0, 1.0e6]),
+        ([7, Q], [(1, 1.0)], TAYLOR, [1.0e-3, 0.5, -30.0, 300.0, 1.0e-7, 3, 1.0, 1.0e-3, -2.0e-6]),
+        ([0, M], [(2, 1.0)], REVERSIBLE, [2.0e-5, 800.0, 3.0e-2]),
+        ([3], [(5, 1.0)], SURFACE, [1.5e-5, 0.046, 0.3]),
+        ([1, 1], [(6, 1.0)], ARRHENIUS, [5.0e2, 0.0, 100.0, 300.0, 0.0]),
+    ]
+    return MechanismDesc(8, rx, parameterized=[(0.0, 0.0, 0.0, 1.0), (0.5, 1.0e-3, 0.0, 0.0)])
