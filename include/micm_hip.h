@@ -102,7 +102,9 @@ extern "C" {
 /* Integration forms of micm_hip_solve. */
 #define MICM_HIP_SOLVE_LOCKSTEP 0   /* one step size per state, as rosenbrock.inl:12-304 */
 #define MICM_HIP_SOLVE_PLAIN 1      /* per-cell step size, one launch per kernel per stage */
-#define MICM_HIP_SOLVE_PERSISTENT 2 /* per-cell step size, persistent work-queue kernel */
+#define MICM_HIP_SOLVE_PERSISTENT 2 /* per-cell step size, persistent work-queue kernel; cells that
+                                       reach the hand-off step count continue in the cooperative form */
+#define MICM_HIP_SOLVE_COOPERATIVE 3 /* per-cell step size, one wavefront per cell, working set in LDS */
 
 /* Flat mechanism description.  Species are the phase's non-parameterized
  * species in declaration order; reactions are in the order given to
@@ -267,6 +269,12 @@ int micm_hip_state_set_tolerances(micm_hip_state* state, double relative_toleran
 int micm_hip_state_get_results(const micm_hip_state* state, int32_t* states, int64_t* stats, double* final_time);
 /* Stream used by the state's solves (hipStream_t; NULL = default stream). */
 int micm_hip_state_set_stream(micm_hip_state* state, void* stream);
+/* Step count at which the persistent form hands a cell to the cooperative form
+ * (0: never; negative: the library's default).  Results do not depend on it. */
+int micm_hip_state_set_handoff(micm_hip_state* state, int64_t steps);
+/* Whether the cooperative form fits this mechanism on the current device (its
+ * per-cell working set must fit one workgroup's LDS); *out = 1 or 0. */
+int micm_hip_mechanism_cooperative_supported(const micm_hip_mechanism* mechanism, int32_t stages, int32_t* out);
 
 /* Solver::UpdateStateParameters (solver.hpp:217-247) on the device. */
 int micm_hip_update_state_parameters(const micm_hip_mechanism* mechanism, micm_hip_state* state);

@@ -28,8 +28,8 @@ import numpy as np
 
 from . import _desc
 from ._desc import MechanismDesc
-from ._lib import (SOLVE_LOCKSTEP, SOLVE_PERSISTENT, SOLVE_PLAIN, DeviceState, Mechanism, MicmHipError,
-                   rosenbrock_parameters)
+from ._lib import (SOLVE_COOPERATIVE, SOLVE_LOCKSTEP, SOLVE_PERSISTENT, SOLVE_PLAIN, DeviceState, Mechanism,
+                   MicmHipError, rosenbrock_parameters)
 
 __all__ = [
     "Species", "Phase", "System", "StoichSpecies", "Process", "ChemicalReactionBuilder",
@@ -332,7 +332,8 @@ class SolverResult:
 
 # ---------------------------------------------------------------- builder, solver, state
 
-_FORMS = {"lockstep": SOLVE_LOCKSTEP, "plain": SOLVE_PLAIN, "persistent": SOLVE_PERSISTENT}
+_FORMS = {"lockstep": SOLVE_LOCKSTEP, "plain": SOLVE_PLAIN, "persistent": SOLVE_PERSISTENT,
+          "cooperative": SOLVE_COOPERATIVE}
 
 
 class HipSolverBuilder:

@@ -17,7 +17,7 @@ GROUP = 64
 OK = 0
 STATE_NAMES = {0: "NotYetCalled", 1: "Running", 2: "Converged", 3: "ConvergenceExceededMaxSteps",
                4: "StepSizeTooSmall", 5: "RepeatedlySingularMatrix", 6: "NaNDetected", 7: "InfDetected"}
-SOLVE_LOCKSTEP, SOLVE_PLAIN, SOLVE_PERSISTENT = 0, 1, 2
+SOLVE_LOCKSTEP, SOLVE_PLAIN, SOLVE_PERSISTENT, SOLVE_COOPERATIVE = 0, 1, 2, 3
 PRESETS = {2: 2, 3: 3, 4: 4, "4da": 41, "6da": 61, 41: 41, 61: 61}
 
 _vp = ctypes.c_void_p
@@ -84,6 +84,8 @@ _SIGNATURES = {
     "micm_hip_state_set_tolerances": [_vp, ctypes.c_double, _dp],
     "micm_hip_state_get_results": [_vp, _i32p, _ip, _dp],
     "micm_hip_state_set_stream": [_vp, _vp],
+    "micm_hip_state_set_handoff": [_vp, _i64],
+    "micm_hip_mechanism_cooperative_supported": [_vp, ctypes.c_int32, _i32p],
     "micm_hip_update_state_parameters": [_vp, _vp],
     "micm_hip_solve": [_vp, _vp, ctypes.c_double, ctypes.POINTER(RosenbrockParameters), ctypes.c_int32],
 }
@@ -213,6 +215,11 @@ class Mechanism:
     def lu_solve(self, n_cells, lower, upper, x, stream=None):
         check(lib().micm_hip_lu_solve(self.h, n_cells, lower, upper, x, stream))
 
+    def cooperative_supported(self, stages):
+        out = ctypes.c_int32(0)
+        check(lib().micm_hip_mechanism_cooperative_supported(self.h, int(stages), ctypes.byref(out)))
+        return bool(out.value)
+
     def normalized_error(self, n_cells, y, ynew, yerr, atol, rtol, error, stream=None):
         check(lib().micm_hip_normalized_error(self.h, n_cells, y, ynew, yerr, atol, float(rtol), error, stream))
 
@@ -271,6 +278,11 @@ class DeviceState:
 
     def set_stream(self, stream):
         check(lib().micm_hip_state_set_stream(self.h, stream))
+
+    def set_handoff(self, steps):
+        """Step count at which the persistent form hands a cell to the cooperative form
+        (0: never, negative: library default)."""
+        check(lib().micm_hip_state_set_handoff(self.h, int(steps)))
 
     def results(self):
         states = np.zeros(self.n_cells, dtype=np.int32)

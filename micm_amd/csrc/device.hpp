@@ -44,17 +44,41 @@ namespace micm_hip
     const int32_t* lu_a;
     const int32_t* lu_div;
     const int32_t* lu_term_offsets;
-    const int32_t* lu_term_l;
-    const int32_t* lu_term_u;
+    const int2* lu_terms;   // (L index, U index)
     const int32_t* lower_diag;
     const int32_t* upper_diag;
     // substitution
     const int32_t* fwd_offsets;
-    const int32_t* fwd_mat;
-    const int32_t* fwd_col;
+    const int2* fwd_terms;  // (L index, column)
     const int32_t* bwd_offsets;
-    const int32_t* bwd_mat;
-    const int32_t* bwd_col;
+    const int2* bwd_terms;  // (U index, column)
+  };
+
+  // Tables of the cooperative (one wavefront per cell) form; see MechanismTables.
+  struct DevCoop
+  {
+    int32_t nnz_alu;
+    int32_t n_terms;
+    const int32_t* jac_to_alu;
+    const int32_t* alu_diag;
+    const int32_t* div_offsets;
+    const int32_t* div;
+    const int32_t* upd_offsets;
+    const int32_t* upd;  // triples
+    const int32_t* fcol_offsets;
+    const int2* fcol;    // (row, ALU position)
+    const int32_t* brow_offsets;
+    const int2* brow;    // (ALU position, column)
+    const uint32_t* fcol16;  // fcol / brow packed as 16-bit pairs (empty if positions exceed 16 bits)
+    const uint32_t* brow16;
+    const uint2* upd16;      // (target | L source << 16, U source)
+    int32_t n_fcol, n_brow;
+    const int32_t* spec_offsets;
+    const int32_t* spec_rxn;
+    const double* spec_coef;
+    const int32_t* jel_offsets;
+    const int32_t* jel_term;
+    const double* jel_coef;
   };
 
   // One lane's view of a grouped array: element v of this lane's cell.
@@ -86,6 +110,7 @@ namespace micm_hip
     double rtol;
     int32_t new_f[6];
     double a[15], c[15], m[6], e[6];
+    int64_t handoff_steps;  // per-lane form hands a cell to the cooperative form at this step count (0: never)
   };
 
   // Per-cell integrator state (rosenbrock.inl:75-78 plus SolverStats).
@@ -98,4 +123,8 @@ namespace micm_hip
   };
 
   constexpr int32_t kRejectLast = 1, kRejectMore = 2, kNewStep = 4;
+
+  // Status of a cell handed from the per-lane form to the cooperative form at a step
+  // boundary; never visible to callers.
+  constexpr int32_t kHandedOff = 100;
 }  // namespace micm_hip

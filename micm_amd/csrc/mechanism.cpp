@@ -135,6 +135,155 @@ namespace micm_hip
     }
   }  // namespace
 
+  namespace
+  {
+    void build_cooperative_schedule(MechanismTables& t)
+    {
+      const int32_t n = static_cast<int32_t>(t.n_species);
+      // in-place pattern: union of L (strict) and U, row by row
+      std::vector<std::vector<int32_t>> rows(n);
+      for (int32_t i = 0; i < n; ++i)
+      {
+        for (int32_t p = t.lower_row_ptr[i]; p < t.lower_row_ptr[i + 1]; ++p)
+          if (t.lower_col[p] < i)
+            rows[i].push_back(t.lower_col[p]);
+        for (int32_t p = t.upper_row_ptr[i]; p < t.upper_row_ptr[i + 1]; ++p)
+          rows[i].push_back(t.upper_col[p]);
+      }
+      to_csr(rows, t.alu_row_ptr, t.alu_col);
+      auto alu = [&](int32_t r, int32_t c) { return position(t.alu_row_ptr, t.alu_col, r, c); };
+      t.alu_diag.clear();
+      for (int32_t i = 0; i < n; ++i)
+        t.alu_diag.push_back(alu(i, i));
+      t.jac_to_alu.clear();
+      for (int32_t i = 0; i < n; ++i)
+        for (int32_t p = t.jac_row_ptr[i]; p < t.jac_row_ptr[i + 1]; ++p)
+          t.jac_to_alu.push_back(alu(i, t.jac_col[p]));
+      t.alu_lower.assign(t.lower_col.size(), -1);
+      t.alu_upper.assign(t.upper_col.size(), -1);
+      for (int32_t i = 0; i < n; ++i)
+      {
+        for (int32_t p = t.lower_row_ptr[i]; p < t.lower_row_ptr[i + 1]; ++p)
+          if (t.lower_col[p] < i)
+            t.alu_lower[p] = alu(i, t.lower_col[p]);
+        for (int32_t p = t.upper_row_ptr[i]; p < t.upper_row_ptr[i + 1]; ++p)
+          t.alu_upper[p] = alu(i, t.upper_col[p]);
+      }
+      // steps of the right-looking elimination
+      std::vector<std::vector<int32_t>> lcol(n), urow(n);
+      for (int32_t i = 0; i < n; ++i)
+      {
+        for (int32_t p = t.lower_row_ptr[i]; p < t.lower_row_ptr[i + 1]; ++p)
+          if (t.lower_col[p] < i)
+            lcol[t.lower_col[p]].push_back(i);
+        for (int32_t p = t.upper_row_ptr[i]; p < t.upper_row_ptr[i + 1]; ++p)
+          if (t.upper_col[p] > i)
+            urow[i].push_back(t.upper_col[p]);
+      }
+      t.coop_div_offsets.assign(1, 0);
+      t.coop_upd_offsets.assign(1, 0);
+      t.coop_div.clear();
+      t.coop_upd.clear();
+      for (int32_t j = 0; j < n; ++j)
+      {
+        for (int32_t k : lcol[j])
+          t.coop_div.push_back(alu(k, j));
+        t.coop_div_offsets.push_back(static_cast<int32_t>(t.coop_div.size()));
+        for (int32_t k : lcol[j])
+          for (int32_t m : urow[j])
+          {
+            t.coop_upd.push_back(alu(k, m));
+            t.coop_upd.push_back(alu(k, j));
+            t.coop_upd.push_back(alu(j, m));
+          }
+        t.coop_upd_offsets.push_back(static_cast<int32_t>(t.coop_upd.size() / 3));
+      }
+      // substitution
+      t.coop_fcol_offsets.assign(1, 0);
+      t.coop_brow_offsets.assign(1, 0);
+      t.coop_fcol.clear();
+      t.coop_brow.clear();
+      for (int32_t j = 0; j < n; ++j)
+      {
+        for (int32_t k : lcol[j])
+        {
+          t.coop_fcol.push_back(k);
+          t.coop_fcol.push_back(alu(k, j));
+        }
+        t.coop_fcol_offsets.push_back(static_cast<int32_t>(t.coop_fcol.size() / 2));
+      }
+      for (int32_t i = 0; i < n; ++i)
+      {
+        for (int32_t m : urow[i])
+        {
+          t.coop_brow.push_back(alu(i, m));
+          t.coop_brow.push_back(m);
+        }
+        t.coop_brow_offsets.push_back(static_cast<int32_t>(t.coop_brow.size() / 2));
+      }
+      t.coop_fcol16.clear();
+      t.coop_brow16.clear();
+      if (t.alu_col.size() < 65536 && n < 65536)
+      {
+        for (size_t q = 0; q < t.coop_fcol.size(); q += 2)
+          t.coop_fcol16.push_back(static_cast<uint32_t>(t.coop_fcol[q]) | (static_cast<uint32_t>(t.coop_fcol[q + 1]) << 16));
+        for (size_t q = 0; q < t.coop_brow.size(); q += 2)
+          t.coop_brow16.push_back(static_cast<uint32_t>(t.coop_brow[q]) | (static_cast<uint32_t>(t.coop_brow[q + 1]) << 16));
+        for (size_t q = 0; q < t.coop_upd.size(); q += 3)
+        {
+          t.coop_upd16.push_back(static_cast<uint32_t>(t.coop_upd[q]) | (static_cast<uint32_t>(t.coop_upd[q + 1]) << 16));
+          t.coop_upd16.push_back(static_cast<uint32_t>(t.coop_upd[q + 2]));
+        }
+      }
+      // forcing gather, in the scatter order of the reference (reactions ascending;
+      // within a reaction its reactant slots, then its product slots)
+      std::vector<std::vector<std::pair<int32_t, double>>> spec(n);
+      for (int32_t r = 0; r < (int32_t)t.n_reactions; ++r)
+      {
+        for (int32_t i = t.reactant_offsets[r]; i < t.reactant_offsets[r + 1]; ++i)
+          spec[t.reactant_ids[i]].emplace_back(r, -1.0);
+        for (int32_t i = t.product_offsets[r]; i < t.product_offsets[r + 1]; ++i)
+          spec[t.product_ids[i]].emplace_back(r, t.yields[i]);
+      }
+      t.spec_offsets.assign(1, 0);
+      t.spec_rxn.clear();
+      t.spec_coef.clear();
+      for (int32_t s = 0; s < n; ++s)
+      {
+        for (auto [r, cf] : spec[s])
+        {
+          t.spec_rxn.push_back(r);
+          t.spec_coef.push_back(cf);
+        }
+        t.spec_offsets.push_back(static_cast<int32_t>(t.spec_rxn.size()));
+      }
+      // Jacobian gather in term order (deps +1, independent +1, products -yield)
+      std::vector<std::vector<std::pair<int32_t, double>>> jel(t.jac_col.size());
+      for (int32_t term = 0; term < (int32_t)t.term_process.size(); ++term)
+      {
+        const int32_t p = t.term_process[term];
+        int32_t f = t.term_flat_offsets[term];
+        for (int32_t i = t.term_dep_offsets[term]; i < t.term_dep_offsets[term + 1]; ++i)
+          jel[t.term_flat[f++]].emplace_back(term, 1.0);
+        jel[t.term_flat[f++]].emplace_back(term, 1.0);
+        for (int32_t i = t.product_offsets[p]; i < t.product_offsets[p + 1]; ++i)
+          jel[t.term_flat[f++]].emplace_back(term, -t.yields[i]);
+      }
+      t.jel_offsets.assign(1, 0);
+      t.jel_term.clear();
+      t.jel_coef.clear();
+      for (auto& v : jel)
+      {
+        for (auto [tt, cf] : v)
+        {
+          t.jel_term.push_back(tt);
+          t.jel_coef.push_back(cf);
+        }
+        t.jel_offsets.push_back(static_cast<int32_t>(t.jel_term.size()));
+      }
+    }
+  }  // namespace
+
   MechanismTables compile_mechanism(const micm_hip_mechanism_desc& d)
   {
     if (d.n_species <= 0)
@@ -446,6 +595,22 @@ namespace micm_hip
           t.bwd_col.push_back(j);
         }
       t.bwd_offsets.push_back(static_cast<int32_t>(t.bwd_mat.size()));
+    }
+    build_cooperative_schedule(t);
+    for (size_t i = 0; i < t.lu_term_l.size(); ++i)
+    {
+      t.lu_term_pairs.push_back(t.lu_term_l[i]);
+      t.lu_term_pairs.push_back(t.lu_term_u[i]);
+    }
+    for (size_t i = 0; i < t.fwd_mat.size(); ++i)
+    {
+      t.fwd_pairs.push_back(t.fwd_mat[i]);
+      t.fwd_pairs.push_back(t.fwd_col[i]);
+    }
+    for (size_t i = 0; i < t.bwd_mat.size(); ++i)
+    {
+      t.bwd_pairs.push_back(t.bwd_mat[i]);
+      t.bwd_pairs.push_back(t.bwd_col[i]);
     }
     return t;
   }

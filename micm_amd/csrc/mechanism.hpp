@@ -57,11 +57,35 @@ namespace micm_hip
     std::vector<int32_t> lu_term_offsets; // n_elements + 1
     std::vector<int32_t> lu_term_l;       // L[i][j] (or L[k][j]) storage index, j ascending
     std::vector<int32_t> lu_term_u;       // U[j][k] (or U[j][i]) storage index
+    std::vector<int32_t> lu_term_pairs;   // interleaved (l, u) of the two arrays above
 
     // ---- substitution (linear_solver.inl:166-224)
     std::vector<int32_t> fwd_offsets, fwd_mat, fwd_col;   // row i: L[i][j], j ascending
     std::vector<int32_t> bwd_offsets, bwd_mat, bwd_col;   // row i (ascending storage): U[i][j], j ascending
+    std::vector<int32_t> fwd_pairs, bwd_pairs;            // interleaved (matrix index, column)
 
+    // ---- cooperative (one wavefront per cell) schedule over the in-place pattern
+    // ALU = strictly-lower L united with U (CSR, row-major).  The right-looking form
+    // applies, at step j, ALU[k][m] -= ALU[k][j] * ALU[j][m] for every (k, m) with
+    // L[k][j] and U[j][m] non-zero -- exactly Doolittle's terms, so each element
+    // receives its products in the same ascending-j order.
+    std::vector<int32_t> alu_row_ptr, alu_col, alu_diag;
+    std::vector<int32_t> jac_to_alu;                       // Jacobian element -> ALU position
+    std::vector<int32_t> alu_lower, alu_upper;             // L / U storage index -> ALU position
+    std::vector<int32_t> coop_div_offsets, coop_div;       // step j: ALU positions of L[k][j], k > j
+    std::vector<int32_t> coop_upd_offsets, coop_upd;       // step j: (target, L source, U source) triples
+    std::vector<int32_t> coop_fcol_offsets, coop_fcol;     // column j of L: (row i, ALU position) pairs
+    std::vector<int32_t> coop_brow_offsets, coop_brow;     // row i of U: (ALU position, column j) pairs, j ascending
+    std::vector<uint32_t> coop_fcol16, coop_brow16;        // the same pairs packed as two 16-bit fields (x | y << 16)
+    std::vector<uint32_t> coop_upd16;                      // triples packed as (target | L source << 16, U source)
+    // forcing as a per-species gather: f[s] += c * rate[r] over (r, c) in reaction order
+    std::vector<int32_t> spec_offsets, spec_rxn;
+    std::vector<double> spec_coef;
+    // -J as a per-element gather: J[e] += c * d[t] over (t, c) in the reference's order
+    std::vector<int32_t> jel_offsets, jel_term;
+    std::vector<double> jel_coef;
+
+    int64_t nnz_alu() const { return (int64_t)alu_col.size(); }
     int64_t nnz_jacobian() const { return (int64_t)jac_col.size(); }
     int64_t nnz_lower() const { return (int64_t)lower_col.size(); }
     int64_t nnz_upper() const { return (int64_t)upper_col.size(); }

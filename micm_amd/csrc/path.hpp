@@ -122,6 +122,35 @@ namespace micm_hip
     }
   }
 
+  // v - sum_t L[pairs[t].x] * U[pairs[t].y] for t in [b, e), subtracting in order.  The
+  // loads of eight terms are issued before their subtractions so that a lane keeps
+  // sixteen memory requests in flight; the subtraction order is unchanged.
+  __device__ __forceinline__ double subtract_products(double v, const int2* __restrict__ pairs, int b, int e,
+                                                      const Lane& X, const Lane& Y)
+  {
+    int t = b;
+    for (; t + 8 <= e; t += 8)
+    {
+      double x[8], y[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+      {
+        const int2 ij = pairs[t + q];
+        x[q] = X[ij.x];
+        y[q] = Y[ij.y];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        v -= x[q] * y[q];
+    }
+    for (; t < e; ++t)
+    {
+      const int2 ij = pairs[t];
+      v -= X[ij.x] * Y[ij.y];
+    }
+    return v;
+  }
+
   // Doolittle LU of A into separate L (unit diagonal stored) and U
   // (lu_decomposition_doolittle.inl:325-403); sums run over j ascending.
   __device__ inline void lu_decompose(const DevMech& m, const Lane& A, const Lane& L, const Lane& U)
@@ -132,9 +161,7 @@ namespace micm_hip
     {
       const int a = m.lu_a[e];
       double v = a >= 0 ? A[a] : 0.0;
-      const int tb = m.lu_term_offsets[e], te = m.lu_term_offsets[e + 1];
-      for (int t = tb; t < te; ++t)
-        v -= L[m.lu_term_l[t]] * U[m.lu_term_u[t]];
+      v = subtract_products(v, m.lu_terms, m.lu_term_offsets[e], m.lu_term_offsets[e + 1], L, U);
       const int div = m.lu_div[e];
       if (div < 0)
         U[m.lu_out[e]] = v;
@@ -151,20 +178,14 @@ namespace micm_hip
     const int n = m.n_species;
     for (int i = 0; i < n; ++i)
     {
-      double v = x[i];
-      const int b = m.fwd_offsets[i], e = m.fwd_offsets[i + 1];
-      for (int t = b; t < e; ++t)
-        v -= L[m.fwd_mat[t]] * x[m.fwd_col[t]];
+      double v = subtract_products(x[i], m.fwd_terms, m.fwd_offsets[i], m.fwd_offsets[i + 1], L, x);
       if (!unit_lower)
         v /= L[m.lower_diag[i]];
       x[i] = v;
     }
     for (int i = n - 1; i >= 0; --i)
     {
-      double v = x[i];
-      const int b = m.bwd_offsets[i], e = m.bwd_offsets[i + 1];
-      for (int t = b; t < e; ++t)
-        v -= U[m.bwd_mat[t]] * x[m.bwd_col[t]];
+      const double v = subtract_products(x[i], m.bwd_terms, m.bwd_offsets[i], m.bwd_offsets[i + 1], U, x);
       x[i] = v / U[m.upper_diag[i]];
     }
   }

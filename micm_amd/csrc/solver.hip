@@ -11,6 +11,7 @@
 //   PLAIN      -- per-cell step size; one launch per kernel per stage, host loop.
 //   PERSISTENT -- per-cell step size; one persistent launch whose wavefronts take
 //                 cells from a work queue and carry each to the end of its step.
+#include "coop.hpp"
 #include "device.hpp"
 #include "mechanism.hpp"
 #include "path.hpp"
@@ -487,7 +488,9 @@ namespace
   __global__ void __launch_bounds__(kBlock) k_persistent(DevMech m, DevParams p, int64_t n_cells, Queue q, Work slot,
                                                          double* __restrict__ y_state, const double* __restrict__ k_state,
                                                          const double* __restrict__ atol, int32_t* status_out,
-                                                         int64_t* stats_out, double* final_time_out)
+                                                         int64_t* stats_out, double* final_time_out,
+                                                         CellControl* handoff_control, int32_t* handoff_list,
+                                                         int* handoff_count)
   {
     const int l = threadIdx.x % kGroup;
     const size_t s = static_cast<size_t>(blockIdx.x) * (kBlock / kGroup) + threadIdx.x / kGroup;
@@ -533,6 +536,16 @@ namespace
       }
       if (!__any(cell >= 0))
         break;
+      if (cell >= 0 && p.handoff_steps > 0 && (cc.flags & kNewStep) && cc.stats[MICM_HIP_STAT_STEPS] >= p.handoff_steps)
+      {
+        // a slow cell: park it at this step boundary for the cooperative form
+        Lane ys = lane_view(y_state, cell / kGroup, n, cell % kGroup);
+        for (int i = 0; i < n; ++i)
+          ys[i] = y[i];
+        handoff_control[cell] = cc;
+        handoff_list[atomicAdd(handoff_count, 1)] = static_cast<int32_t>(cell);
+        cell = -1;
+      }
       if (cell >= 0)
       {
         bool running = factor_attempt(m, p, y, k, f0, J, L, U, cc);
@@ -561,6 +574,170 @@ namespace
       }
     }
   }
+
+  // ---- cooperative form: one wavefront per cell, working set in LDS
+
+  __global__ void __launch_bounds__(kCoopThreads) k_coop(DevMech m, DevCoop c, DevParams p, int tables_in_lds, int64_t n_fixed,
+                                                         const int32_t* __restrict__ list, const int* list_count,
+                                                         int* next, const CellControl* resume,
+                                                         double* __restrict__ y_state, const double* __restrict__ k_state,
+                                                         const double* __restrict__ atol, int32_t* status_out,
+                                                         int64_t* stats_out, double* final_time_out)
+  {
+    extern __shared__ double lds[];
+    __shared__ int s_idx;
+    __shared__ double s_error;
+    const int lane = threadIdx.x;
+    const int n = m.n_species;
+    const CoopLds L = coop_layout(lds, m, c, p.stages);
+    const int64_t count = list ? static_cast<int64_t>(*list_count) : n_fixed;
+    // the substitution tables are the same for every cell: stage them once
+    CoopSolveTables T{ c.fcol16, c.brow16 };
+    if (tables_in_lds)
+    {
+      uint32_t* t = reinterpret_cast<uint32_t*>(lds + coop_lds_doubles(c.nnz_alu, m.nnz_jac, n, p.stages,
+                                                                         m.n_reactions, c.n_terms));
+      for (int q = lane; q < c.n_fcol; q += kCoopThreads)
+        t[q] = c.fcol16[q];
+      for (int q = lane; q < c.n_brow; q += kCoopThreads)
+        t[c.n_fcol + q] = c.brow16[q];
+      T.fcol = t;
+      T.brow = t + c.n_fcol;
+      coop_sync();
+    }
+    while (true)
+    {
+      if (lane == 0)
+        s_idx = atomicAdd(next, 1);
+      coop_sync();
+      const int64_t idx = s_idx;
+      coop_sync();
+      if (idx >= count)
+        return;
+      const int64_t cell = list ? list[idx] : idx;
+      const size_t g = cell / kGroup;
+      const int cl = cell % kGroup;
+      {
+        Lane ys = lane_view(y_state, g, n, cl);
+        Lane ks = lane_view(const_cast<double*>(k_state), g, m.n_reactions, cl);
+        for (int i = lane; i < n; i += kCoopThreads)
+          L.y[i] = ys[i];
+        for (int r = lane; r < m.n_reactions; r += kCoopThreads)
+          L.k[r] = ks[r];
+      }
+      CellControl cc;
+      if (resume)
+        cc = resume[cell];
+      else
+        control_init(p, cc);
+      coop_sync();
+      while (true)
+      {
+        if (cc.flags & kNewStep)
+        {
+          if (!begin_step(p, cc))
+            break;
+          coop_forcing(m, c, L, L.y, L.f0);
+          cc.stats[MICM_HIP_STAT_FUNCTION_CALLS] += 1;
+          coop_jacobian(m, c, L);
+          cc.stats[MICM_HIP_STAT_JACOBIAN_UPDATES] += 1;
+          cc.last_alpha = 0.0;
+          cc.flags &= ~kNewStep;
+        }
+        const double alpha = next_alpha(p, cc);
+        for (int i = lane; i < n; i += kCoopThreads)
+          L.jac[m.jac_diag[i]] += alpha * 1.0;
+        coop_sync();
+        coop_decompose(m, c, L);
+        cc.stats[MICM_HIP_STAT_DECOMPOSITIONS] += 1;
+        for (int s = 0; s < p.stages; ++s)
+        {
+          const int sc = s * (s - 1) / 2;
+          double* Ks = L.K + static_cast<size_t>(s) * n;
+          if (s == 0)
+          {
+            for (int i = lane; i < n; i += kCoopThreads)
+              Ks[i] = L.f0[i];
+          }
+          else if (p.new_f[s])
+          {
+            for (int i = lane; i < n; i += kCoopThreads)
+            {
+              double v = L.y[i];
+              for (int j = 0; j < s; ++j)
+                v += p.a[sc + j] * L.K[static_cast<size_t>(j) * n + i];
+              L.ynew[i] = v;
+            }
+            coop_sync();
+            coop_forcing(m, c, L, L.ynew, Ks);
+            cc.stats[MICM_HIP_STAT_FUNCTION_CALLS] += 1;
+          }
+          coop_sync();
+          if (s + 1 < p.stages && !p.new_f[s + 1])
+            for (int i = lane; i < n; i += kCoopThreads)
+              Ks[static_cast<size_t>(n) + i] = Ks[i];
+          if (s > 0)
+          {
+            double coef[6];
+            for (int j = 0; j < s; ++j)
+              coef[j] = p.c[sc + j] / cc.H;
+            for (int i = lane; i < n; i += kCoopThreads)
+            {
+              double v = Ks[i];
+              for (int j = 0; j < s; ++j)
+                v += coef[j] * L.K[static_cast<size_t>(j) * n + i];
+              Ks[i] = v;
+            }
+          }
+          coop_sync();
+          coop_solve(m, c, L, T, Ks);
+          cc.stats[MICM_HIP_STAT_SOLVES] += 1;
+        }
+        for (int i = lane; i < n; i += kCoopThreads)
+        {
+          double v = L.y[i], e = 0.0;
+          for (int s = 0; s < p.stages; ++s)
+            v += p.m[s] * L.K[static_cast<size_t>(s) * n + i];
+          for (int s = 0; s < p.stages; ++s)
+            e += p.e[s] * L.K[static_cast<size_t>(s) * n + i];
+          L.ynew[i] = v;
+          L.yerr[i] = e;
+          const double a = fabs(L.y[i]), b = fabs(v);
+          const double q = e / (atol[i] + p.rtol * (a > b ? a : b));
+          L.scratch[i] = q * q;
+        }
+        coop_sync();
+        if (lane == 0)
+        {
+          double acc = 0.0;
+          for (int i = 0; i < n; ++i)
+            acc += L.scratch[i];
+          s_error = smax(sqrt(acc / static_cast<double>(n > 0 ? n : 1)), 1.0e-10);
+        }
+        coop_sync();
+        const double error = s_error;
+        if (control_update(p, cc, error))
+          for (int i = lane; i < n; i += kCoopThreads)
+            L.y[i] = L.ynew[i];
+        coop_sync();
+        if (cc.status != MICM_HIP_STATE_RUNNING)
+          break;
+      }
+      {
+        Lane ys = lane_view(y_state, g, n, cl);
+        for (int i = lane; i < n; i += kCoopThreads)
+          ys[i] = smax(L.y[i], 0.0);
+      }
+      if (lane == 0)
+      {
+        status_out[cell] = cc.status;
+        for (int i = 0; i < 7; ++i)
+          stats_out[cell * 7 + i] = cc.stats[i];
+        final_time_out[cell] = cc.t;
+      }
+      coop_sync();
+    }
+  }
 }  // namespace
 
 // ====================================================================== C-ABI
@@ -569,6 +746,7 @@ struct micm_hip_mechanism
 {
   MechanismTables t;
   DevMech dev{};
+  DevCoop coop{};
   void* blob = nullptr;  // one device allocation for every table
   int device = 0;
 };
@@ -596,6 +774,11 @@ struct micm_hip_state
   double* slot_blob = nullptr;
   int32_t* order = nullptr;  // persistent processing order
   bool has_order = false;
+  // hand-off to the cooperative form
+  int64_t handoff_steps = -1;
+  int32_t* handoff_list = nullptr;
+  int* handoff_count = nullptr;
+  int* coop_next = nullptr;
 };
 
 namespace
@@ -611,6 +794,14 @@ namespace
       std::memcpy(staging.data() + off, v.data(), bytes);
     dst = reinterpret_cast<const T*>(static_cast<char*>(blob) + off);
     off += bytes;
+    return 0;
+  }
+
+  int upload_pairs(void* blob, size_t& off, const std::vector<int32_t>& v, const int2*& dst, std::vector<char>& staging)
+  {
+    const int32_t* p = nullptr;
+    upload(blob, off, v, p, staging);
+    dst = reinterpret_cast<const int2*>(p);
     return 0;
   }
 
@@ -784,10 +975,17 @@ namespace
              padded_bytes(t.yields) + padded_bytes(t.term_process) + padded_bytes(t.term_dep_offsets) +
              padded_bytes(t.term_dep_ids) + padded_bytes(t.term_flat_offsets) + padded_bytes(t.term_flat) +
              padded_bytes(t.jac_diag) + padded_bytes(t.lu_out) + padded_bytes(t.lu_a) + padded_bytes(t.lu_div) +
-             padded_bytes(t.lu_term_offsets) + padded_bytes(t.lu_term_l) + padded_bytes(t.lu_term_u) +
+             padded_bytes(t.lu_term_offsets) + padded_bytes(t.lu_term_pairs) +
              padded_bytes(t.lower_diag) + padded_bytes(t.upper_diag) + padded_bytes(t.fwd_offsets) +
-             padded_bytes(t.fwd_mat) + padded_bytes(t.fwd_col) + padded_bytes(t.bwd_offsets) + padded_bytes(t.bwd_mat) +
-             padded_bytes(t.bwd_col) + 256;
+             padded_bytes(t.fwd_pairs) + padded_bytes(t.bwd_offsets) + padded_bytes(t.bwd_pairs) +
+             padded_bytes(t.jac_to_alu) + padded_bytes(t.alu_diag) + padded_bytes(t.coop_div_offsets) +
+             padded_bytes(t.coop_div) + padded_bytes(t.coop_upd_offsets) + padded_bytes(t.coop_upd) +
+             padded_bytes(t.coop_fcol_offsets) + padded_bytes(t.coop_fcol) + padded_bytes(t.coop_brow_offsets) +
+             padded_bytes(t.coop_brow) + padded_bytes(t.coop_fcol16) + padded_bytes(t.coop_brow16) +
+             padded_bytes(t.coop_upd16) +
+             padded_bytes(t.spec_offsets) + padded_bytes(t.spec_rxn) +
+             padded_bytes(t.spec_coef) + padded_bytes(t.jel_offsets) + padded_bytes(t.jel_term) +
+             padded_bytes(t.jel_coef) + 256;
     HIP_TRY(hipMalloc(&m->blob, bytes));
     std::vector<char> staging;
     size_t off = 0;
@@ -820,19 +1018,101 @@ namespace
     upload(m->blob, off, t.lu_a, d.lu_a, staging);
     upload(m->blob, off, t.lu_div, d.lu_div, staging);
     upload(m->blob, off, t.lu_term_offsets, d.lu_term_offsets, staging);
-    upload(m->blob, off, t.lu_term_l, d.lu_term_l, staging);
-    upload(m->blob, off, t.lu_term_u, d.lu_term_u, staging);
+    upload_pairs(m->blob, off, t.lu_term_pairs, d.lu_terms, staging);
     upload(m->blob, off, t.lower_diag, d.lower_diag, staging);
     upload(m->blob, off, t.upper_diag, d.upper_diag, staging);
     upload(m->blob, off, t.fwd_offsets, d.fwd_offsets, staging);
-    upload(m->blob, off, t.fwd_mat, d.fwd_mat, staging);
-    upload(m->blob, off, t.fwd_col, d.fwd_col, staging);
+    upload_pairs(m->blob, off, t.fwd_pairs, d.fwd_terms, staging);
     upload(m->blob, off, t.bwd_offsets, d.bwd_offsets, staging);
-    upload(m->blob, off, t.bwd_mat, d.bwd_mat, staging);
-    upload(m->blob, off, t.bwd_col, d.bwd_col, staging);
+    upload_pairs(m->blob, off, t.bwd_pairs, d.bwd_terms, staging);
+    DevCoop& c = m->coop;
+    c.nnz_alu = static_cast<int32_t>(t.nnz_alu());
+    c.n_terms = static_cast<int32_t>(t.term_process.size());
+    upload(m->blob, off, t.jac_to_alu, c.jac_to_alu, staging);
+    upload(m->blob, off, t.alu_diag, c.alu_diag, staging);
+    upload(m->blob, off, t.coop_div_offsets, c.div_offsets, staging);
+    upload(m->blob, off, t.coop_div, c.div, staging);
+    upload(m->blob, off, t.coop_upd_offsets, c.upd_offsets, staging);
+    upload(m->blob, off, t.coop_upd, c.upd, staging);
+    upload(m->blob, off, t.coop_fcol_offsets, c.fcol_offsets, staging);
+    upload_pairs(m->blob, off, t.coop_fcol, c.fcol, staging);
+    upload(m->blob, off, t.coop_brow_offsets, c.brow_offsets, staging);
+    upload_pairs(m->blob, off, t.coop_brow, c.brow, staging);
+    c.n_fcol = static_cast<int32_t>(t.coop_fcol16.size());
+    c.n_brow = static_cast<int32_t>(t.coop_brow16.size());
+    if (!t.coop_upd16.empty() || t.coop_upd.empty())
+    {
+      upload(m->blob, off, t.coop_fcol16, c.fcol16, staging);
+      upload(m->blob, off, t.coop_brow16, c.brow16, staging);
+      const uint32_t* u16 = nullptr;
+      upload(m->blob, off, t.coop_upd16, u16, staging);
+      c.upd16 = reinterpret_cast<const uint2*>(u16);
+    }
+    else
+    {
+      c.fcol16 = nullptr;
+      c.brow16 = nullptr;
+      c.upd16 = nullptr;
+    }
+    upload(m->blob, off, t.spec_offsets, c.spec_offsets, staging);
+    upload(m->blob, off, t.spec_rxn, c.spec_rxn, staging);
+    upload(m->blob, off, t.spec_coef, c.spec_coef, staging);
+    upload(m->blob, off, t.jel_offsets, c.jel_offsets, staging);
+    upload(m->blob, off, t.jel_term, c.jel_term, staging);
+    upload(m->blob, off, t.jel_coef, c.jel_coef, staging);
     if (off > bytes)
       return fail(MICM_HIP_ERR_DEVICE, "internal table sizing error");
     HIP_TRY(hipMemcpy(m->blob, staging.data(), off, hipMemcpyHostToDevice));
+    return MICM_HIP_OK;
+  }
+
+  // Dynamic LDS of the cooperative kernel for this mechanism, or 0 if it does not
+  // fit one workgroup on the current device.
+  size_t coop_lds_bytes(const micm_hip_mechanism* m, int stages, bool* tables_in_lds = nullptr)
+  {
+    const size_t bytes = coop_lds_doubles(m->coop.nnz_alu, m->dev.nnz_jac, m->dev.n_species, stages,
+                                          m->dev.n_reactions, m->coop.n_terms) *
+                         sizeof(double);
+    int dev = 0, max_lds = 0, optin = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+      return 0;
+    (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev);
+    const size_t limit = static_cast<size_t>(std::max(max_lds, optin)) - 64;  // static __shared__ of the kernel
+    if (bytes > limit || !m->coop.fcol16 || !m->coop.brow16 || !m->coop.upd16)
+      return 0;
+    const size_t with_tables = bytes + coop_table_bytes(m->coop.n_fcol, m->coop.n_brow);
+    const bool packed = m->coop.fcol16 != nullptr && m->coop.brow16 != nullptr && m->coop.upd16 != nullptr;
+    const bool fits = packed && m->coop.n_fcol >= 0 && with_tables <= limit;
+    if (tables_in_lds)
+      *tables_in_lds = fits;
+    return fits ? with_tables : bytes;
+  }
+
+  int launch_coop(micm_hip_state* s, const DevParams& p, const int32_t* list, const int* count, int64_t n_fixed,
+                  const CellControl* resume, hipStream_t st)
+  {
+    const micm_hip_mechanism* m = s->mech;
+    bool tables_in_lds = false;
+    const size_t lds = coop_lds_bytes(m, p.stages, &tables_in_lds);
+    if (lds == 0)
+      return fail(MICM_HIP_ERR_UNSUPPORTED, "the cooperative form does not fit this mechanism in LDS");
+    if (!tables_in_lds && (!m->coop.fcol16 || !m->coop.brow16 || !m->coop.upd16))
+      return fail(MICM_HIP_ERR_UNSUPPORTED, "the cooperative form needs ALU positions below 65536");
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_coop), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                static_cast<int>(lds)));
+    int dev = 0, cus = 0, per_cu = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coop, kCoopThreads, lds));
+    int64_t blocks = static_cast<int64_t>(cus) * std::max(per_cu, 1);
+    if (!list)
+      blocks = std::max<int64_t>(1, std::min(blocks, n_fixed));
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, s->coop_next);
+    hipLaunchKernelGGL(k_coop, dim3(static_cast<unsigned>(blocks)), dim3(kCoopThreads), lds, st, m->dev, m->coop, p,
+                       tables_in_lds ? 1 : 0, n_fixed, list, count, s->coop_next, resume, s->y, s->k, s->atol, s->status, s->stats,
+                       s->final_time);
+    HIP_TRY(hipGetLastError());
     return MICM_HIP_OK;
   }
 }  // namespace
@@ -1107,6 +1387,9 @@ extern "C"
     HIP_TRY(hipMalloc(&s->active, sizeof(unsigned int)));
     HIP_TRY(hipMalloc(&s->queue_next, sizeof(int)));
     HIP_TRY(hipMalloc(&s->order, sizeof(int32_t) * n_cells));
+    HIP_TRY(hipMalloc(&s->handoff_list, sizeof(int32_t) * n_cells));
+    HIP_TRY(hipMalloc(&s->handoff_count, sizeof(int)));
+    HIP_TRY(hipMalloc(&s->coop_next, sizeof(int)));
     HIP_TRY(hipHostMalloc(&s->host_active, sizeof(unsigned int)));
     HIP_TRY(hipMemcpy(s->atol, m->t.absolute_tolerances.data(), sizeof(double) * d.n_species, hipMemcpyHostToDevice));
     *out = s.release();
@@ -1133,6 +1416,12 @@ extern "C"
       (void)hipFree(s->queue_next);
     if (s->order)
       (void)hipFree(s->order);
+    if (s->handoff_list)
+      (void)hipFree(s->handoff_list);
+    if (s->handoff_count)
+      (void)hipFree(s->handoff_count);
+    if (s->coop_next)
+      (void)hipFree(s->coop_next);
     if (s->host_active)
       (void)hipHostFree(s->host_active);
     delete s;
@@ -1220,6 +1509,24 @@ extern "C"
     return MICM_HIP_OK;
   }
 
+  int micm_hip_state_set_handoff(micm_hip_state* s, int64_t steps)
+  {
+    if (!s)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null state");
+    s->handoff_steps = steps;
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_mechanism_cooperative_supported(const micm_hip_mechanism* m, int32_t stages, int32_t* out)
+  {
+    if (int rc = check_mech(m))
+      return rc;
+    if (!out || stages < 1 || stages > 6)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "bad argument");
+    *out = coop_lds_bytes(m, stages) > 0 ? 1 : 0;
+    return MICM_HIP_OK;
+  }
+
   int micm_hip_update_state_parameters(const micm_hip_mechanism* m, micm_hip_state* s)
   {
     if (int rc = check_mech(m))
@@ -1296,11 +1603,26 @@ extern "C"
       if (int rc = ensure_slots(s, blocks * (kBlock / kGroup), p.stages))
         return rc;
       Work w = slot_work(s);
+      DevParams pp = p;
+      const bool coop_ok = coop_lds_bytes(m, p.stages) > 0;
+      pp.handoff_steps = !coop_ok ? 0 : (s->handoff_steps >= 0 ? s->handoff_steps : 48);
       hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, s->queue_next);
+      hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, s->handoff_count);
       Queue q{ s->queue_next, s->has_order ? s->order : nullptr };
-      hipLaunchKernelGGL(k_persistent, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, st, d, p, s->n_cells, q, w,
-                         s->y, s->k, s->atol, s->status, s->stats, s->final_time);
+      hipLaunchKernelGGL(k_persistent, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, st, d, pp, s->n_cells, q, w,
+                         s->y, s->k, s->atol, s->status, s->stats, s->final_time, s->ctl, s->handoff_list,
+                         s->handoff_count);
       HIP_TRY(hipGetLastError());
+      if (pp.handoff_steps > 0)
+        if (int rc = launch_coop(s, p, s->handoff_list, s->handoff_count, 0, s->ctl, st))
+          return rc;
+      HIP_TRY(hipStreamSynchronize(st));
+      return MICM_HIP_OK;
+    }
+    if (form == MICM_HIP_SOLVE_COOPERATIVE)
+    {
+      if (int rc = launch_coop(s, p, nullptr, nullptr, s->n_cells, nullptr, st))
+        return rc;
       HIP_TRY(hipStreamSynchronize(st));
       return MICM_HIP_OK;
     }

@@ -115,10 +115,11 @@ def test_kernels_match_oracle(case, n_cells):
     assert bits_equal(e_dev.cpu().numpy(), e_ref)
 
 
-def _solve(mech, cells, order, tableau, dt, form, n_solves=1, rate_constants=None):
+def _solve(mech, cells, order, tableau, dt, form, n_solves=1, rate_constants=None, handoff=-1):
     """n_solves x (UpdateStateParameters, Solve); with rate_constants given, those are
     loaded instead of being computed on the device."""
     state = _lib.DeviceState(mech, cells.n_cells)
+    state.set_handoff(handoff)
     state.set_variables(cells.variables[:, order])
     state.set_conditions(cells.conditions)
     state.set_custom_parameters(cells.custom)
@@ -139,7 +140,10 @@ def _n_cells(name):
     return 200 if name == "210_560" else 515
 
 
-@pytest.mark.parametrize("form", [_lib.SOLVE_PLAIN, _lib.SOLVE_PERSISTENT])
+FORMS = [_lib.SOLVE_PLAIN, _lib.SOLVE_PERSISTENT, _lib.SOLVE_COOPERATIVE]
+
+
+@pytest.mark.parametrize("form", FORMS)
 def test_per_cell_solve_exact_with_shared_rate_constants(case, form):
     """Given the oracle's rate constants, every cell (failed ones included) takes the
     oracle's exact step sequence: identical outcomes, counters and concentrations."""
@@ -157,7 +161,7 @@ def test_per_cell_solve_exact_with_shared_rate_constants(case, form):
     assert bits_equal(y_gpu, y_ref)
 
 
-@pytest.mark.parametrize("form", [_lib.SOLVE_PLAIN, _lib.SOLVE_PERSISTENT])
+@pytest.mark.parametrize("form", FORMS)
 def test_per_cell_solve_matches_oracle(case, form):
     """End to end (rate constants from the device libm): species above 1e-20 agree to
     1e-8 relative in every cell both solvers converge; at most 2 % of cells (those the
@@ -181,16 +185,25 @@ def test_per_cell_solve_matches_oracle(case, form):
           f"{np.mean(stats_gpu[:, 2] == stats_ref[:, 2]):.4f}, max rel {max_rel(y_gpu[both], y_ref[both], 1e-20):.2e}")
 
 
-def test_plain_and_persistent_forms_agree():
-    """Both per-cell forms run the same per-cell arithmetic: bit-identical results."""
-    desc = synthetic.mechanism_60_200()
+@pytest.mark.parametrize("name", ["60_200", "210_560"])
+def test_per_cell_forms_agree_bitwise(name):
+    """Every per-cell form runs the same per-cell arithmetic (the cooperative form
+    with its work spread over a wavefront without reordering any sum), so plain,
+    persistent (with and without hand-off) and cooperative results are identical."""
+    desc, inputs, tableau, dt = CASES[name]
     mech = _lib.Mechanism(desc)
-    cells = synthetic.inputs_60_200(desc, 1000, seed=21)
+    cells = inputs(desc, 700 if name == "60_200" else 130, seed=21)
     order = mech.species_order()
-    a = _solve(mech, cells, order, 4, 300.0, _lib.SOLVE_PLAIN)
-    b = _solve(mech, cells, order, 4, 300.0, _lib.SOLVE_PERSISTENT)
-    assert bits_equal(a[0], b[0])
-    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    print(f"forms agree: {name}", flush=True)
+    ref = _solve(mech, cells, order, tableau, dt, _lib.SOLVE_PLAIN)
+    runs = [_solve(mech, cells, order, tableau, dt, _lib.SOLVE_PERSISTENT, handoff=0),
+            _solve(mech, cells, order, tableau, dt, _lib.SOLVE_PERSISTENT, handoff=3)]
+    if mech.cooperative_supported(tableau):
+        runs.append(_solve(mech, cells, order, tableau, dt, _lib.SOLVE_COOPERATIVE))
+    for r in runs:
+        assert bits_equal(r[0], ref[0])
+        assert np.array_equal(r[1], ref[1]) and np.array_equal(r[2], ref[2])
+        assert bits_equal(r[3], ref[3])
 
 
 def test_lockstep_solve_matches_oracle():

@@ -1,0 +1,114 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports every
+function of include/micm_hip.h, the mechanism compiler reproduces the
+reference's species order and sparsity patterns, the API mirror builds the same
+description as the flat generators, the seeded generators are stable, and bad
+mechanisms are rejected with status codes."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from micm_amd import _lib, synthetic
+from micm_amd import (ArrheniusRateConstantParameters, ChemicalReactionBuilder, HipSolverBuilder, Phase,
+                      RosenbrockSolverParameters, Species, StoichSpecies, System)
+from micm_amd._desc import ARRHENIUS, MechanismDesc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "micm_hip.h")).read()
+    return sorted(set(re.findall(r"^(?:int|void|const char\*)\s+(micm_hip_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_function():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = header_functions()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+@pytest.mark.parametrize("name,fn", [("readme", synthetic.readme_mechanism), ("60_200", synthetic.mechanism_60_200),
+                                     ("210_560", synthetic.mechanism_210_560)])
+def test_mechanism_compiler_matches_reference(name, fn):
+    g = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    m = _lib.Mechanism(fn())  # host compilation only; nothing touches the device
+    assert np.array_equal(m.species_order(), g["species_order"])
+    p = m.patterns()
+    for key, (r, c) in (("jac", p["jacobian"]), ("lower", p["lower"]), ("upper", p["upper"])):
+        assert np.array_equal(r, g[f"{key}_rows"]) and np.array_equal(c, g[f"{key}_cols"])
+    d = m.jacobian_diagonal()
+    assert np.array_equal(p["jacobian"][0][d], np.arange(m.n_species))
+    assert np.array_equal(p["jacobian"][1][d], np.arange(m.n_species))
+
+
+def test_reaction_order_is_stable_type_sort():
+    desc = synthetic.mechanism_60_200()
+    m = _lib.Mechanism(desc)
+    assert list(m.reaction_order()) == desc.sorted_reaction_order()
+
+
+def test_api_mirror_builds_the_readme_description():
+    foo, bar, baz = Species("Foo"), Species("Bar"), Species("Baz")
+    gas = Phase("gas", [foo, bar, baz])
+    r1 = (ChemicalReactionBuilder().SetReactants([foo]).SetProducts([StoichSpecies(bar, 0.8), StoichSpecies(baz, 0.2)])
+          .SetRateConstant(ArrheniusRateConstantParameters(A=1.0e-3)).SetPhase(gas).Build())
+    r2 = (ChemicalReactionBuilder().SetReactants([foo, bar]).SetProducts([StoichSpecies(baz, 1)])
+          .SetRateConstant(ArrheniusRateConstantParameters(A=1.0e-5, C=110.0)).SetPhase(gas).Build())
+    builder = HipSolverBuilder(RosenbrockSolverParameters.ThreeStageRosenbrockParameters())
+    desc, _ = builder.SetSystem(System(gas)).SetReactions([r1, r2]).description()
+    ref = synthetic.readme_mechanism()
+    for a in ("reactant_offsets", "reactants", "product_offsets", "products", "yields", "rate_types", "rate_params"):
+        assert np.array_equal(getattr(desc, a), getattr(ref, a)), a
+
+
+def test_tableau_presets():
+    p3 = _lib.rosenbrock_parameters(3)
+    assert p3.stages == 3 and p3.max_number_of_steps == 1000 and p3.factor_max == 6.0
+    p4 = _lib.rosenbrock_parameters(4)
+    assert p4.stages == 4 and list(p4.new_function_evaluation[:4]) == [1, 1, 1, 0]
+    with pytest.raises(_lib.MicmHipError):
+        _lib.check(_lib.lib().micm_hip_rosenbrock_parameters_preset(5, ctypes.byref(_lib.RosenbrockParameters())))
+
+
+def test_synthetic_generators_are_stable():
+    d = synthetic.mechanism_210_560()
+    assert d.n_species == 210 and d.n_reactions == 560
+    assert np.bincount(d.rate_types, minlength=8)[[0, 1, 2, 7]].tolist() == [420, 56, 28, 56]
+    assert int(d.reactants.sum()) == 84397 and abs(float(d.yields.sum()) - 1165.48035429837) < 1e-9
+    cells = synthetic.inputs_210_560(d, 1000)
+    assert abs(cells.night.mean() - 0.5) < 0.05
+    assert np.all(cells.custom[cells.night] == 0) and np.all(cells.custom[~cells.night] > 0)
+    assert 180 <= cells.conditions[:, 0].min() and cells.conditions[:, 0].max() <= 320
+    again = synthetic.inputs_210_560(d, 1000)
+    assert np.array_equal(again.variables, cells.variables)
+
+
+def test_xoshiro_matches_published_algorithm():
+    """xoshiro256** seeded by SplitMix64 from 0: the first outputs of the published
+    C implementation (Blackman & Vigna), and per-cell streams equal scalar streams."""
+    r = synthetic.Xoshiro256ss(0)
+    assert [r.next_u64() for _ in range(3)] == [11091344671253066420, 13793997310169335082, 1900383378846508768]
+    cs = synthetic.CellStreams(77, 5)
+    draws = np.stack([cs.next_u64() for _ in range(4)], 1)
+    for c in range(5):
+        sc = synthetic.Xoshiro256ss((77 + (c + 1) * synthetic.GOLDEN) & synthetic.M64)
+        assert [int(v) for v in draws[c]] == [sc.next_u64() for _ in range(4)]
+
+
+def test_bad_mechanisms_are_rejected():
+    bad_species = MechanismDesc(2, [([0], [(5, 1.0)], ARRHENIUS, [1.0])])
+    with pytest.raises(_lib.MicmHipError) as e:
+        _lib.Mechanism(bad_species)
+    assert e.value.code == -5
+    unused = MechanismDesc(3, [([0], [(1, 1.0)], ARRHENIUS, [1.0])], ignore_unused_species=False)
+    with pytest.raises(_lib.MicmHipError):
+        _lib.Mechanism(unused)
+    bad_type = MechanismDesc(2, [([0], [(1, 1.0)], 42, [1.0])])
+    with pytest.raises(_lib.MicmHipError) as e:
+        _lib.Mechanism(bad_type)
+    assert e.value.code == -1
