@@ -1,10 +1,12 @@
 // path.hpp -- per-lane device functions of the Rosenbrock path.  Each
 // reproduces the arithmetic order of the reference function it replaces, so
 // that with FMA contraction disabled the results are bit-identical to the
-// reference's CPU solver for +, -, *, / (transcendentals use the device libm).
+// reference's CPU solver for +, -, *, / and sqrt; exp, log10 and pow come from
+// pmath.h, whose results are the same on the host and on the device.
 #pragma once
 
 #include "device.hpp"
+#include "pmath.h"
 
 #include "../../include/micm_hip.h"
 
@@ -30,37 +32,37 @@ namespace micm_hip
     double k = 0.0;
     switch (m.rate_type[r])
     {
-      case MICM_HIP_RATE_ARRHENIUS: k = p[0] * exp(p[2] / T) * pow(T / p[3], p[1]) * (1.0 + p[4] * P); break;
+      case MICM_HIP_RATE_ARRHENIUS: k = p[0] * pmath::exp(p[2] / T) * pmath::pow(T / p[3], p[1]) * (1.0 + p[4] * P); break;
       case MICM_HIP_RATE_TROE:
       case MICM_HIP_RATE_TERNARY:
       {
         const double scale = m.rate_type[r] == MICM_HIP_RATE_TROE ? rho : 1.0;
-        const double k0 = p[0] * exp(p[2] / T) * pow(T / 300.0, p[1]);
-        const double kinf = p[3] * exp(p[5] / T) * pow(T / 300.0, p[4]);
+        const double k0 = p[0] * pmath::exp(p[2] / T) * pmath::pow(T / 300.0, p[1]);
+        const double kinf = p[3] * pmath::exp(p[5] / T) * pmath::pow(T / 300.0, p[4]);
         const double ratio = k0 * rho / kinf;
-        k = k0 * scale / (1.0 + ratio) * pow(p[6], p[7] / (p[7] + pow(log10(ratio), 2.0)));
+        k = k0 * scale / (1.0 + ratio) * pmath::pow(p[6], p[7] / (p[7] + pmath::pow(pmath::log10(ratio), 2.0)));
         break;
       }
       case MICM_HIP_RATE_BRANCHED:
       {
         const double a = p[5] * rho;
-        const double b = 0.43 * pow(T / 298.0, -8.0);
-        const double A_val = a / (1.0 + a / b) * pow(0.41, 1.0 / (1.0 + pow(log10(a / b), 2.0)));
-        const double pre = p[1] * exp(-p[2] / T);
+        const double b = 0.43 * pmath::pow(T / 298.0, -8.0);
+        const double A_val = a / (1.0 + a / b) * pmath::pow(0.41, 1.0 / (1.0 + pmath::pow(pmath::log10(a / b), 2.0)));
+        const double pre = p[1] * pmath::exp(-p[2] / T);
         k = p[0] == 0.0 ? pre * (p[6] / (p[6] + A_val)) : pre * (A_val / (A_val + p[6]));
         break;
       }
-      case MICM_HIP_RATE_TUNNELING: k = p[0] * exp(-p[1] / T + p[2] / (T * T * T)); break;
+      case MICM_HIP_RATE_TUNNELING: k = p[0] * pmath::exp(-p[1] / T + p[2] / (T * T * T)); break;
       case MICM_HIP_RATE_TAYLOR:
       {
         double poly = 0.0, t_pow = 1.0;
         const int nc = static_cast<int>(p[5]);
         for (int j = 0; j < nc; ++j, t_pow *= T)
           poly += p[6 + j] * t_pow;
-        k = poly * p[0] * exp(p[2] / T) * pow(T / p[3], p[1]) * (1.0 + p[4] * P);
+        k = poly * p[0] * pmath::exp(p[2] / T) * pmath::pow(T / p[3], p[1]) * (1.0 + p[4] * P);
         break;
       }
-      case MICM_HIP_RATE_REVERSIBLE: k = p[0] * exp(p[1] / T) * p[2]; break;
+      case MICM_HIP_RATE_REVERSIBLE: k = p[0] * pmath::exp(p[1] / T) * p[2]; break;
       case MICM_HIP_RATE_USER_DEFINED: k = custom[m.custom_index[r]] * p[0]; break;
       case MICM_HIP_RATE_SURFACE:
       {
@@ -257,7 +259,7 @@ namespace micm_hip
   // true when the step was accepted (the caller then takes Ynew as Y).
   __device__ inline bool control_update(const DevParams& p, CellControl& c, double error)
   {
-    const double fac = smin(p.factor_max, smax(p.factor_min, p.safety / pow(error, p.inv_elo)));
+    const double fac = smin(p.factor_max, smax(p.factor_min, p.safety / pmath::pow(error, p.inv_elo)));
     double Hnew = c.H * fac;
     c.stats[MICM_HIP_STAT_STEPS] += 1;
     if (isnan(error))

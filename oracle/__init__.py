@@ -5,6 +5,8 @@ Two CPU solvers with one ctypes interface:
 * `Oracle`     -- liboracle.so, this project's CPU restatement of the
                   reference's Rosenbrock path (oracle/micm_oracle.cpp).  It is
                   the checker on the GPU box and the "port" CPU baseline.
+* `OraclePortableMath` -- liboracle_pm.so, the restatement with the device
+                  path's own exp / log10 / pow: the checker for whole solves.
 * `Reference`  -- oracle/_ref/libmicm_ref.so, the reference's own header-only
                   solver compiled from its sources (oracle/ref_driver.cpp).  It
                   exists only where the reference headers were available at
@@ -21,6 +23,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+ORACLE_PM_LIB = os.path.join(HERE, "liboracle_pm.so")
 REF_LIB = os.path.join(HERE, "_ref", "libmicm_ref.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -188,3 +191,9 @@ class Oracle(_Base):
     """The CPU restatement."""
     prefix = "oracle_"
     path = ORACLE_LIB
+
+
+class OraclePortableMath(_Base):
+    """The CPU restatement with the HIP path's exp / log10 / pow (micm_amd/csrc/pmath.h)."""
+    prefix = "oracle_"
+    path = ORACLE_PM_LIB

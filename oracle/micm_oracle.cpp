@@ -26,6 +26,21 @@
 
 #include "../include/micm_hip.h"
 
+// ORACLE_PORTABLE_MATH: exp, log10 and pow of the rate constants and of the step
+// controller come from micm_amd/csrc/pmath.h (the functions the HIP path uses)
+// instead of the C library; everything else is unchanged.  This build isolates
+// the device path's only deviation from the reference -- its math library.
+#ifdef ORACLE_PORTABLE_MATH
+#include "../micm_amd/csrc/pmath.h"
+#define RC_EXP pmath::exp
+#define RC_POW pmath::pow
+#define RC_LOG10 pmath::log10
+#else
+#define RC_EXP std::exp
+#define RC_POW std::pow
+#define RC_LOG10 std::log10
+#endif
+
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -524,37 +539,37 @@ namespace
     double k = 0;
     switch (rx.type)
     {
-      case MICM_HIP_RATE_ARRHENIUS: k = p[0] * std::exp(p[2] / T) * std::pow(T / p[3], p[1]) * (1.0 + p[4] * P); break;
+      case MICM_HIP_RATE_ARRHENIUS: k = p[0] * RC_EXP(p[2] / T) * RC_POW(T / p[3], p[1]) * (1.0 + p[4] * P); break;
       case MICM_HIP_RATE_TROE:
       case MICM_HIP_RATE_TERNARY:
       {
         double scale = rx.type == MICM_HIP_RATE_TROE ? rho : 1.0;
-        double k0 = p[0] * std::exp(p[2] / T) * std::pow(T / 300.0, p[1]);
-        double kinf = p[3] * std::exp(p[5] / T) * std::pow(T / 300.0, p[4]);
+        double k0 = p[0] * RC_EXP(p[2] / T) * RC_POW(T / 300.0, p[1]);
+        double kinf = p[3] * RC_EXP(p[5] / T) * RC_POW(T / 300.0, p[4]);
         double ratio = k0 * rho / kinf;
-        k = k0 * scale / (1.0 + ratio) * std::pow(p[6], p[7] / (p[7] + std::pow(std::log10(ratio), 2.0)));
+        k = k0 * scale / (1.0 + ratio) * RC_POW(p[6], p[7] / (p[7] + RC_POW(RC_LOG10(ratio), 2.0)));
         break;
       }
       case MICM_HIP_RATE_BRANCHED:
       {
         double a = p[5] * rho;
-        double b = 0.43 * std::pow(T / 298.0, -8.0);
-        double A_val = a / (1.0 + a / b) * std::pow(0.41, 1.0 / (1.0 + std::pow(std::log10(a / b), 2.0)));
-        double pre = p[1] * std::exp(-p[2] / T);
+        double b = 0.43 * RC_POW(T / 298.0, -8.0);
+        double A_val = a / (1.0 + a / b) * RC_POW(0.41, 1.0 / (1.0 + RC_POW(RC_LOG10(a / b), 2.0)));
+        double pre = p[1] * RC_EXP(-p[2] / T);
         k = p[0] == 0.0 ? pre * (p[6] / (p[6] + A_val)) : pre * (A_val / (A_val + p[6]));
         break;
       }
-      case MICM_HIP_RATE_TUNNELING: k = p[0] * std::exp(-p[1] / T + p[2] / (T * T * T)); break;
+      case MICM_HIP_RATE_TUNNELING: k = p[0] * RC_EXP(-p[1] / T + p[2] / (T * T * T)); break;
       case MICM_HIP_RATE_TAYLOR:
       {
         double poly = 0.0, t_pow = 1.0;
         int64_t nc = static_cast<int64_t>(p[5]);
         for (int64_t j = 0; j < nc; ++j, t_pow *= T)
           poly += p[6 + j] * t_pow;
-        k = poly * p[0] * std::exp(p[2] / T) * std::pow(T / p[3], p[1]) * (1.0 + p[4] * P);
+        k = poly * p[0] * RC_EXP(p[2] / T) * RC_POW(T / p[3], p[1]) * (1.0 + p[4] * P);
         break;
       }
-      case MICM_HIP_RATE_REVERSIBLE: k = p[0] * std::exp(p[1] / T) * p[2]; break;
+      case MICM_HIP_RATE_REVERSIBLE: k = p[0] * RC_EXP(p[1] / T) * p[2]; break;
       case MICM_HIP_RATE_USER_DEFINED: k = custom[rx.custom] * p[0]; break;
       case MICM_HIP_RATE_SURFACE:
       {
@@ -764,7 +779,7 @@ namespace
           }
         const double Ntot = static_cast<double>(std::max<int64_t>(1, nc * N));
         error = std::max(std::sqrt(error / Ntot), 1.0e-10);
-        double fac = std::min(p.factor_max, std::max(p.factor_min, p.safety / std::pow(error, 1 / p.elo)));
+        double fac = std::min(p.factor_max, std::max(p.factor_min, p.safety / RC_POW(error, 1 / p.elo)));
         double Hnew = H * fac;
         s[MICM_HIP_STAT_STEPS] += 1;
         if (std::isnan(error))
@@ -1023,3 +1038,23 @@ extern "C"
     return 0;
   }
 }
+
+#ifdef ORACLE_PORTABLE_MATH
+// Element-wise evaluation of the portable math functions, for their accuracy
+// tests: which = 0 exp(x), 1 log(x), 2 log10(x), 3 pow(x, y).
+extern "C" int oracle_pmath(int32_t which, int64_t n, const double* x, const double* y, double* out)
+{
+  for (int64_t i = 0; i < n; ++i)
+  {
+    switch (which)
+    {
+      case 0: out[i] = pmath::exp(x[i]); break;
+      case 1: out[i] = pmath::log(x[i]); break;
+      case 2: out[i] = pmath::log10(x[i]); break;
+      case 3: out[i] = pmath::pow(x[i], y[i]); break;
+      default: return -1;
+    }
+  }
+  return 0;
+}
+#endif
