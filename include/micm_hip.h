@@ -272,6 +272,13 @@ int micm_hip_state_set_stream(micm_hip_state* state, void* stream);
 /* Step count at which the persistent form hands a cell to the cooperative form
  * (0: never; negative: the library's default).  Results do not depend on it. */
 int micm_hip_state_set_handoff(micm_hip_state* state, int64_t steps);
+/* Launch shape of the cooperative form: threads per cell (64, 128 or 256; 0: the
+ * library's choice) and which constant tables are staged in LDS (bit 0 the
+ * substitution tables, bit 1 the LU tables; -1: the library's choice; tables that
+ * do not fit next to the cell's data stay in global memory).  A tuning
+ * knob in the role of the vector length L of the reference's GPU policies
+ * (cuda_solver_builder.hpp:24-38); results do not depend on it. */
+int micm_hip_state_set_cooperative_shape(micm_hip_state* state, int32_t threads, int32_t stage);
 /* Whether the cooperative form fits this mechanism on the current device (its
  * per-cell working set must fit one workgroup's LDS); *out = 1 or 0. */
 int micm_hip_mechanism_cooperative_supported(const micm_hip_mechanism* mechanism, int32_t stages, int32_t* out);

@@ -25,6 +25,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int64)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64 = ctypes.c_int64
+_i32 = ctypes.c_int32
 
 
 class MechanismInfo(ctypes.Structure):
@@ -85,6 +86,7 @@ _SIGNATURES = {
     "micm_hip_state_get_results": [_vp, _i32p, _ip, _dp],
     "micm_hip_state_set_stream": [_vp, _vp],
     "micm_hip_state_set_handoff": [_vp, _i64],
+    "micm_hip_state_set_cooperative_shape": [_vp, _i32, _i32],
     "micm_hip_mechanism_cooperative_supported": [_vp, ctypes.c_int32, _i32p],
     "micm_hip_update_state_parameters": [_vp, _vp],
     "micm_hip_solve": [_vp, _vp, ctypes.c_double, ctypes.POINTER(RosenbrockParameters), ctypes.c_int32],
@@ -283,6 +285,11 @@ class DeviceState:
         """Step count at which the persistent form hands a cell to the cooperative form
         (0: never, negative: library default)."""
         check(lib().micm_hip_state_set_handoff(self.h, int(steps)))
+
+    def set_cooperative_shape(self, threads=0, stage=-1):
+        """Threads per cell (64/128/256, 0 = library choice) and LDS staging of the
+        substitution (bit 0) and LU (bit 1) tables (-1 = library choice)."""
+        check(lib().micm_hip_state_set_cooperative_shape(self.h, int(threads), int(stage)))
 
     def results(self):
         states = np.zeros(self.n_cells, dtype=np.int32)

@@ -1,17 +1,19 @@
-// coop.hpp -- the cooperative form: one wavefront integrates one cell with the
-// cell's whole working set (in-place LU, Jacobian, stage vectors) in LDS.
+// coop.hpp -- the cooperative form: one workgroup (NT = 64, 128 or 256 threads)
+// integrates one cell with the cell's whole working set (in-place LU, Jacobian,
+// stage vectors) in LDS.
 //
-// Work inside a cell is spread over the 64 lanes without changing any rounding:
+// Work inside a cell is spread over the threads without changing any rounding:
 //   forcing   per-species gather, terms in the reference's scatter order
 //             (process_set.hpp:541-579)
 //   -J        per-element gather, terms in the reference's order (process_set.hpp:613-659)
-//   LU        right-looking elimination over the Doolittle fill pattern; element
-//             (k, m) receives its products in ascending j exactly as in
-//             lu_decomposition_doolittle.inl:347-398, and L[k][j] is divided by U[j][j]
-//             before it is used
-//   forward   column-oriented; x_i receives L_ij x_j in ascending j (linear_solver.inl:250-265)
-//   backward  row-oriented by one lane, ascending j, then the division (linear_solver.inl:270-287)
-//   error     squares formed by all lanes, summed in species order by one lane
+//   LU        level-scheduled Doolittle: one task per element of the fill pattern,
+//             subtracting L[i][j] U[j][k] for j ascending and dividing L elements
+//             by U[k][k], as in lu_decomposition_doolittle.inl:347-398; a task runs
+//             once every position it reads was written at a lower level
+//   forward   level-scheduled rows, x_i -= L_ij x_j for j ascending (linear_solver.inl:250-265)
+//   backward  level-scheduled rows, x_i -= U_ij x_j for j ascending, then / U_ii
+//             (linear_solver.inl:270-287)
+//   error     squares formed by all threads, summed in species order by one thread
 //             (rosenbrock.inl:377-392)
 #pragma once
 
@@ -20,38 +22,43 @@
 
 namespace micm_hip
 {
-  constexpr int kCoopThreads = 64;
+  constexpr int kWave = 64;
 
-  // Synchronisation inside the cooperative workgroup, which is exactly one wavefront:
-  // its lanes issue in lockstep, so it is enough that this wave's LDS operations have
+  // Synchronisation inside the cooperative workgroup.  With a single wavefront its
+  // lanes issue in lockstep, so it is enough that this wave's LDS operations have
   // completed (lgkmcnt(0)) and that the compiler moves no memory access across the
-  // point.  Unlike __syncthreads() it does not wait for outstanding global loads, so
-  // prefetched table reads stay in flight.
+  // point; unlike __syncthreads() this does not wait for outstanding global loads.
+  template<int NT>
   __device__ __forceinline__ void coop_sync()
   {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_s_waitcnt(0xc07f);  // vmcnt(63) expcnt(7) lgkmcnt(0)
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if constexpr (NT == kWave)
+    {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_s_waitcnt(0xc07f);  // vmcnt(63) expcnt(7) lgkmcnt(0)
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    else
+    {
+      __syncthreads();
+    }
   }
 
   struct CoopLds
   {
     double *alu, *jac, *y, *ynew, *f0, *yerr, *K, *k, *scratch;
+    double* jterm;  // Jacobian partial products: the ALU array when it is large enough
   };
 
+  // Doubles of per-cell LDS data.  The Jacobian's partial products live in the ALU
+  // array (rebuilt from the Jacobian before every factorisation) when it is large enough.
   __host__ __device__ inline size_t coop_lds_doubles(int nnz_alu, int nnz_jac, int n, int stages, int n_reactions,
                                                       int n_terms)
   {
-    int scratch = n_reactions > n_terms ? n_reactions : n_terms;
-    scratch = scratch > n ? scratch : n;
+    int scratch = n_reactions > n ? n_reactions : n;
+    if (n_terms > nnz_alu && n_terms > scratch)
+      scratch = n_terms;
     return static_cast<size_t>(nnz_alu) + nnz_jac + static_cast<size_t>(4 + stages) * n + n_reactions + scratch;
-  }
-
-  // Bytes of the packed substitution tables staged after the per-cell data.
-  __host__ __device__ inline size_t coop_table_bytes(int n_fcol, int n_brow)
-  {
-    return (static_cast<size_t>(n_fcol) + n_brow) * sizeof(uint32_t);
   }
 
   __device__ inline CoopLds coop_layout(double* lds, const DevMech& m, const DevCoop& c, int stages)
@@ -67,168 +74,147 @@ namespace micm_hip
     L.K = L.yerr + n;
     L.k = L.K + static_cast<size_t>(stages) * n;
     L.scratch = L.k + m.n_reactions;
+    L.jterm = c.n_terms <= c.nnz_alu ? L.alu : L.scratch;
     return L;
   }
 
   // out[s] = f(yv)[s]
+  template<int NT>
   __device__ inline void coop_forcing(const DevMech& m, const DevCoop& c, const CoopLds& L, const double* yv,
                                       double* out)
   {
-    const int lane = threadIdx.x;
-    for (int r = lane; r < m.n_reactions; r += kCoopThreads)
+    const int tid = threadIdx.x;
+    for (int r = tid; r < m.n_reactions; r += NT)
     {
       double rate = L.k[r];
       for (int i = m.reactant_offsets[r]; i < m.reactant_offsets[r + 1]; ++i)
         rate *= yv[m.reactant_ids[i]];
       L.scratch[r] = rate;
     }
-    coop_sync();
-    for (int s = lane; s < m.n_species; s += kCoopThreads)
+    coop_sync<NT>();
+    for (int s = tid; s < m.n_species; s += NT)
     {
       double v = 0.0;
       for (int q = c.spec_offsets[s]; q < c.spec_offsets[s + 1]; ++q)
         v += c.spec_coef[q] * L.scratch[c.spec_rxn[q]];
       out[s] = v;
     }
-    coop_sync();
+    coop_sync<NT>();
   }
 
   // L.jac = -J(y)
+  template<int NT>
   __device__ inline void coop_jacobian(const DevMech& m, const DevCoop& c, const CoopLds& L)
   {
-    const int lane = threadIdx.x;
-    for (int t = lane; t < c.n_terms; t += kCoopThreads)
+    const int tid = threadIdx.x;
+    for (int t = tid; t < c.n_terms; t += NT)
     {
       double d = L.k[m.term_process[t]];
       for (int i = m.term_dep_offsets[t]; i < m.term_dep_offsets[t + 1]; ++i)
         d *= L.y[m.term_dep_ids[i]];
-      L.scratch[t] = d;
+      L.jterm[t] = d;
     }
-    coop_sync();
-    for (int e = lane; e < m.nnz_jac; e += kCoopThreads)
+    coop_sync<NT>();
+    for (int e = tid; e < m.nnz_jac; e += NT)
     {
       double v = 0.0;
       for (int q = c.jel_offsets[e]; q < c.jel_offsets[e + 1]; ++q)
-        v += c.jel_coef[q] * L.scratch[c.jel_term[q]];
+        v += c.jel_coef[q] * L.jterm[c.jel_term[q]];
       L.jac[e] = v;
     }
-    coop_sync();
+    coop_sync<NT>();
   }
 
-  // In-place LU of L.jac into L.alu.  Each lane applies the triples q = lane, lane+64, ...
-  // of a step; a lane's first eight triples of step j + 1 are loaded while step j runs.
-  __device__ inline void coop_decompose(const DevMech& m, const DevCoop& c, const CoopLds& L)
+  // v - sum_q A[lo(terms[q])] * X[hi(terms[q])] for q in [b, e), subtracting in order;
+  // the loads of eight terms are issued before their subtractions.
+  __device__ __forceinline__ double coop_dot(double v, const uint32_t* terms, uint32_t b, uint32_t e, const double* A,
+                                             const double* X)
   {
-    const int lane = threadIdx.x;
-    for (int q = lane; q < c.nnz_alu; q += kCoopThreads)
-      L.alu[q] = 0.0;
-    coop_sync();
-    for (int e = lane; e < m.nnz_jac; e += kCoopThreads)
-      L.alu[c.jac_to_alu[e]] = L.jac[e];
-    coop_sync();
-    constexpr int B = 8;
-    uint2 next[B];
+    uint32_t q = b;
+    for (; q + 8 <= e; q += 8)
     {
-      const int b = c.upd_offsets[0] + lane, e = c.upd_offsets[1];
+      double a[8], x[8];
 #pragma unroll
-      for (int r = 0; r < B; ++r)
-        if (b + r * kCoopThreads < e)
-          next[r] = c.upd16[b + r * kCoopThreads];
+      for (int r = 0; r < 8; ++r)
+      {
+        const uint32_t t = terms[q + r];
+        a[r] = A[t & 0xffffu];
+        x[r] = X[t >> 16];
+      }
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        v -= a[r] * x[r];
     }
-    for (int j = 0; j < m.n_species; ++j)
+    for (; q < e; ++q)
     {
-      uint2 cur[B];
-#pragma unroll
-      for (int r = 0; r < B; ++r)
-        cur[r] = next[r];
-      if (j + 1 < m.n_species)
+      const uint32_t t = terms[q];
+      v -= A[t & 0xffffu] * X[t >> 16];
+    }
+    return v;
+  }
+
+  // Runs a level schedule: task t computes out[o] = (out[o] - dot) / A[diag] (no
+  // division when diag is 0xffff), reading A and X.
+  template<int NT>
+  __device__ inline void coop_levels(const DevLevels& lv, const uint2* tasks, const uint32_t* terms, const double* A,
+                                     const double* X, double* out)
+  {
+    const int tid = threadIdx.x;
+    for (int l = 0; l < lv.n_levels; ++l)
+    {
+      const int tb = lv.offsets[l], te = lv.offsets[l + 1];
+      for (int t = tb + tid; t < te; t += NT)
       {
-        const int b = c.upd_offsets[j + 1] + lane, e = c.upd_offsets[j + 2];
-#pragma unroll
-        for (int r = 0; r < B; ++r)
-          if (b + r * kCoopThreads < e)
-            next[r] = c.upd16[b + r * kCoopThreads];
+        const uint2 task = tasks[t];
+        const uint32_t qe = tasks[t + 1].y;
+        const uint32_t o = task.x & 0xffffu, d = task.x >> 16;
+        double v = coop_dot(out[o], terms, task.y, qe, A, X);
+        if (d != 0xffffu)
+          v = v / A[d];
+        out[o] = v;
       }
-      const double piv = L.alu[c.alu_diag[j]];
-      for (int q = c.div_offsets[j] + lane; q < c.div_offsets[j + 1]; q += kCoopThreads)
-        L.alu[c.div[q]] = L.alu[c.div[q]] / piv;
-      coop_sync();
-      // the triples of one step touch distinct targets, so their order is free
-      const int b = c.upd_offsets[j] + lane, e = c.upd_offsets[j + 1];
-#pragma unroll
-      for (int r = 0; r < B; ++r)
-        if (b + r * kCoopThreads < e)
-        {
-          const uint2 t = cur[r];
-          L.alu[t.x & 0xffffu] -= L.alu[t.x >> 16] * L.alu[t.y];
-        }
-      for (int q0 = b + B * kCoopThreads; q0 < e; q0 += B * kCoopThreads)
-      {
-        uint2 tr[B];
-#pragma unroll
-        for (int r = 0; r < B; ++r)
-          if (q0 + r * kCoopThreads < e)
-            tr[r] = c.upd16[q0 + r * kCoopThreads];
-#pragma unroll
-        for (int r = 0; r < B; ++r)
-          if (q0 + r * kCoopThreads < e)
-            L.alu[tr[r].x & 0xffffu] -= L.alu[tr[r].x >> 16] * L.alu[tr[r].y];
-      }
-      coop_sync();
+      coop_sync<NT>();
     }
   }
 
-  // Substitution tables: packed 16-bit pairs, staged in LDS once per workgroup
-  struct CoopSolveTables
+  // Table pointers for one launch: global memory or the workgroup's LDS copy.
+  struct CoopTables
   {
-    const uint32_t* fcol;
-    const uint32_t* brow;
+    const uint2 *lu_tasks, *fwd_tasks, *bwd_tasks;
+    const uint32_t *lu_terms, *fwd_terms, *bwd_terms;
   };
 
-  __device__ inline void coop_solve(const DevMech& m, const DevCoop& c, const CoopLds& L, const CoopSolveTables& T,
-                                    double* x)
+  // In-place LU of L.jac into L.alu.
+  template<int NT>
+  __device__ inline void coop_decompose(const DevMech& m, const DevCoop& c, const CoopLds& L, const CoopTables& T)
   {
-    const int lane = threadIdx.x;
-    const int n = m.n_species;
-    for (int j = 0; j < n; ++j)
-    {
-      const double xj = x[j];
-      for (int q = c.fcol_offsets[j] + lane; q < c.fcol_offsets[j + 1]; q += kCoopThreads)
-      {
-        const uint32_t ip = T.fcol[q];
-        x[ip & 0xffffu] -= L.alu[ip >> 16] * xj;
-      }
-      coop_sync();
-    }
-    if (lane == 0)
-    {
-      // row i of U right of the diagonal is contiguous in ALU; only the columns are looked up
-      for (int i = n - 1; i >= 0; --i)
-      {
-        double v = x[i];
-        const int d = c.alu_diag[i];
-        int q = c.brow_offsets[i];
-        const int e = c.brow_offsets[i + 1];
-        const double* u = L.alu + d + 1 - q;
-        constexpr int B = 16;
-        for (; q + B <= e; q += B)
-        {
-          double a[B], b[B];
-#pragma unroll
-          for (int r = 0; r < B; ++r)
-          {
-            a[r] = u[q + r];
-            b[r] = x[T.brow[q + r] >> 16];
-          }
-#pragma unroll
-          for (int r = 0; r < B; ++r)
-            v -= a[r] * b[r];
-        }
-        for (; q < e; ++q)
-          v -= u[q] * x[T.brow[q] >> 16];
-        x[i] = v / L.alu[d];
-      }
-    }
-    coop_sync();
+    const int tid = threadIdx.x;
+    for (int q = tid; q < c.nnz_alu; q += NT)
+      L.alu[q] = 0.0;
+    coop_sync<NT>();
+    for (int e = tid; e < m.nnz_jac; e += NT)
+      L.alu[c.jac_to_alu[e]] = L.jac[e];
+    coop_sync<NT>();
+    coop_levels<NT>(c.lu, T.lu_tasks, T.lu_terms, L.alu, L.alu, L.alu);
+  }
+
+  // x <- U^-1 L^-1 x with the factors in L.alu (unit lower diagonal).
+  template<int NT>
+  __device__ inline void coop_solve(const DevCoop& c, const CoopLds& L, const CoopTables& T, double* x)
+  {
+    coop_levels<NT>(c.fwd, T.fwd_tasks, T.fwd_terms, L.alu, x, x);
+    coop_levels<NT>(c.bwd, T.bwd_tasks, T.bwd_terms, L.alu, x, x);
+  }
+
+  // 32-bit words of the tables a launch stages in LDS.
+  __host__ __device__ inline size_t coop_solve_table_words(const DevCoop& c)
+  {
+    return 2 * (static_cast<size_t>(c.fwd.n_tasks) + 1) + c.fwd.n_terms + 2 * (static_cast<size_t>(c.bwd.n_tasks) + 1) +
+           c.bwd.n_terms;
+  }
+
+  __host__ __device__ inline size_t coop_lu_table_words(const DevCoop& c)
+  {
+    return 2 * (static_cast<size_t>(c.lu.n_tasks) + 1) + c.lu.n_terms;
   }
 }  // namespace micm_hip

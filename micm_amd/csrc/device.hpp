@@ -54,25 +54,25 @@ namespace micm_hip
     const int2* bwd_terms;  // (U index, column)
   };
 
-  // Tables of the cooperative (one wavefront per cell) form; see MechanismTables.
+  // One level schedule (tasks of a level are independent).  tasks[t].x = out | diag << 16
+  // (diag 0xffff: none), tasks[t].y = first term; tasks[n_tasks] is a sentinel whose .y
+  // closes the last term range.
+  struct DevLevels
+  {
+    const int32_t* offsets;  // n_levels + 1
+    const uint2* tasks;      // n_tasks + 1
+    const uint32_t* terms;   // two 16-bit positions
+    int32_t n_levels, n_tasks, n_terms;
+  };
+
+  // Tables of the cooperative (one workgroup per cell) form; see MechanismTables.
   struct DevCoop
   {
     int32_t nnz_alu;
     int32_t n_terms;
     const int32_t* jac_to_alu;
     const int32_t* alu_diag;
-    const int32_t* div_offsets;
-    const int32_t* div;
-    const int32_t* upd_offsets;
-    const int32_t* upd;  // triples
-    const int32_t* fcol_offsets;
-    const int2* fcol;    // (row, ALU position)
-    const int32_t* brow_offsets;
-    const int2* brow;    // (ALU position, column)
-    const uint32_t* fcol16;  // fcol / brow packed as 16-bit pairs (empty if positions exceed 16 bits)
-    const uint32_t* brow16;
-    const uint2* upd16;      // (target | L source << 16, U source)
-    int32_t n_fcol, n_brow;
+    DevLevels lu, fwd, bwd;
     const int32_t* spec_offsets;
     const int32_t* spec_rxn;
     const double* spec_coef;

@@ -137,6 +137,45 @@ namespace micm_hip
 
   namespace
   {
+    struct LevelTask
+    {
+      int32_t level = 0;
+      uint32_t out = 0, diag = 0;
+      std::vector<uint32_t> terms;
+    };
+
+    // Tasks grouped by level; within a level the longest first, so that a lane's
+    // round-robin share is balanced.  tasks[2 t] = out | diag << 16, tasks[2 t + 1] =
+    // first term; a sentinel task closes the last term range.
+    LevelSchedule pack_levels(std::vector<LevelTask>& v)
+    {
+      std::stable_sort(v.begin(), v.end(), [](const LevelTask& a, const LevelTask& b) {
+        if (a.level != b.level)
+          return a.level < b.level;
+        return a.terms.size() > b.terms.size();
+      });
+      LevelSchedule s;
+      s.offsets.push_back(0);
+      int32_t lv = 0;
+      for (size_t q = 0; q < v.size(); ++q)
+      {
+        while (v[q].level > lv)
+        {
+          s.offsets.push_back(static_cast<int32_t>(q));
+          ++lv;
+        }
+        s.tasks.push_back(v[q].out | (v[q].diag << 16));
+        s.tasks.push_back(static_cast<uint32_t>(s.terms.size()));
+        s.terms.insert(s.terms.end(), v[q].terms.begin(), v[q].terms.end());
+        s.max_terms = std::max<int32_t>(s.max_terms, static_cast<int32_t>(v[q].terms.size()));
+      }
+      if (!v.empty())
+        s.offsets.push_back(static_cast<int32_t>(v.size()));
+      s.tasks.push_back(0xffffffffu);
+      s.tasks.push_back(static_cast<uint32_t>(s.terms.size()));
+      return s;
+    }
+
     void build_cooperative_schedule(MechanismTables& t)
     {
       const int32_t n = static_cast<int32_t>(t.n_species);
@@ -169,71 +208,89 @@ namespace micm_hip
         for (int32_t p = t.upper_row_ptr[i]; p < t.upper_row_ptr[i + 1]; ++p)
           t.alu_upper[p] = alu(i, t.upper_col[p]);
       }
-      // steps of the right-looking elimination
-      std::vector<std::vector<int32_t>> lcol(n), urow(n);
-      for (int32_t i = 0; i < n; ++i)
+      // level schedules (16-bit positions; 0xffff marks "no divisor")
+      t.coop_levels_ok = t.alu_col.size() < 0xffff && n < 0xffff;
+      t.lv_lu = LevelSchedule{};
+      t.lv_fwd = LevelSchedule{};
+      t.lv_bwd = LevelSchedule{};
+      if (!t.coop_levels_ok)
+        return;
+      const uint32_t kNone = 0xffff;
+      // LU: one task per Doolittle element that has products to subtract or a divisor;
+      // the level of an ALU position is one more than the highest level it reads
       {
-        for (int32_t p = t.lower_row_ptr[i]; p < t.lower_row_ptr[i + 1]; ++p)
-          if (t.lower_col[p] < i)
-            lcol[t.lower_col[p]].push_back(i);
-        for (int32_t p = t.upper_row_ptr[i]; p < t.upper_row_ptr[i + 1]; ++p)
-          if (t.upper_col[p] > i)
-            urow[i].push_back(t.upper_col[p]);
-      }
-      t.coop_div_offsets.assign(1, 0);
-      t.coop_upd_offsets.assign(1, 0);
-      t.coop_div.clear();
-      t.coop_upd.clear();
-      for (int32_t j = 0; j < n; ++j)
-      {
-        for (int32_t k : lcol[j])
-          t.coop_div.push_back(alu(k, j));
-        t.coop_div_offsets.push_back(static_cast<int32_t>(t.coop_div.size()));
-        for (int32_t k : lcol[j])
-          for (int32_t m : urow[j])
+        std::vector<int32_t> level(t.alu_col.size(), 0);
+        std::vector<LevelTask> tasks;
+        const size_t n_el = t.lu_out.size();
+        for (size_t e = 0; e < n_el; ++e)
+        {
+          const bool lower = t.lu_div[e] >= 0;
+          const int32_t out = lower ? t.alu_lower[t.lu_out[e]] : t.alu_upper[t.lu_out[e]];
+          const int32_t tb = t.lu_term_offsets[e], te = t.lu_term_offsets[e + 1];
+          if (tb == te && !lower)
+            continue;  // U element equal to its Jacobian entry
+          LevelTask task;
+          task.out = static_cast<uint32_t>(out);
+          task.diag = lower ? static_cast<uint32_t>(t.alu_upper[t.lu_div[e]]) : kNone;
+          int32_t lv = lower ? level[task.diag] : 0;
+          for (int32_t q = tb; q < te; ++q)
           {
-            t.coop_upd.push_back(alu(k, m));
-            t.coop_upd.push_back(alu(k, j));
-            t.coop_upd.push_back(alu(j, m));
+            const int32_t l = t.alu_lower[t.lu_term_l[q]], u = t.alu_upper[t.lu_term_u[q]];
+            task.terms.push_back(static_cast<uint32_t>(l) | (static_cast<uint32_t>(u) << 16));
+            lv = std::max(lv, std::max(level[l], level[u]));
           }
-        t.coop_upd_offsets.push_back(static_cast<int32_t>(t.coop_upd.size() / 3));
-      }
-      // substitution
-      t.coop_fcol_offsets.assign(1, 0);
-      t.coop_brow_offsets.assign(1, 0);
-      t.coop_fcol.clear();
-      t.coop_brow.clear();
-      for (int32_t j = 0; j < n; ++j)
-      {
-        for (int32_t k : lcol[j])
-        {
-          t.coop_fcol.push_back(k);
-          t.coop_fcol.push_back(alu(k, j));
+          task.level = lv;
+          level[out] = lv + 1;
+          tasks.push_back(std::move(task));
         }
-        t.coop_fcol_offsets.push_back(static_cast<int32_t>(t.coop_fcol.size() / 2));
+        t.lv_lu = pack_levels(tasks);
       }
-      for (int32_t i = 0; i < n; ++i)
+      // forward substitution: row i subtracts L[i][j] x[j], j ascending (unit diagonal)
       {
-        for (int32_t m : urow[i])
+        std::vector<int32_t> level(n, 0);
+        std::vector<LevelTask> tasks;
+        for (int32_t i = 0; i < n; ++i)
         {
-          t.coop_brow.push_back(alu(i, m));
-          t.coop_brow.push_back(m);
+          const int32_t b = t.fwd_offsets[i], e = t.fwd_offsets[i + 1];
+          if (b == e)
+            continue;
+          LevelTask task;
+          task.out = static_cast<uint32_t>(i);
+          task.diag = kNone;
+          int32_t lv = 0;
+          for (int32_t q = b; q < e; ++q)
+          {
+            const int32_t j = t.fwd_col[q];
+            task.terms.push_back(static_cast<uint32_t>(t.alu_lower[t.fwd_mat[q]]) | (static_cast<uint32_t>(j) << 16));
+            lv = std::max(lv, level[j]);
+          }
+          task.level = lv;
+          level[i] = lv + 1;
+          tasks.push_back(std::move(task));
         }
-        t.coop_brow_offsets.push_back(static_cast<int32_t>(t.coop_brow.size() / 2));
+        t.lv_fwd = pack_levels(tasks);
       }
-      t.coop_fcol16.clear();
-      t.coop_brow16.clear();
-      if (t.alu_col.size() < 65536 && n < 65536)
+      // back substitution: row i subtracts U[i][j] x[j], j ascending, then divides by U[i][i]
       {
-        for (size_t q = 0; q < t.coop_fcol.size(); q += 2)
-          t.coop_fcol16.push_back(static_cast<uint32_t>(t.coop_fcol[q]) | (static_cast<uint32_t>(t.coop_fcol[q + 1]) << 16));
-        for (size_t q = 0; q < t.coop_brow.size(); q += 2)
-          t.coop_brow16.push_back(static_cast<uint32_t>(t.coop_brow[q]) | (static_cast<uint32_t>(t.coop_brow[q + 1]) << 16));
-        for (size_t q = 0; q < t.coop_upd.size(); q += 3)
+        std::vector<int32_t> level(n, 0);
+        std::vector<LevelTask> tasks;
+        for (int32_t i = n - 1; i >= 0; --i)
         {
-          t.coop_upd16.push_back(static_cast<uint32_t>(t.coop_upd[q]) | (static_cast<uint32_t>(t.coop_upd[q + 1]) << 16));
-          t.coop_upd16.push_back(static_cast<uint32_t>(t.coop_upd[q + 2]));
+          LevelTask task;
+          task.out = static_cast<uint32_t>(i);
+          task.diag = static_cast<uint32_t>(t.alu_diag[i]);
+          int32_t lv = 0;
+          for (int32_t q = t.bwd_offsets[i]; q < t.bwd_offsets[i + 1]; ++q)
+          {
+            const int32_t j = t.bwd_col[q];
+            task.terms.push_back(static_cast<uint32_t>(t.alu_upper[t.bwd_mat[q]]) | (static_cast<uint32_t>(j) << 16));
+            lv = std::max(lv, level[j]);
+          }
+          task.level = lv;
+          level[i] = lv + 1;
+          tasks.push_back(std::move(task));
         }
+        t.lv_bwd = pack_levels(tasks);
       }
       // forcing gather, in the scatter order of the reference (reactions ascending;
       // within a reaction its reactant slots, then its product slots)

@@ -12,6 +12,17 @@
 
 namespace micm_hip
 {
+  // Independent tasks grouped by dependency level (see MechanismTables::lv_lu).
+  struct LevelSchedule
+  {
+    std::vector<int32_t> offsets;  // level -> first task, n_levels + 1
+    std::vector<uint32_t> tasks;   // 2 per task + sentinel: out | diag << 16, first term
+    std::vector<uint32_t> terms;   // two 16-bit fields
+    int32_t max_terms = 0;
+    int32_t n_levels() const { return offsets.empty() ? 0 : static_cast<int32_t>(offsets.size()) - 1; }
+    int32_t n_tasks() const { return static_cast<int32_t>(tasks.size() / 2) - 1; }
+  };
+
   // Rate parameters as consumed by the device (per sorted reaction, stride
   // MICM_HIP_RATE_PARAM_STRIDE).  Derived values live in spare slots:
   //   BRANCHED  slot 5 = k0, slot 6 = z   (reaction_rate_store.hpp:286-293)
@@ -64,20 +75,18 @@ namespace micm_hip
     std::vector<int32_t> bwd_offsets, bwd_mat, bwd_col;   // row i (ascending storage): U[i][j], j ascending
     std::vector<int32_t> fwd_pairs, bwd_pairs;            // interleaved (matrix index, column)
 
-    // ---- cooperative (one wavefront per cell) schedule over the in-place pattern
-    // ALU = strictly-lower L united with U (CSR, row-major).  The right-looking form
-    // applies, at step j, ALU[k][m] -= ALU[k][j] * ALU[j][m] for every (k, m) with
-    // L[k][j] and U[j][m] non-zero -- exactly Doolittle's terms, so each element
-    // receives its products in the same ascending-j order.
+    // ---- cooperative (one workgroup per cell) schedule over the in-place pattern
+    // ALU = strictly-lower L united with U (CSR, row-major), factorised in place.
     std::vector<int32_t> alu_row_ptr, alu_col, alu_diag;
     std::vector<int32_t> jac_to_alu;                       // Jacobian element -> ALU position
     std::vector<int32_t> alu_lower, alu_upper;             // L / U storage index -> ALU position
-    std::vector<int32_t> coop_div_offsets, coop_div;       // step j: ALU positions of L[k][j], k > j
-    std::vector<int32_t> coop_upd_offsets, coop_upd;       // step j: (target, L source, U source) triples
-    std::vector<int32_t> coop_fcol_offsets, coop_fcol;     // column j of L: (row i, ALU position) pairs
-    std::vector<int32_t> coop_brow_offsets, coop_brow;     // row i of U: (ALU position, column j) pairs, j ascending
-    std::vector<uint32_t> coop_fcol16, coop_brow16;        // the same pairs packed as two 16-bit fields (x | y << 16)
-    std::vector<uint32_t> coop_upd16;                      // triples packed as (target | L source << 16, U source)
+    // Level schedules: every task reads only positions written at lower levels, and
+    // subtracts its products in the reference's order, so the tasks of one level run
+    // in parallel without changing any rounding.
+    bool coop_levels_ok = false;                           // positions fit the 16-bit fields
+    LevelSchedule lv_lu;   // out = ALU position, diag = U[i][i] position (L elements), terms (l | u << 16)
+    LevelSchedule lv_fwd;  // out = row, terms (ALU position | column << 16)
+    LevelSchedule lv_bwd;  // out = row, diag = U[i][i] position, terms (ALU position | column << 16)
     // forcing as a per-species gather: f[s] += c * rate[r] over (r, c) in reaction order
     std::vector<int32_t> spec_offsets, spec_rxn;
     std::vector<double> spec_coef;

@@ -575,14 +575,16 @@ namespace
     }
   }
 
-  // ---- cooperative form: one wavefront per cell, working set in LDS
+  // ---- cooperative form: one workgroup per cell, working set in LDS
 
-  __global__ void __launch_bounds__(kCoopThreads) k_coop(DevMech m, DevCoop c, DevParams p, int tables_in_lds, int64_t n_fixed,
-                                                         const int32_t* __restrict__ list, const int* list_count,
-                                                         int* next, const CellControl* resume,
-                                                         double* __restrict__ y_state, const double* __restrict__ k_state,
-                                                         const double* __restrict__ atol, int32_t* status_out,
-                                                         int64_t* stats_out, double* final_time_out)
+  // stage: bit 0 the substitution tables, bit 1 the LU tables, copied into LDS once per
+  // workgroup after the per-cell data
+  template<int NT>
+  __global__ void __launch_bounds__(NT) k_coop(DevMech m, DevCoop c, DevParams p, int stage, int64_t n_fixed,
+                                               const int32_t* __restrict__ list, const int* list_count, int* next,
+                                               const CellControl* resume, double* __restrict__ y_state,
+                                               const double* __restrict__ k_state, const double* __restrict__ atol,
+                                               int32_t* status_out, int64_t* stats_out, double* final_time_out)
   {
     extern __shared__ double lds[];
     __shared__ int s_idx;
@@ -591,27 +593,41 @@ namespace
     const int n = m.n_species;
     const CoopLds L = coop_layout(lds, m, c, p.stages);
     const int64_t count = list ? static_cast<int64_t>(*list_count) : n_fixed;
-    // the substitution tables are the same for every cell: stage them once
-    CoopSolveTables T{ c.fcol16, c.brow16 };
-    if (tables_in_lds)
+    CoopTables T{ c.lu.tasks, c.fwd.tasks, c.bwd.tasks, c.lu.terms, c.fwd.terms, c.bwd.terms };
     {
-      uint32_t* t = reinterpret_cast<uint32_t*>(lds + coop_lds_doubles(c.nnz_alu, m.nnz_jac, n, p.stages,
+      uint32_t* w = reinterpret_cast<uint32_t*>(lds + coop_lds_doubles(c.nnz_alu, m.nnz_jac, n, p.stages,
                                                                          m.n_reactions, c.n_terms));
-      for (int q = lane; q < c.n_fcol; q += kCoopThreads)
-        t[q] = c.fcol16[q];
-      for (int q = lane; q < c.n_brow; q += kCoopThreads)
-        t[c.n_fcol + q] = c.brow16[q];
-      T.fcol = t;
-      T.brow = t + c.n_fcol;
-      coop_sync();
+      auto stage_copy = [&](const uint32_t* src, size_t words) {
+        for (size_t q = lane; q < words; q += NT)
+          w[q] = src[q];
+        const uint32_t* dst = w;
+        w += (words + 1) & ~size_t(1);  // keep 8-byte alignment for the task pairs
+        return dst;
+      };
+      if (stage & 1)
+      {
+        T.fwd_tasks = reinterpret_cast<const uint2*>(
+            stage_copy(reinterpret_cast<const uint32_t*>(c.fwd.tasks), 2 * (static_cast<size_t>(c.fwd.n_tasks) + 1)));
+        T.fwd_terms = stage_copy(c.fwd.terms, c.fwd.n_terms);
+        T.bwd_tasks = reinterpret_cast<const uint2*>(
+            stage_copy(reinterpret_cast<const uint32_t*>(c.bwd.tasks), 2 * (static_cast<size_t>(c.bwd.n_tasks) + 1)));
+        T.bwd_terms = stage_copy(c.bwd.terms, c.bwd.n_terms);
+      }
+      if (stage & 2)
+      {
+        T.lu_tasks = reinterpret_cast<const uint2*>(
+            stage_copy(reinterpret_cast<const uint32_t*>(c.lu.tasks), 2 * (static_cast<size_t>(c.lu.n_tasks) + 1)));
+        T.lu_terms = stage_copy(c.lu.terms, c.lu.n_terms);
+      }
+      coop_sync<NT>();
     }
     while (true)
     {
       if (lane == 0)
         s_idx = atomicAdd(next, 1);
-      coop_sync();
+      coop_sync<NT>();
       const int64_t idx = s_idx;
-      coop_sync();
+      coop_sync<NT>();
       if (idx >= count)
         return;
       const int64_t cell = list ? list[idx] : idx;
@@ -620,9 +636,9 @@ namespace
       {
         Lane ys = lane_view(y_state, g, n, cl);
         Lane ks = lane_view(const_cast<double*>(k_state), g, m.n_reactions, cl);
-        for (int i = lane; i < n; i += kCoopThreads)
+        for (int i = lane; i < n; i += NT)
           L.y[i] = ys[i];
-        for (int r = lane; r < m.n_reactions; r += kCoopThreads)
+        for (int r = lane; r < m.n_reactions; r += NT)
           L.k[r] = ks[r];
       }
       CellControl cc;
@@ -630,25 +646,25 @@ namespace
         cc = resume[cell];
       else
         control_init(p, cc);
-      coop_sync();
+      coop_sync<NT>();
       while (true)
       {
         if (cc.flags & kNewStep)
         {
           if (!begin_step(p, cc))
             break;
-          coop_forcing(m, c, L, L.y, L.f0);
+          coop_forcing<NT>(m, c, L, L.y, L.f0);
           cc.stats[MICM_HIP_STAT_FUNCTION_CALLS] += 1;
-          coop_jacobian(m, c, L);
+          coop_jacobian<NT>(m, c, L);
           cc.stats[MICM_HIP_STAT_JACOBIAN_UPDATES] += 1;
           cc.last_alpha = 0.0;
           cc.flags &= ~kNewStep;
         }
         const double alpha = next_alpha(p, cc);
-        for (int i = lane; i < n; i += kCoopThreads)
+        for (int i = lane; i < n; i += NT)
           L.jac[m.jac_diag[i]] += alpha * 1.0;
-        coop_sync();
-        coop_decompose(m, c, L);
+        coop_sync<NT>();
+        coop_decompose<NT>(m, c, L, T);
         cc.stats[MICM_HIP_STAT_DECOMPOSITIONS] += 1;
         for (int s = 0; s < p.stages; ++s)
         {
@@ -656,32 +672,32 @@ namespace
           double* Ks = L.K + static_cast<size_t>(s) * n;
           if (s == 0)
           {
-            for (int i = lane; i < n; i += kCoopThreads)
+            for (int i = lane; i < n; i += NT)
               Ks[i] = L.f0[i];
           }
           else if (p.new_f[s])
           {
-            for (int i = lane; i < n; i += kCoopThreads)
+            for (int i = lane; i < n; i += NT)
             {
               double v = L.y[i];
               for (int j = 0; j < s; ++j)
                 v += p.a[sc + j] * L.K[static_cast<size_t>(j) * n + i];
               L.ynew[i] = v;
             }
-            coop_sync();
-            coop_forcing(m, c, L, L.ynew, Ks);
+            coop_sync<NT>();
+            coop_forcing<NT>(m, c, L, L.ynew, Ks);
             cc.stats[MICM_HIP_STAT_FUNCTION_CALLS] += 1;
           }
-          coop_sync();
+          coop_sync<NT>();
           if (s + 1 < p.stages && !p.new_f[s + 1])
-            for (int i = lane; i < n; i += kCoopThreads)
+            for (int i = lane; i < n; i += NT)
               Ks[static_cast<size_t>(n) + i] = Ks[i];
           if (s > 0)
           {
             double coef[6];
             for (int j = 0; j < s; ++j)
               coef[j] = p.c[sc + j] / cc.H;
-            for (int i = lane; i < n; i += kCoopThreads)
+            for (int i = lane; i < n; i += NT)
             {
               double v = Ks[i];
               for (int j = 0; j < s; ++j)
@@ -689,11 +705,11 @@ namespace
               Ks[i] = v;
             }
           }
-          coop_sync();
-          coop_solve(m, c, L, T, Ks);
+          coop_sync<NT>();
+          coop_solve<NT>(c, L, T, Ks);
           cc.stats[MICM_HIP_STAT_SOLVES] += 1;
         }
-        for (int i = lane; i < n; i += kCoopThreads)
+        for (int i = lane; i < n; i += NT)
         {
           double v = L.y[i], e = 0.0;
           for (int s = 0; s < p.stages; ++s)
@@ -706,7 +722,7 @@ namespace
           const double q = e / (atol[i] + p.rtol * (a > b ? a : b));
           L.scratch[i] = q * q;
         }
-        coop_sync();
+        coop_sync<NT>();
         if (lane == 0)
         {
           double acc = 0.0;
@@ -714,18 +730,18 @@ namespace
             acc += L.scratch[i];
           s_error = smax(sqrt(acc / static_cast<double>(n > 0 ? n : 1)), 1.0e-10);
         }
-        coop_sync();
+        coop_sync<NT>();
         const double error = s_error;
         if (control_update(p, cc, error))
-          for (int i = lane; i < n; i += kCoopThreads)
+          for (int i = lane; i < n; i += NT)
             L.y[i] = L.ynew[i];
-        coop_sync();
+        coop_sync<NT>();
         if (cc.status != MICM_HIP_STATE_RUNNING)
           break;
       }
       {
         Lane ys = lane_view(y_state, g, n, cl);
-        for (int i = lane; i < n; i += kCoopThreads)
+        for (int i = lane; i < n; i += NT)
           ys[i] = smax(L.y[i], 0.0);
       }
       if (lane == 0)
@@ -735,7 +751,7 @@ namespace
           stats_out[cell * 7 + i] = cc.stats[i];
         final_time_out[cell] = cc.t;
       }
-      coop_sync();
+      coop_sync<NT>();
     }
   }
 }  // namespace
@@ -779,6 +795,9 @@ struct micm_hip_state
   int32_t* handoff_list = nullptr;
   int* handoff_count = nullptr;
   int* coop_next = nullptr;
+  // cooperative launch shape (-1 / 0: chosen from the mechanism)
+  int coop_stage = -1;
+  int coop_threads = 0;
 };
 
 namespace
@@ -804,11 +823,28 @@ namespace
     dst = reinterpret_cast<const int2*>(p);
     return 0;
   }
+  void upload_levels(void* blob, size_t& off, const LevelSchedule& l, DevLevels& d, std::vector<char>& staging)
+  {
+    upload(blob, off, l.offsets, d.offsets, staging);
+    const uint32_t* tasks = nullptr;
+    upload(blob, off, l.tasks, tasks, staging);
+    d.tasks = reinterpret_cast<const uint2*>(tasks);
+    upload(blob, off, l.terms, d.terms, staging);
+    d.n_levels = l.n_levels();
+    d.n_tasks = l.tasks.empty() ? 0 : l.n_tasks();
+    d.n_terms = static_cast<int32_t>(l.terms.size());
+  }
+
 
   template<class T>
   size_t padded_bytes(const std::vector<T>& v)
   {
     return ((v.size() * sizeof(T) + 255) & ~size_t(255)) + 256;
+  }
+
+  size_t level_bytes(const LevelSchedule& l)
+  {
+    return padded_bytes(l.offsets) + padded_bytes(l.tasks) + padded_bytes(l.terms);
   }
 
   int check_mech(const micm_hip_mechanism* m)
@@ -978,11 +1014,8 @@ namespace
              padded_bytes(t.lu_term_offsets) + padded_bytes(t.lu_term_pairs) +
              padded_bytes(t.lower_diag) + padded_bytes(t.upper_diag) + padded_bytes(t.fwd_offsets) +
              padded_bytes(t.fwd_pairs) + padded_bytes(t.bwd_offsets) + padded_bytes(t.bwd_pairs) +
-             padded_bytes(t.jac_to_alu) + padded_bytes(t.alu_diag) + padded_bytes(t.coop_div_offsets) +
-             padded_bytes(t.coop_div) + padded_bytes(t.coop_upd_offsets) + padded_bytes(t.coop_upd) +
-             padded_bytes(t.coop_fcol_offsets) + padded_bytes(t.coop_fcol) + padded_bytes(t.coop_brow_offsets) +
-             padded_bytes(t.coop_brow) + padded_bytes(t.coop_fcol16) + padded_bytes(t.coop_brow16) +
-             padded_bytes(t.coop_upd16) +
+             padded_bytes(t.jac_to_alu) + padded_bytes(t.alu_diag) + level_bytes(t.lv_lu) + level_bytes(t.lv_fwd) +
+             level_bytes(t.lv_bwd) +
              padded_bytes(t.spec_offsets) + padded_bytes(t.spec_rxn) +
              padded_bytes(t.spec_coef) + padded_bytes(t.jel_offsets) + padded_bytes(t.jel_term) +
              padded_bytes(t.jel_coef) + 256;
@@ -1030,30 +1063,9 @@ namespace
     c.n_terms = static_cast<int32_t>(t.term_process.size());
     upload(m->blob, off, t.jac_to_alu, c.jac_to_alu, staging);
     upload(m->blob, off, t.alu_diag, c.alu_diag, staging);
-    upload(m->blob, off, t.coop_div_offsets, c.div_offsets, staging);
-    upload(m->blob, off, t.coop_div, c.div, staging);
-    upload(m->blob, off, t.coop_upd_offsets, c.upd_offsets, staging);
-    upload(m->blob, off, t.coop_upd, c.upd, staging);
-    upload(m->blob, off, t.coop_fcol_offsets, c.fcol_offsets, staging);
-    upload_pairs(m->blob, off, t.coop_fcol, c.fcol, staging);
-    upload(m->blob, off, t.coop_brow_offsets, c.brow_offsets, staging);
-    upload_pairs(m->blob, off, t.coop_brow, c.brow, staging);
-    c.n_fcol = static_cast<int32_t>(t.coop_fcol16.size());
-    c.n_brow = static_cast<int32_t>(t.coop_brow16.size());
-    if (!t.coop_upd16.empty() || t.coop_upd.empty())
-    {
-      upload(m->blob, off, t.coop_fcol16, c.fcol16, staging);
-      upload(m->blob, off, t.coop_brow16, c.brow16, staging);
-      const uint32_t* u16 = nullptr;
-      upload(m->blob, off, t.coop_upd16, u16, staging);
-      c.upd16 = reinterpret_cast<const uint2*>(u16);
-    }
-    else
-    {
-      c.fcol16 = nullptr;
-      c.brow16 = nullptr;
-      c.upd16 = nullptr;
-    }
+    upload_levels(m->blob, off, t.lv_lu, c.lu, staging);
+    upload_levels(m->blob, off, t.lv_fwd, c.fwd, staging);
+    upload_levels(m->blob, off, t.lv_bwd, c.bwd, staging);
     upload(m->blob, off, t.spec_offsets, c.spec_offsets, staging);
     upload(m->blob, off, t.spec_rxn, c.spec_rxn, staging);
     upload(m->blob, off, t.spec_coef, c.spec_coef, staging);
@@ -1066,54 +1078,106 @@ namespace
     return MICM_HIP_OK;
   }
 
-  // Dynamic LDS of the cooperative kernel for this mechanism, or 0 if it does not
-  // fit one workgroup on the current device.
-  size_t coop_lds_bytes(const micm_hip_mechanism* m, int stages, bool* tables_in_lds = nullptr)
+  // Launch plan of the cooperative kernel: dynamic LDS, which tables are staged in
+  // LDS, threads per cell.  lds == 0: the mechanism does not fit.
+  struct CoopPlan
   {
-    const size_t bytes = coop_lds_doubles(m->coop.nnz_alu, m->dev.nnz_jac, m->dev.n_species, stages,
-                                          m->dev.n_reactions, m->coop.n_terms) *
-                         sizeof(double);
+    size_t lds = 0;
+    int stage = 0;
+    int threads = kWave;
+  };
+
+  size_t lds_limit()
+  {
     int dev = 0, max_lds = 0, optin = 0;
     if (hipGetDevice(&dev) != hipSuccess)
       return 0;
     (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev);
-    const size_t limit = static_cast<size_t>(std::max(max_lds, optin)) - 64;  // static __shared__ of the kernel
-    if (bytes > limit || !m->coop.fcol16 || !m->coop.brow16 || !m->coop.upd16)
-      return 0;
-    const size_t with_tables = bytes + coop_table_bytes(m->coop.n_fcol, m->coop.n_brow);
-    const bool packed = m->coop.fcol16 != nullptr && m->coop.brow16 != nullptr && m->coop.upd16 != nullptr;
-    const bool fits = packed && m->coop.n_fcol >= 0 && with_tables <= limit;
-    if (tables_in_lds)
-      *tables_in_lds = fits;
-    return fits ? with_tables : bytes;
+    return static_cast<size_t>(std::max(max_lds, optin)) - 64;  // static __shared__ of the kernel
+  }
+
+  // stage_request < 0: stage the substitution tables when they fit and the LU tables
+  // when that costs no resident cell per CU; threads_request 0: one wavefront.
+  CoopPlan plan_coop(const micm_hip_mechanism* m, int stages, int stage_request = -1, int threads_request = 0)
+  {
+    CoopPlan plan;
+    const DevCoop& c = m->coop;
+    if (!m->t.coop_levels_ok)
+      return plan;
+    const size_t base = coop_lds_doubles(c.nnz_alu, m->dev.nnz_jac, m->dev.n_species, stages, m->dev.n_reactions,
+                                         c.n_terms) *
+                        sizeof(double);
+    const size_t solve = (coop_solve_table_words(c) + 4) * sizeof(uint32_t);
+    const size_t lu = (coop_lu_table_words(c) + 2) * sizeof(uint32_t);
+    const size_t limit = lds_limit();
+    if (base > limit)
+      return plan;
+    int stage = 0;
+    if (stage_request >= 0)
+    {
+      stage = stage_request & 3;
+    }
+    else
+    {
+      constexpr size_t kLdsPerCu = 160 * 1024;
+      if (base + solve <= limit)
+        stage |= 1;
+      const size_t with_solve = base + ((stage & 1) ? solve : 0);
+      if (with_solve + lu <= limit && kLdsPerCu / (with_solve + lu) >= kLdsPerCu / with_solve)
+        stage |= 2;
+    }
+    // a requested staging that does not fit drops the LU tables first, then the rest
+    if (base + ((stage & 1) ? solve : 0) + ((stage & 2) ? lu : 0) > limit)
+      stage &= ~2;
+    if (base + ((stage & 1) ? solve : 0) > limit)
+      stage &= ~1;
+    const size_t bytes = base + ((stage & 1) ? solve : 0) + ((stage & 2) ? lu : 0);
+    plan.lds = bytes;
+    plan.stage = stage;
+    plan.threads = (threads_request == 128 || threads_request == 256) ? threads_request : kWave;
+    return plan;
+  }
+
+  size_t coop_lds_bytes(const micm_hip_mechanism* m, int stages)
+  {
+    return plan_coop(m, stages).lds;
+  }
+
+  template<int NT>
+  int launch_coop_nt(micm_hip_state* s, const DevParams& p, const CoopPlan& plan, const int32_t* list,
+                     const int* count, int64_t n_fixed, const CellControl* resume, hipStream_t st)
+  {
+    const micm_hip_mechanism* m = s->mech;
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_coop<NT>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                static_cast<int>(plan.lds)));
+    int dev = 0, cus = 0, per_cu = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coop<NT>, NT, plan.lds));
+    int64_t blocks = static_cast<int64_t>(cus) * std::max(per_cu, 1);
+    if (!list)
+      blocks = std::max<int64_t>(1, std::min(blocks, n_fixed));
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, s->coop_next);
+    hipLaunchKernelGGL(k_coop<NT>, dim3(static_cast<unsigned>(blocks)), dim3(NT), plan.lds, st, m->dev, m->coop, p,
+                       plan.stage, n_fixed, list, count, s->coop_next, resume, s->y, s->k, s->atol, s->status,
+                       s->stats, s->final_time);
+    HIP_TRY(hipGetLastError());
+    return MICM_HIP_OK;
   }
 
   int launch_coop(micm_hip_state* s, const DevParams& p, const int32_t* list, const int* count, int64_t n_fixed,
                   const CellControl* resume, hipStream_t st)
   {
-    const micm_hip_mechanism* m = s->mech;
-    bool tables_in_lds = false;
-    const size_t lds = coop_lds_bytes(m, p.stages, &tables_in_lds);
-    if (lds == 0)
+    const CoopPlan plan = plan_coop(s->mech, p.stages, s->coop_stage, s->coop_threads);
+    if (plan.lds == 0)
       return fail(MICM_HIP_ERR_UNSUPPORTED, "the cooperative form does not fit this mechanism in LDS");
-    if (!tables_in_lds && (!m->coop.fcol16 || !m->coop.brow16 || !m->coop.upd16))
-      return fail(MICM_HIP_ERR_UNSUPPORTED, "the cooperative form needs ALU positions below 65536");
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_coop), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                static_cast<int>(lds)));
-    int dev = 0, cus = 0, per_cu = 0;
-    HIP_TRY(hipGetDevice(&dev));
-    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coop, kCoopThreads, lds));
-    int64_t blocks = static_cast<int64_t>(cus) * std::max(per_cu, 1);
-    if (!list)
-      blocks = std::max<int64_t>(1, std::min(blocks, n_fixed));
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, s->coop_next);
-    hipLaunchKernelGGL(k_coop, dim3(static_cast<unsigned>(blocks)), dim3(kCoopThreads), lds, st, m->dev, m->coop, p,
-                       tables_in_lds ? 1 : 0, n_fixed, list, count, s->coop_next, resume, s->y, s->k, s->atol, s->status, s->stats,
-                       s->final_time);
-    HIP_TRY(hipGetLastError());
-    return MICM_HIP_OK;
+    switch (plan.threads)
+    {
+      case 128: return launch_coop_nt<128>(s, p, plan, list, count, n_fixed, resume, st);
+      case 256: return launch_coop_nt<256>(s, p, plan, list, count, n_fixed, resume, st);
+      default: return launch_coop_nt<kWave>(s, p, plan, list, count, n_fixed, resume, st);
+    }
   }
 }  // namespace
 
@@ -1514,6 +1578,19 @@ extern "C"
     if (!s)
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null state");
     s->handoff_steps = steps;
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_state_set_cooperative_shape(micm_hip_state* s, int32_t threads, int32_t stage)
+  {
+    if (!s)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null state");
+    if (threads != 0 && threads != 64 && threads != 128 && threads != 256)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "threads per cell must be 0, 64, 128 or 256");
+    if (stage < -1 || stage > 3)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "stage must be -1 .. 3");
+    s->coop_threads = threads;
+    s->coop_stage = stage;
     return MICM_HIP_OK;
   }
 

@@ -1,0 +1,82 @@
+// schedule_harness.cpp -- CPU execution of the cooperative form's level schedules
+// (micm_amd/csrc/mechanism.cpp), for tests/test_schedule.py.
+//
+// Every level reads the values as they stood when the level began, so a task that
+// depended on another task of its own level would read a stale value and the result
+// would differ from the sequential Doolittle factorisation.
+#include "../micm_amd/csrc/mechanism.hpp"
+
+#include <cstring>
+#include <vector>
+
+using micm_hip::LevelSchedule;
+using micm_hip::MechanismTables;
+
+namespace
+{
+  // out = (out - sum A[lo] X[hi]) / A[diag], every task of a level from a snapshot
+  void run(const LevelSchedule& s, const double* A_now, std::vector<double>& A_state, std::vector<double>& X_state,
+           bool out_is_a, bool x_is_a)
+  {
+    for (int32_t l = 0; l < s.n_levels(); ++l)
+    {
+      const std::vector<double> A0 = A_state, X0 = X_state;
+      const double* A = out_is_a ? A0.data() : A_now;
+      const double* X = x_is_a ? A0.data() : X0.data();
+      for (int32_t t = s.offsets[l]; t < s.offsets[l + 1]; ++t)
+      {
+        const uint32_t o = s.tasks[2 * t] & 0xffffu, d = s.tasks[2 * t] >> 16;
+        double v = out_is_a ? A0[o] : X0[o];
+        for (uint32_t q = s.tasks[2 * t + 1]; q < s.tasks[2 * t + 3]; ++q)
+          v -= A[s.terms[q] & 0xffffu] * X[s.terms[q] >> 16];
+        if (d != 0xffffu)
+          v = v / A[d];
+        (out_is_a ? A_state : X_state)[o] = v;
+      }
+    }
+  }
+}  // namespace
+
+extern "C"
+{
+  // sizes: n_species, nnz_jac, nnz_alu, nnz_lower, nnz_upper, levels (lu, fwd, bwd), ok
+  int harness_sizes(const micm_hip_mechanism_desc* desc, int64_t* sizes)
+  {
+    const MechanismTables t = micm_hip::compile_mechanism(*desc);
+    sizes[0] = t.n_species;
+    sizes[1] = t.nnz_jacobian();
+    sizes[2] = t.nnz_alu();
+    sizes[3] = t.nnz_lower();
+    sizes[4] = t.nnz_upper();
+    sizes[5] = t.lv_lu.n_levels();
+    sizes[6] = t.lv_fwd.n_levels();
+    sizes[7] = t.lv_bwd.n_levels();
+    sizes[8] = t.coop_levels_ok ? 1 : 0;
+    return 0;
+  }
+
+  // One cell: -J (state order, CSR) with alpha added on the diagonal, factorised in
+  // place by the LU schedule; L (unit diagonal) and U scattered out in the reference's
+  // storage order; then b is solved in place by the substitution schedules.
+  int harness_factor_solve(const micm_hip_mechanism_desc* desc, const double* jac, double alpha, double* lower,
+                           double* upper, double* b)
+  {
+    const MechanismTables t = micm_hip::compile_mechanism(*desc);
+    const int32_t n = static_cast<int32_t>(t.n_species);
+    std::vector<double> alu(t.nnz_alu(), 0.0), none;
+    for (size_t e = 0; e < t.jac_to_alu.size(); ++e)
+      alu[t.jac_to_alu[e]] = jac[e];
+    for (int32_t i = 0; i < n; ++i)
+      alu[t.alu_diag[i]] += alpha;  // as AlphaMinusJacobian on -J
+    run(t.lv_lu, nullptr, alu, none, true, true);
+    for (size_t p = 0; p < t.alu_lower.size(); ++p)
+      lower[p] = t.alu_lower[p] >= 0 ? alu[t.alu_lower[p]] : 1.0;
+    for (size_t p = 0; p < t.alu_upper.size(); ++p)
+      upper[p] = alu[t.alu_upper[p]];
+    std::vector<double> x(b, b + n);
+    run(t.lv_fwd, alu.data(), none, x, false, false);
+    run(t.lv_bwd, alu.data(), none, x, false, false);
+    std::memcpy(b, x.data(), sizeof(double) * n);
+    return 0;
+  }
+}

@@ -122,11 +122,12 @@ def test_kernels_match_oracle(case, n_cells):
     assert bits_equal(e_dev.cpu().numpy(), e_ref)
 
 
-def _solve(mech, cells, order, tableau, dt, form, n_solves=1, rate_constants=None, handoff=-1):
+def _solve(mech, cells, order, tableau, dt, form, n_solves=1, rate_constants=None, handoff=-1, coop_shape=(0, -1)):
     """n_solves x (UpdateStateParameters, Solve); with rate_constants given, those are
     loaded instead of being computed on the device."""
     state = _lib.DeviceState(mech, cells.n_cells)
     state.set_handoff(handoff)
+    state.set_cooperative_shape(*coop_shape)
     state.set_variables(cells.variables[:, order])
     state.set_conditions(cells.conditions)
     state.set_custom_parameters(cells.custom)
@@ -171,30 +172,6 @@ def test_per_cell_solve_bitwise(case, form):
     assert bits_equal(y_gpu, y_ref)
 
 
-@pytest.mark.parametrize("form", FORMS)
-def test_per_cell_solve_exact_with_shared_rate_constants(case, form):
-    """Given the reference's rate constants (C library math), every cell takes the
-    reference's exact step sequence: the rest of the path has no other deviation."""
-    name, desc, inputs, tableau, dt, mech, oracle, oracle_pm = case
-    if form == _lib.SOLVE_COOPERATIVE and not mech.cooperative_supported(tableau):
-        pytest.skip("mechanism does not fit the cooperative form's LDS budget")
-    n_cells = _n_cells(name)
-    print(f"exact solve, shared k: {name} form={form} n_cells={n_cells}", flush=True)
-    cells = inputs(desc, n_cells)
-    order = mech.species_order()
-    k = oracle.rate_constants(cells.conditions, cells.custom)
-    y_gpu, st_gpu, stats_gpu, ft_gpu = _solve(mech, cells, order, tableau, dt, form, 1, rate_constants=k)
-    y_ref, st_ref, stats_ref, ft_ref = oracle.solve(cells.conditions, cells.custom, cells.variables[:, order], dt, 1)
-    # the step controller's pow(error, 1/ELO) is the one remaining difference: cells whose
-    # step sequence it changes are few; every other cell is identical to the reference
-    same = (stats_gpu == stats_ref).all(1)
-    assert np.mean(same) >= 0.9
-    assert np.array_equal(st_gpu[same], st_ref[same])
-    assert bits_equal(ft_gpu[same], ft_ref[same])
-    print(f"  identical step sequence in {np.mean(same):.4f} of cells; bitwise y in "
-          f"{np.mean((y_gpu == y_ref).all(1)):.4f}", flush=True)
-
-
 @pytest.mark.parametrize("form", [_lib.SOLVE_PERSISTENT])
 def test_per_cell_solve_against_reference_math(case, form):
     """Against the C-library restatement (bitwise equal to the reference): species above
@@ -231,8 +208,9 @@ def test_per_cell_solve_against_reference_math(case, form):
 @pytest.mark.parametrize("name", ["60_200", "210_560"])
 def test_per_cell_forms_agree_bitwise(name):
     """Every per-cell form runs the same per-cell arithmetic (the cooperative form
-    with its work spread over a wavefront without reordering any sum), so plain,
-    persistent (with and without hand-off) and cooperative results are identical."""
+    with its work spread over a workgroup without reordering any sum), so plain,
+    persistent (with and without hand-off) and cooperative results (every launch
+    shape and table staging) are identical."""
     desc, inputs, tableau, dt = CASES[name]
     mech = _lib.Mechanism(desc)
     cells = inputs(desc, 700 if name == "60_200" else 130, seed=21)
@@ -242,7 +220,8 @@ def test_per_cell_forms_agree_bitwise(name):
     runs = [_solve(mech, cells, order, tableau, dt, _lib.SOLVE_PERSISTENT, handoff=0),
             _solve(mech, cells, order, tableau, dt, _lib.SOLVE_PERSISTENT, handoff=3)]
     if mech.cooperative_supported(tableau):
-        runs.append(_solve(mech, cells, order, tableau, dt, _lib.SOLVE_COOPERATIVE))
+        for threads, stage in ((64, -1), (128, 0), (256, 3), (64, 1)):
+            runs.append(_solve(mech, cells, order, tableau, dt, _lib.SOLVE_COOPERATIVE, coop_shape=(threads, stage)))
     for r in runs:
         assert bits_equal(r[0], ref[0])
         assert np.array_equal(r[1], ref[1]) and np.array_equal(r[2], ref[2])

@@ -1,0 +1,75 @@
+"""The cooperative form's level schedules (LU, forward and back substitution),
+executed on the CPU with every level reading the values as they stood when the
+level began: they must reproduce the sequential Doolittle factorisation and
+substitution of the oracle bit for bit.  A task scheduled at the level of one of
+its inputs would read a stale value and fail this."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import bits_equal
+from mechanisms import all_rate_types
+from micm_amd import synthetic
+from oracle import Oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int64)
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("harness") / "libschedule_harness.so")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-fPIC", "-shared", "-ffp-contract=off",
+                    os.path.join(ROOT, "tests", "schedule_harness.cpp"),
+                    os.path.join(ROOT, "micm_amd", "csrc", "mechanism.cpp"), "-o", out], check=True)
+    lib = ctypes.CDLL(out)
+    lib.harness_sizes.argtypes = [ctypes.c_void_p, _ip]
+    lib.harness_factor_solve.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp, _dp, _dp]
+    return lib
+
+
+CASES = {
+    "readme": (synthetic.readme_mechanism, lambda d, n: synthetic.readme_inputs(n)),
+    "all_types": (all_rate_types, None),
+    "20_50": (synthetic.mechanism_20_50, synthetic.inputs_60_200),
+    "60_200": (synthetic.mechanism_60_200, synthetic.inputs_60_200),
+    "210_560": (synthetic.mechanism_210_560, synthetic.inputs_210_560),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_level_schedules_reproduce_doolittle(harness, name):
+    make, inputs = CASES[name]
+    desc = make()
+    cdesc = desc.as_c()
+    sizes = np.zeros(9, dtype=np.int64)
+    harness.harness_sizes(ctypes.byref(cdesc), sizes.ctypes.data_as(_ip))
+    n, nnz_j, nnz_alu, nnz_l, nnz_u, lv_lu, lv_f, lv_b, ok = (int(v) for v in sizes)
+    assert ok == 1
+    # levels never exceed the sequential step counts
+    assert lv_f <= n and lv_b <= n and lv_lu <= nnz_alu
+    o = Oracle(desc, 4)
+    rng = np.random.default_rng(7)
+    if inputs is None:
+        y = rng.uniform(1e-6, 1.0, (3, n))
+        cond = np.tile([280.0, 9e4, 38.0], (3, 1))
+        custom = rng.uniform(1e-6, 1e-2, (3, o.n_custom))
+    else:
+        cells = inputs(desc, 3)
+        y, cond, custom = cells.variables[:, o.species_order()], cells.conditions, cells.custom
+    k = o.rate_constants(cond, custom)
+    jac = o.jacobian(k, y)
+    f = o.forcing(k, y)
+    for c in range(3):
+        alpha = [1.0 / (300.0 * 0.57282), 3.7e2, 1e-3][c]
+        _, lo_ref, up_ref = o.lu(jac[c:c + 1], alpha)
+        x_ref = o.lu_solve(lo_ref, up_ref, f[c:c + 1])
+        lo, up, x = np.zeros(nnz_l), np.zeros(nnz_u), f[c].copy()
+        harness.harness_factor_solve(ctypes.byref(cdesc), np.ascontiguousarray(jac[c]).ctypes.data_as(_dp), alpha,
+                                     lo.ctypes.data_as(_dp), up.ctypes.data_as(_dp), x.ctypes.data_as(_dp))
+        assert bits_equal(lo, lo_ref[0]) and bits_equal(up, up_ref[0])
+        assert bits_equal(x, x_ref[0])
