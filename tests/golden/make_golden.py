@@ -16,19 +16,22 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(HERE))
 
-from mechanisms import all_rate_types  # noqa: E402
-from micm_amd import synthetic  # noqa: E402
+import subprocess  # noqa: E402
+
+from mechanisms import TS1_JSON, all_rate_types, ts1_inputs, ts1_mechanism  # noqa: E402
+from micm_amd import mechanism_io, synthetic  # noqa: E402
 from oracle import Reference, reference_available  # noqa: E402
 
-
-def all_types_inputs(desc, n, seed=5):
-    st = synthetic.CellStreams(seed, n)
-    T = st.uniform(200.0, 320.0)
-    P = st.uniform(5e3, 1.1e5)
-    cond = np.stack([T, P, P / (synthetic.GAS_CONSTANT * T)], 1)
-    custom = np.stack([st.log_uniform(1e-6, 1e-1), st.log_uniform(1e-8, 1e-5), st.log_uniform(1e6, 1e10)], 1)
-    y = np.stack([st.log_uniform(1e-8, 1.0) for _ in range(desc.n_species)], 1)
-    return synthetic.CellInputs(cond, y, custom, seed)
+def export_ts1():
+    """tests/golden/ts1_mechanism.json from the reference's TS1 benchmark header
+    (oracle/_ref/ts1_export, built by `make -C oracle ts1`), with the compiled orders
+    and patterns of this project's mechanism compiler attached."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ts1_export")
+    subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ts1"], check=True)
+    import json
+    d = json.loads(subprocess.run([exe], check=True, capture_output=True, text=True).stdout)
+    desc = mechanism_io.from_dict(d)
+    mechanism_io.save(desc, TS1_JSON, include_compiled=True, initial=d["initial"])
 
 
 CASES = {
@@ -37,6 +40,8 @@ CASES = {
     "all_types": (all_rate_types, all_types_inputs, 4, 60.0, 16, 3),
     "60_200": (synthetic.mechanism_60_200, synthetic.inputs_60_200, 4, 300.0, 12, 1),
     "210_560": (synthetic.mechanism_210_560, synthetic.inputs_210_560, 4, 900.0, 4, 1),
+    # the reference's TS1 benchmark mechanism, its 3-stage solver and 30 s step
+    "ts1": (ts1_mechanism, ts1_inputs, 3, 30.0, 6, 3),
 }
 ALPHA = 1.0 / (37.5 * 0.5728200000000000)
 
@@ -77,5 +82,8 @@ def make(name):
 if __name__ == "__main__":
     if not reference_available():
         sys.exit("oracle/_ref/libmicm_ref.so is not built (make -C oracle ref)")
-    for name in CASES:
+    names = sys.argv[1:] or list(CASES)
+    if "ts1" in names:
+        export_ts1()
+    for name in names:
         make(name)

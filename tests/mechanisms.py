@@ -1,4 +1,9 @@
 """Mechanisms used by the tests besides the synthetic ones."""
+import os
+
+import numpy as np
+
+from micm_amd import mechanism_io, synthetic
 from micm_amd._desc import (ARRHENIUS, BRANCHED, REVERSIBLE, SURFACE, TAYLOR, TERNARY, TROE, TUNNELING,
                             USER_DEFINED, MechanismDesc)
 
@@ -23,3 +28,27 @@ def all_rate_types():
         ([1, 1], [(6, 1.0)], ARRHENIUS, [5.0e2, 0.0, 100.0, 300.0, 0.0]),
     ]
     return MechanismDesc(8, rx, parameterized=[(0.0, 0.0, 0.0, 1.0), (0.5, 1.0e-3, 0.0, 0.0)])
+
+
+TS1_JSON = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ts1_mechanism.json")
+
+
+def ts1_mechanism():
+    """The reference's TS1 benchmark mechanism (test/benchmark/ts1_mechanism.hpp:
+    209 species plus the third body M, 547 reactions), as exported from the
+    reference's own header into tests/golden/ts1_mechanism.json."""
+    return mechanism_io.load(TS1_JSON)
+
+
+def ts1_inputs(desc, n, seed=13):
+    """The reference benchmark's cell (bench::Ts1::InitState) in cell 0; the other
+    cells vary its temperature, pressure and custom parameters."""
+    cond0, y, custom = mechanism_io.initial_arrays(desc, n)
+    st = synthetic.CellStreams(seed, n)
+    first = np.arange(n) == 0
+    T = np.where(first, cond0[:, 0], st.uniform(220.0, 310.0))
+    P = np.where(first, cond0[:, 1], st.log_uniform(2e4, 1.01e5))
+    scale = np.where(first, 1.0, st.log_uniform(0.1, 3.0))
+    cond = np.stack([T, P, P / (synthetic.GAS_CONSTANT * T)], 1)
+    cond[0] = cond0[0]
+    return synthetic.CellInputs(cond, y, custom * scale[:, None], seed)

@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 from helpers import bits_equal, dev, dev_zeros, host, max_rel, stream
-from mechanisms import all_rate_types
+from mechanisms import all_rate_types, ts1_inputs, ts1_mechanism
 from micm_amd import _lib, synthetic
 from oracle import Oracle, OraclePortableMath
 
@@ -32,6 +32,8 @@ def _cases():
         "all_types": (all_rate_types(), _all_types_inputs, 4, 60.0),
         "60_200": (synthetic.mechanism_60_200(), synthetic.inputs_60_200, 4, 300.0),
         "210_560": (synthetic.mechanism_210_560(), synthetic.inputs_210_560, 4, 900.0),
+        # the reference's TS1 benchmark mechanism with its benchmark's 3-stage solver and 30 s step
+        "ts1": (ts1_mechanism(), ts1_inputs, 3, 30.0),
     }
 
 
@@ -159,7 +161,7 @@ def test_per_cell_solve_bitwise(case, form):
     if form == _lib.SOLVE_COOPERATIVE and not mech.cooperative_supported(tableau):
         pytest.skip("mechanism does not fit the cooperative form's LDS budget")
     n_cells = _n_cells(name)
-    n_solves = 3 if name in ("readme", "all_types") else 1
+    n_solves = 3 if name in ("readme", "all_types", "ts1") else 1
     print(f"bitwise solve: {name} form={form} n_cells={n_cells}", flush=True)
     cells = inputs(desc, n_cells)
     order = mech.species_order()
@@ -184,7 +186,7 @@ def test_per_cell_solve_against_reference_math(case, form):
     C-library and portable-math restatements on the CPU."""
     name, desc, inputs, tableau, dt, mech, oracle, oracle_pm = case
     n_cells = _n_cells(name)
-    n_solves = 3 if name in ("readme", "all_types") else 1
+    n_solves = 3 if name in ("readme", "all_types", "ts1") else 1
     print(f"solve vs reference math: {name} n_cells={n_cells}", flush=True)
     cells = inputs(desc, n_cells)
     order = mech.species_order()
@@ -205,7 +207,7 @@ def test_per_cell_solve_against_reference_math(case, form):
         assert np.mean(scaled[both].max(axis=1) <= 10.0) >= 0.95
 
 
-@pytest.mark.parametrize("name", ["60_200", "210_560"])
+@pytest.mark.parametrize("name", ["60_200", "210_560", "ts1"])
 def test_per_cell_forms_agree_bitwise(name):
     """Every per-cell form runs the same per-cell arithmetic (the cooperative form
     with its work spread over a workgroup without reordering any sum), so plain,

@@ -10,6 +10,7 @@ import re
 import numpy as np
 import pytest
 
+from mechanisms import ts1_mechanism
 from micm_amd import _lib, synthetic
 from micm_amd import (ArrheniusRateConstantParameters, ChemicalReactionBuilder, HipSolverBuilder, Phase,
                       RosenbrockSolverParameters, Species, StoichSpecies, System)
@@ -33,7 +34,7 @@ def test_library_exports_every_declared_function():
 
 
 @pytest.mark.parametrize("name,fn", [("readme", synthetic.readme_mechanism), ("60_200", synthetic.mechanism_60_200),
-                                     ("210_560", synthetic.mechanism_210_560)])
+                                     ("210_560", synthetic.mechanism_210_560), ("ts1", ts1_mechanism)])
 def test_mechanism_compiler_matches_reference(name, fn):
     g = np.load(os.path.join(GOLDEN, f"{name}.npz"))
     m = _lib.Mechanism(fn())  # host compilation only; nothing touches the device
@@ -112,3 +113,45 @@ def test_bad_mechanisms_are_rejected():
     with pytest.raises(_lib.MicmHipError) as e:
         _lib.Mechanism(bad_type)
     assert e.value.code == -1
+
+
+def test_mechanism_file_round_trip(tmp_path):
+    """The on-disk description reproduces the mechanism exactly, and a file whose
+    stored orders / patterns disagree with the compiler is rejected."""
+    import json
+
+    from micm_amd import mechanism_io
+    desc = synthetic.mechanism_60_200()
+    path = tmp_path / "m.json"
+    mechanism_io.save(desc, str(path), include_compiled=True)
+    back = mechanism_io.load(str(path))
+    for key in ("reactant_offsets", "reactants", "product_offsets", "products", "yields", "rate_types", "rate_params"):
+        assert np.array_equal(getattr(back, key), getattr(desc, key)), key
+    a, b = _lib.Mechanism(desc), _lib.Mechanism(back)
+    assert np.array_equal(a.species_order(), b.species_order())
+    d = json.loads(path.read_text())
+    d["compiled"]["species_order"][0], d["compiled"]["species_order"][1] = \
+        d["compiled"]["species_order"][1], d["compiled"]["species_order"][0]
+    with pytest.raises(mechanism_io.MechanismFileError):
+        mechanism_io.from_dict(d)
+    d = json.loads(path.read_text())
+    d["reactions"][0]["type"] = "no_such_type"
+    with pytest.raises(mechanism_io.MechanismFileError):
+        mechanism_io.from_dict(d)
+
+
+def test_ts1_file_matches_the_reference_benchmark():
+    """tests/golden/ts1_mechanism.json was exported from the reference's TS1 header:
+    its sizes are the reference's (test/benchmark/ts1_mechanism.hpp:9), its custom
+    parameters are those bench::Ts1::InitState sets, and the compiler's species
+    order and fill match the reference's (golden ts1.npz)."""
+    from micm_amd import mechanism_io
+    desc = ts1_mechanism()
+    assert desc.n_species == 209 and desc.n_reactions == 547
+    types = np.bincount(desc.rate_types[:desc.n_reactions], minlength=9)
+    assert types[0] == 361 and types[1] == 31 and types[7] == 142 and types[8] == 13
+    labels = mechanism_io.custom_parameter_labels(desc)
+    assert len(labels) == 168 and set(labels) == set(desc.initial["custom"])
+    m = _lib.Mechanism(desc)
+    info = m.info
+    assert (info.nnz_jacobian, info.nnz_lower, info.nnz_upper) == (1950, 1453, 1137)

@@ -7,14 +7,14 @@ import numpy as np
 import pytest
 
 from helpers import bits_equal
-from mechanisms import all_rate_types
+from mechanisms import all_rate_types, ts1_inputs, ts1_mechanism
 from micm_amd import synthetic
 from oracle import Oracle, Reference, reference_available
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-CASES = ["readme", "all_types", "60_200", "210_560"]
+CASES = ["readme", "all_types", "60_200", "210_560", "ts1"]
 MECHS = {"readme": synthetic.readme_mechanism, "all_types": all_rate_types, "60_200": synthetic.mechanism_60_200,
-         "210_560": synthetic.mechanism_210_560}
+         "210_560": synthetic.mechanism_210_560, "ts1": ts1_mechanism}
 
 
 def golden(name):
@@ -61,14 +61,16 @@ def test_cell_grouping_matches_reference():
 @pytest.mark.parametrize("name", CASES)
 def test_oracle_matches_live_reference(name):
     desc = MECHS[name]()
-    tableau = 3 if name == "readme" else 4
+    tableau = 3 if name in ("readme", "ts1") else 4
     ref, o = Reference(desc, tableau), Oracle(desc, tableau)
-    n = 40 if name != "210_560" else 8
+    n = 40 if name not in ("210_560", "ts1") else 8
     if name == "readme":
         cells = synthetic.readme_inputs(n, seed=9)
     elif name == "all_types":
         from test_gpu_path import _all_types_inputs
         cells = _all_types_inputs(desc, n, seed=9)
+    elif name == "ts1":
+        cells = ts1_inputs(desc, n, seed=9)
     else:
         cells = synthetic.random_inputs(desc, n, 9, (190.0, 320.0), log_pressure=(2e3, 1.05e5), night_fraction=0.3)
     y = cells.variables[:, ref.species_order()]

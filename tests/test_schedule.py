@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from helpers import bits_equal
-from mechanisms import all_rate_types
+from mechanisms import all_rate_types, ts1_inputs, ts1_mechanism
 from micm_amd import synthetic
 from oracle import Oracle
 
@@ -38,6 +38,7 @@ CASES = {
     "20_50": (synthetic.mechanism_20_50, synthetic.inputs_60_200),
     "60_200": (synthetic.mechanism_60_200, synthetic.inputs_60_200),
     "210_560": (synthetic.mechanism_210_560, synthetic.inputs_210_560),
+    "ts1": (ts1_mechanism, ts1_inputs),
 }
 
 
