@@ -6,16 +6,20 @@ Workload (default): seeded random mechanism of 210 species and 560 reactions
 photolysis zero in a random 50 % of cells, FP64, 4-stage Rosenbrock, one
 Solve of 900 s per step.  Inputs are resident in HBM before timing starts; a
 step restores the initial concentrations (device-to-device copy) and runs
-Solver::Solve with the persistent form.
+Solver::Solve with the per-cell form the autotuner picks (micm_amd/autotune.py;
+every form gives bit-identical results).  `--workload ts1` runs the reference's
+own benchmark instead: TS1, 10,000 cells, 3-stage, 30 s steps.
 
-Prints one JSON line (rank 0).  `roofline` relates the algorithmic bytes of the
-path -- per attempt F x forcing + Jacobian + LU + S x substitution + error, with
-per-cell byte counts of each kernel as in the design notes -- summed over the
-attempts the cells actually took, to the measured time of the solve kernels
-(HIP events on the solver's stream).  `cpu_baseline` times the reference's own
-CPU solver (or the CPU restatement, where the reference build is absent) on a
-bounded sample of the same cells with one 1-cell State per cell and one worker
-thread per host core.
+Prints one JSON line.  `roofline` relates the algorithmic bytes of the path --
+per attempt F x forcing + Jacobian + LU + S x substitution + error, with the
+per-cell byte counts of each kernel -- summed over the attempts the cells
+actually took, to the measured time of the Solve launches (HIP events on the
+solver's stream).  `plain_form` times the plain per-step form and the chosen
+form on the same strided sample of cells.  `end_to_end` includes the PCIe
+transfers of inputs and results.  `cpu_baseline` times the reference's own CPU
+solver (or the CPU restatement, where the reference build is absent) on a
+bounded sample of the same cells with one 1-cell State per cell, on every host
+core and on one.
 """
 import argparse
 import json
@@ -47,7 +51,7 @@ def algorithmic_bytes(info, n_species, n_reactions, stats):
     return int(s[0] * forcing + s[1] * jacobian + s[5] * lu + s[6] * solve + s[2] * error)
 
 
-def cpu_baseline(desc, cells, order, dt, sample, threads):
+def cpu_baseline(desc, cells, order, dt, sample, threads, tableau=4):
     """Cell-solves/s of the reference CPU solver (per-cell States) on `sample` cells."""
     from oracle import Oracle, Reference, reference_available
     kind = "reference" if reference_available() else "port"
@@ -55,13 +59,13 @@ def cpu_baseline(desc, cells, order, dt, sample, threads):
     sub = cells.subset(idx)
     y = sub.variables[:, order]
     if kind == "port":
-        os.environ["OMP_NUM_THREADS"] = str(threads)
-        solver = Oracle(desc, 4)
+        solver = Oracle(desc, tableau)
+        solver.lib.oracle_set_threads(int(threads))
         t0 = time.perf_counter()
         solver.solve(sub.conditions, sub.custom, y, dt, 1)
         el = time.perf_counter() - t0
         return kind, sample / el, el
-    solvers = [Reference(desc, 4) for _ in range(threads)]
+    solvers = [Reference(desc, tableau) for _ in range(threads)]
     chunks = np.array_split(idx, threads)
 
     def work(k):
@@ -79,13 +83,46 @@ def cpu_baseline(desc, cells, order, dt, sample, threads):
     return kind, sample / el, el
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def workload(name, n_cells):
+    """(description, cell inputs, tableau preset, time step, label) of a named workload."""
+    from micm_amd import synthetic
+    if name == "210_560":
+        desc = synthetic.mechanism_210_560()
+        cells = synthetic.inputs_210_560(desc, n_cells)
+        return desc, cells, 4, 900.0, (f"random 210-species/560-reaction mechanism (seed {desc.seed}), {n_cells} cells "
+                                      f"(seed {cells.seed}), T 180-320 K, P 1-101 kPa, 50 % night, 4-stage Rosenbrock, "
+                                      f"1 x 900 s")
+    if name == "ts1":
+        from micm_amd import mechanism_io
+        desc = mechanism_io.load(os.path.join(ROOT, "tests", "golden", "ts1_mechanism.json"))
+        cond, y, custom = mechanism_io.initial_arrays(desc, n_cells)
+        cells = synthetic.CellInputs(cond, y, custom, None)
+        return desc, cells, 3, 30.0, (f"TS1 (209 species, 547 reactions), {n_cells} identical cells of the reference "
+                                      f"benchmark's initial state, 3-stage Rosenbrock, 1 x 30 s")
+    raise SystemExit(f"unknown workload {name}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cells", type=int, default=100000)
-    ap.add_argument("--form", default="persistent", choices=["persistent", "plain", "cooperative"])
+    ap.add_argument("--workload", default="210_560", choices=["210_560", "ts1"])
+    ap.add_argument("--cells", type=int, default=0, help="cells (0: 100000 for 210_560, 10000 for ts1)")
+    ap.add_argument("--form", default="autotune", choices=["autotune", "persistent", "plain", "cooperative"])
+    ap.add_argument("--compare-sample", type=int, default=8192,
+                    help="cells on which the plain form and the chosen form are timed side by side (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=512, help="cells in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU-baseline threads (0: all host cores)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -107,27 +144,41 @@ def main():
 
     import torch
 
-    from micm_amd import _lib, synthetic
+    from micm_amd import _lib, autotune
 
+    n_cells = args.cells or (100000 if args.workload == "210_560" else 10000)
     torch.cuda.set_device(0)
-    desc = synthetic.mechanism_210_560()
-    cells = synthetic.inputs_210_560(desc, args.cells)
-    dt = 900.0
+    desc, cells, tableau, dt, label = workload(args.workload, n_cells)
     mech = _lib.Mechanism(desc)
     order = mech.species_order()
-    state = _lib.DeviceState(mech, args.cells)
+    params = _lib.rosenbrock_parameters(tableau)
+    y_host = np.ascontiguousarray(cells.variables[:, order])
+
+    # end to end from host buffers (PCIe included): inputs up, rate constants, Solve, results down
+    state = _lib.DeviceState(mech, n_cells)
     stream = torch.cuda.current_stream()
     state.set_stream(stream.cuda_stream)
-    state.set_variables(cells.variables[:, order])
+
+    t0 = time.perf_counter()
+    state.set_variables(y_host)
     state.set_conditions(cells.conditions)
     state.set_custom_parameters(cells.custom)
+    torch.cuda.synchronize()
+    upload_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     state.update_state_parameters()
     torch.cuda.synchronize()
     update_ms = (time.perf_counter() - t0) * 1e3
-    params = _lib.rosenbrock_parameters(4)
-    form = {"persistent": _lib.SOLVE_PERSISTENT, "plain": _lib.SOLVE_PLAIN,
-            "cooperative": _lib.SOLVE_COOPERATIVE}[args.form]
+
+    if args.form == "autotune":
+        choice = autotune.autotune(mech, state, dt, params)
+        form = autotune.apply(state, choice)
+        form_label = f"autotuned: {choice['label']}"
+    else:
+        form = {"persistent": _lib.SOLVE_PERSISTENT, "plain": _lib.SOLVE_PLAIN,
+                "cooperative": _lib.SOLVE_COOPERATIVE}[args.form]
+        form_label = args.form
+        choice = None
 
     ptr = state.pointers()
     n_y = ptr.n_groups * 64 * mech.n_species
@@ -152,12 +203,18 @@ def main():
     kernel_ms = start_ev.elapsed_time(end_ev) / args.steps
     status, stats, _ = state.results()
     ms_per_step = wall * 1e3 / args.steps
-    value = args.cells * args.steps / wall
+    value = n_cells * args.steps / wall
+
+    t0 = time.perf_counter()
+    y_out = state.get_variables()
+    download_s = time.perf_counter() - t0
+    end_to_end = n_cells / (upload_s + update_ms * 1e-3 + ms_per_step * 1e-3 + download_s)
 
     bytes_per_step = algorithmic_bytes(mech.info, mech.n_species, mech.n_reactions, stats)
     achieved = bytes_per_step / (kernel_ms * 1e-3) / 1e9
     result = {
-        "metric": "cell-solves/s, 210-species x 100k cells, persistent batched Rosenbrock (FP64)",
+        "metric": "cell-solves/s, 210-species x 100k cells, batched Rosenbrock (FP64)"
+        if args.workload == "210_560" else "cell-solves/s, TS1 x 10k cells (reference benchmark mechanism)",
         "value": round(value, 2),
         "unit": "cell-solves/s",
         "n_gpus": 1,
@@ -168,24 +225,54 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded mechanism and cells)",
-        "config": {"workload": f"random 210-species/560-reaction mechanism, {args.cells} cells, 4-stage Rosenbrock, "
-                               f"1 x 900 s, 50 % night cells", "form": args.form, "cells": args.cells,
-                   "mechanism_seed": desc.seed, "cells_seed": cells.seed},
+        "data": "synthetic (seeded mechanism and cells)" if args.workload == "210_560"
+                else "the reference's TS1 benchmark mechanism and initial state",
+        "config": {"workload": label, "form": form_label, "cells": n_cells, "tableau_stages": int(params.stages),
+                   "time_step_s": dt},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "solve kernels (k_persistent + k_coop), algorithmic bytes per solve "
-                               f"{bytes_per_step}"},
+                     "kernel": "the Solve launch (one persistent or cooperative kernel per step); algorithmic bytes "
+                               f"per step {bytes_per_step} summed over the attempts the cells took"},
         "attempts_per_step": int(stats[:, 2].sum()),
         "converged_fraction": round(float(np.mean(status == 2)), 4),
         "update_state_parameters_ms": round(update_ms, 3),
+        "end_to_end": {"value": round(end_to_end, 2), "unit": "cell-solves/s",
+                       "note": "host inputs uploaded, rate constants, one Solve, results downloaded (PCIe included)"},
     }
+    if choice is not None:
+        result["autotune"] = {k: (round(v * 1e3, 2) if isinstance(v, float) else v)
+                              for k, v in choice["seconds"].items()}
+        result["autotune_sample_cells"] = choice["sample"]
+    if args.compare_sample > 0:
+        m = min(args.compare_sample, n_cells)
+        sub = cells.subset(np.linspace(0, n_cells - 1, m).round().astype(np.int64))
+        probe = _lib.DeviceState(mech, m)
+        probe.set_variables(sub.variables[:, order])
+        probe.set_conditions(sub.conditions)
+        probe.set_custom_parameters(sub.custom)
+        probe.update_state_parameters()
+        if choice is not None:
+            autotune.apply(probe, choice)
+        cmp = {}
+        for name, f in (("plain", _lib.SOLVE_PLAIN), ("chosen", form)):
+            probe.set_variables(sub.variables[:, order])
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            probe.solve(dt, params, f)
+            cmp[name] = m / (time.perf_counter() - t0)
+        result["plain_form"] = {"value": round(cmp["plain"], 2), "unit": "cell-solves/s", "cells": m,
+                                "chosen_form_same_cells": round(cmp["chosen"], 2),
+                                "speedup": round(cmp["chosen"] / cmp["plain"], 3)}
     if args.cpu_sample > 0:
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
-        kind, cps, el = cpu_baseline(desc, cells, order, dt, min(args.cpu_sample, args.cells), threads)
+        sample = min(args.cpu_sample, n_cells)
+        kind, cps, el = cpu_baseline(desc, cells, order, dt, sample, threads, tableau)
         result["cpu_baseline"] = {"value": round(cps, 2), "unit": "cell-solves/s", "cores": threads, "kind": kind,
-                                  "sample": f"first {min(args.cpu_sample, args.cells)} cells of the same workload, "
-                                            f"1-cell States, {el:.1f} s wall"}
+                                  "cpu": cpu_model(),
+                                  "sample": f"first {sample} cells of the same workload, one 1-cell State per "
+                                            f"cell, {el:.1f} s wall"}
+        kind1, cps1, el1 = cpu_baseline(desc, cells, order, dt, max(8, sample // threads), 1, tableau)
+        result["cpu_baseline"]["single_core_value"] = round(cps1, 2)
     print(json.dumps(result), flush=True)
 
 

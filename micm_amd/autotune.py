@@ -1,0 +1,85 @@
+"""Launch-shape autotuner for the per-cell forms.
+
+The fastest way to run a mechanism depends on its size and on how unevenly its
+cells behave: the per-lane persistent form (one cell per lane, with or without
+handing slow cells to the cooperative form), or the cooperative form (one
+workgroup per cell) with 64, 128 or 256 threads and with its constant tables
+staged in LDS or read from global memory.  `autotune` times every candidate on a
+strided sample of the state's own cells and returns the fastest.  Every
+candidate produces bit-identical results, so the choice only affects speed.
+
+Results are cached per (mechanism layout, tableau, time step, sample) in this
+process.
+"""
+import time
+
+import numpy as np
+
+from . import _lib
+
+# (label, form, handoff steps, cooperative threads, staging)
+CANDIDATES = (
+    ("persistent", _lib.SOLVE_PERSISTENT, 0, 0, -1),
+    ("persistent+handoff16", _lib.SOLVE_PERSISTENT, 16, 0, -1),
+    ("persistent+handoff48", _lib.SOLVE_PERSISTENT, 48, 0, -1),
+    ("cooperative64", _lib.SOLVE_COOPERATIVE, -1, 64, -1),
+    ("cooperative128", _lib.SOLVE_COOPERATIVE, -1, 128, -1),
+    ("cooperative256", _lib.SOLVE_COOPERATIVE, -1, 256, -1),
+    ("cooperative64-global-tables", _lib.SOLVE_COOPERATIVE, -1, 64, 0),
+    ("cooperative128-global-tables", _lib.SOLVE_COOPERATIVE, -1, 128, 0),
+)
+
+_cache = {}
+
+
+def _key(mech, tableau_stages, dt, n):
+    i = mech.info
+    return (i.n_species, i.n_reactions, i.nnz_jacobian, i.nnz_lower, i.nnz_upper, i.lu_multiply_adds,
+            tableau_stages, float(dt), int(n))
+
+
+def apply(state, choice):
+    """Configure `state` with a choice returned by `autotune`."""
+    state.set_handoff(choice["handoff"])
+    state.set_cooperative_shape(choice["threads"], choice["stage"])
+    return choice["form"]
+
+
+def autotune(mech, state, dt, params, sample=2048, repeats=1, candidates=CANDIDATES, verbose=False):
+    """Pick the fastest per-cell form for `state` (whose inputs and rate constants
+    are set) on a strided sample of its cells.  Returns a dict with keys label,
+    form, handoff, threads, stage, seconds (per candidate: label -> seconds)."""
+    n = state.n_cells
+    m = min(sample, n)
+    key = _key(mech, params.stages, dt, m)
+    if key in _cache:
+        return dict(_cache[key])
+    idx = np.linspace(0, n - 1, m).round().astype(np.int64)
+    y = state.get_variables()[idx]
+    k = state.get_rate_constants()[idx]
+    probe = _lib.DeviceState(mech, m)
+    probe.set_rate_constants(k)
+    supported = mech.cooperative_supported(params.stages)
+    times = {}
+    best = None
+    for label, form, handoff, threads, stage in candidates:
+        if form == _lib.SOLVE_COOPERATIVE and not supported:
+            continue
+        probe.set_handoff(handoff)
+        probe.set_cooperative_shape(threads, stage)
+        el = []
+        for _ in range(repeats + 1):  # the first run compiles / warms up
+            probe.set_variables(y)
+            t0 = time.perf_counter()
+            probe.solve(dt, params, form)
+            el.append(time.perf_counter() - t0)
+        times[label] = min(el[1:])
+        if verbose:
+            print(f"autotune: {label:32s} {times[label] * 1e3:9.2f} ms", flush=True)
+        if best is None or times[label] < times[best[0]]:
+            best = (label, form, handoff, threads, stage)
+    label, form, handoff, threads, stage = best
+    choice = {"label": label, "form": form, "handoff": handoff, "threads": threads, "stage": stage,
+              "sample": m, "seconds": times}
+    _cache[key] = dict(choice)
+    return choice

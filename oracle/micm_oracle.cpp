@@ -41,6 +41,8 @@
 #define RC_LOG10 std::log10
 #endif
 
+#include <omp.h>
+
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -1037,6 +1039,15 @@ extern "C"
     }
     return 0;
   }
+}
+
+// Worker threads of the per-cell solves (OpenMP).
+extern "C" int oracle_set_threads(int32_t n)
+{
+  if (n < 1)
+    return -1;
+  omp_set_num_threads(n);
+  return 0;
 }
 
 #ifdef ORACLE_PORTABLE_MATH
