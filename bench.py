@@ -83,6 +83,58 @@ def cpu_baseline(desc, cells, order, dt, sample, threads, tableau=4):
     return kind, sample / el, el
 
 
+def kernel_rooflines(mech, state, stream, n_cells, repeats=5):
+    """Per-kernel time of the single-phase entry points (one cell per lane) over the
+    whole State, and their algorithmic bytes as a fraction of the HBM peak."""
+    import torch
+    N, R, P = mech.n_species, mech.n_reactions, mech.n_custom
+    i = mech.info
+    ptr = state.pointers()
+    groups = ptr.n_groups
+    f = torch.zeros(groups * 64 * N, dtype=torch.float64, device="cuda")
+    x = torch.zeros_like(f)
+    err = torch.zeros(n_cells, dtype=torch.float64, device="cuda")
+    st = stream.cuda_stream
+    per_cell = {
+        "rate_constants": 8 * (3 + P + R),
+        "add_forcing": 8 * (2 * N + R),
+        "subtract_jacobian": 8 * (N + R + i.nnz_jacobian),
+        "alpha_minus_jacobian": 8 * 2 * N,
+        "lu_decompose": 8 * (i.nnz_jacobian + i.nnz_lower + i.nnz_upper),
+        "lu_solve": 8 * (i.nnz_lower + i.nnz_upper),
+        "normalized_error": 8 * 3 * N,
+    }
+    calls = {
+        "rate_constants": lambda: mech.calculate_rate_constants(n_cells, ptr.conditions, ptr.custom_parameters,
+                                                                ptr.rate_constants, st),
+        "add_forcing": lambda: mech.add_forcing_terms(n_cells, ptr.rate_constants, ptr.variables, f.data_ptr(), st),
+        "subtract_jacobian": lambda: mech.subtract_jacobian_terms(n_cells, ptr.rate_constants, ptr.variables,
+                                                                  ptr.jacobian, st),
+        "alpha_minus_jacobian": lambda: mech.alpha_minus_jacobian(n_cells, ptr.jacobian, 1.0e3, st),
+        "lu_decompose": lambda: mech.lu_decompose(n_cells, ptr.jacobian, ptr.lower, ptr.upper, st),
+        "lu_solve": lambda: mech.lu_solve(n_cells, ptr.lower, ptr.upper, x.data_ptr(), st),
+        "normalized_error": lambda: mech.normalized_error(n_cells, ptr.variables, ptr.variables, f.data_ptr(),
+                                                          ptr.absolute_tolerances, 1e-6, err.data_ptr(), st),
+    }
+    out = {}
+    for name, call in calls.items():
+        if name == "lu_solve":
+            x.copy_(f)
+        call()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(repeats):
+            call()
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / repeats
+        gbs = per_cell[name] * n_cells / (ms * 1e-3) / 1e9
+        out[name] = {"ms": round(ms, 4), "bytes_per_cell": per_cell[name], "GB/s": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -123,6 +175,7 @@ def main():
     ap.add_argument("--form", default="autotune", choices=["autotune", "persistent", "plain", "cooperative"])
     ap.add_argument("--compare-sample", type=int, default=8192,
                     help="cells on which the plain form and the chosen form are timed side by side (0: skip)")
+    ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel roofline of the single-phase kernels")
     ap.add_argument("--cpu-sample", type=int, default=512, help="cells in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU-baseline threads (0: all host cores)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -263,6 +316,8 @@ def main():
         result["plain_form"] = {"value": round(cmp["plain"], 2), "unit": "cell-solves/s", "cells": m,
                                 "chosen_form_same_cells": round(cmp["chosen"], 2),
                                 "speedup": round(cmp["chosen"] / cmp["plain"], 3)}
+    if not args.no_kernels:
+        result["kernels"] = kernel_rooflines(mech, state, stream, n_cells)
     if args.cpu_sample > 0:
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
         sample = min(args.cpu_sample, n_cells)

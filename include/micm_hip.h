@@ -105,6 +105,11 @@ extern "C" {
 #define MICM_HIP_SOLVE_PERSISTENT 2 /* per-cell step size, persistent work-queue kernel; cells that
                                        reach the hand-off step count continue in the cooperative form */
 #define MICM_HIP_SOLVE_COOPERATIVE 3 /* per-cell step size, one wavefront per cell, working set in LDS */
+/* OR-ed into the form: UpdateStateParameters fused into the Solve launch -- each
+ * cell's rate constants are computed from its conditions and custom parameters as
+ * the cell is loaded (and stored in the state), instead of by a separate
+ * micm_hip_update_state_parameters call.  Same results. */
+#define MICM_HIP_SOLVE_UPDATE_RATES 0x100
 
 /* Flat mechanism description.  Species are the phase's non-parameterized
  * species in declaration order; reactions are in the order given to

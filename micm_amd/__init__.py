@@ -333,7 +333,7 @@ class SolverResult:
 # ---------------------------------------------------------------- builder, solver, state
 
 _FORMS = {"lockstep": SOLVE_LOCKSTEP, "plain": SOLVE_PLAIN, "persistent": SOLVE_PERSISTENT,
-          "cooperative": SOLVE_COOPERATIVE}
+          "cooperative": SOLVE_COOPERATIVE, "auto": None}
 
 
 class HipSolverBuilder:
@@ -504,7 +504,12 @@ class Solver:
                 self.custom_parameter_map[labels[r] + ".particle number concentration [# m-3]"] = p + 1
                 p += 2
 
-    def GetState(self, number_of_grid_cells=1, form="plain"):
+    def GetState(self, number_of_grid_cells=1, form="lockstep"):
+        """form: "lockstep" (default) integrates all cells with one step size, as one
+        reference State of that many cells does; "plain", "persistent",
+        "cooperative" and "auto" (autotuned at the first Solve) give every cell its
+        own step control, as one reference State per cell does.  The per-cell
+        forms return bit-identical results."""
         return State(self, int(number_of_grid_cells), _FORMS[form])
 
     def UpdateStateParameters(self, state):
@@ -515,6 +520,10 @@ class Solver:
         if parameters is not None:
             self.parameters = parameters
         state.device.set_variables(state.variables)
+        if state.form is None:
+            from . import autotune
+            choice = autotune.autotune(self.mechanism, state.device, time_step, self.parameters._c)
+            state.form = autotune.apply(state.device, choice)
         state.device.solve(time_step, self.parameters._c, state.form)
         state.SyncOutputsToHost()
         stats = SolverStats()

@@ -18,6 +18,7 @@ OK = 0
 STATE_NAMES = {0: "NotYetCalled", 1: "Running", 2: "Converged", 3: "ConvergenceExceededMaxSteps",
                4: "StepSizeTooSmall", 5: "RepeatedlySingularMatrix", 6: "NaNDetected", 7: "InfDetected"}
 SOLVE_LOCKSTEP, SOLVE_PLAIN, SOLVE_PERSISTENT, SOLVE_COOPERATIVE = 0, 1, 2, 3
+SOLVE_UPDATE_RATES = 0x100  # OR-ed into a form: UpdateStateParameters fused into the Solve launch
 PRESETS = {2: 2, 3: 3, 4: 4, "4da": 41, "6da": 61, 41: 41, 61: 61}
 
 _vp = ctypes.c_void_p

@@ -486,11 +486,12 @@ namespace
   // back, and the lane takes the next cell.  The loop ends when the queue is drained
   // and every lane is idle, so every wavefront exits.
   __global__ void __launch_bounds__(kBlock) k_persistent(DevMech m, DevParams p, int64_t n_cells, Queue q, Work slot,
-                                                         double* __restrict__ y_state, const double* __restrict__ k_state,
+                                                         double* __restrict__ y_state, double* __restrict__ k_state,
                                                          const double* __restrict__ atol, int32_t* status_out,
                                                          int64_t* stats_out, double* final_time_out,
                                                          CellControl* handoff_control, int32_t* handoff_list,
-                                                         int* handoff_count)
+                                                         int* handoff_count, const double* __restrict__ cond_state,
+                                                         const double* __restrict__ custom_state)
   {
     const int l = threadIdx.x % kGroup;
     const size_t s = static_cast<size_t>(blockIdx.x) * (kBlock / kGroup) + threadIdx.x / kGroup;
@@ -523,11 +524,27 @@ namespace
           {
             cell = q.order ? q.order[pos] : pos;
             Lane ys = lane_view(y_state, cell / kGroup, n, cell % kGroup);
-            Lane ks = lane_view(const_cast<double*>(k_state), cell / kGroup, m.n_reactions, cell % kGroup);
+            Lane ks = lane_view(k_state, cell / kGroup, m.n_reactions, cell % kGroup);
             for (int i = 0; i < n; ++i)
               y[i] = ys[i];
-            for (int r = 0; r < m.n_reactions; ++r)
-              k[r] = ks[r];
+            if (cond_state)
+            {
+              // UpdateStateParameters fused into the launch
+              Lane cd = lane_view(const_cast<double*>(cond_state), cell / kGroup, 3, cell % kGroup);
+              Lane cu = lane_view(const_cast<double*>(custom_state), cell / kGroup, m.n_custom, cell % kGroup);
+              const double T = cd[0], P = cd[1], rho = cd[2];
+              for (int r = 0; r < m.n_reactions; ++r)
+              {
+                const double kr = rate_constant(m, r, T, P, rho, cu);
+                k[r] = kr;
+                ks[r] = kr;
+              }
+            }
+            else
+            {
+              for (int r = 0; r < m.n_reactions; ++r)
+                k[r] = ks[r];
+            }
             control_init(p, cc);
           }
           else
@@ -583,8 +600,10 @@ namespace
   __global__ void __launch_bounds__(NT) k_coop(DevMech m, DevCoop c, DevParams p, int stage, int64_t n_fixed,
                                                const int32_t* __restrict__ list, const int* list_count, int* next,
                                                const CellControl* resume, double* __restrict__ y_state,
-                                               const double* __restrict__ k_state, const double* __restrict__ atol,
-                                               int32_t* status_out, int64_t* stats_out, double* final_time_out)
+                                               double* __restrict__ k_state, const double* __restrict__ atol,
+                                               int32_t* status_out, int64_t* stats_out, double* final_time_out,
+                                               const double* __restrict__ cond_state,
+                                               const double* __restrict__ custom_state)
   {
     extern __shared__ double lds[];
     __shared__ int s_idx;
@@ -635,11 +654,27 @@ namespace
       const int cl = cell % kGroup;
       {
         Lane ys = lane_view(y_state, g, n, cl);
-        Lane ks = lane_view(const_cast<double*>(k_state), g, m.n_reactions, cl);
+        Lane ks = lane_view(k_state, g, m.n_reactions, cl);
         for (int i = lane; i < n; i += NT)
           L.y[i] = ys[i];
-        for (int r = lane; r < m.n_reactions; r += NT)
-          L.k[r] = ks[r];
+        if (cond_state)
+        {
+          // UpdateStateParameters fused into the launch
+          Lane cd = lane_view(const_cast<double*>(cond_state), g, 3, cl);
+          Lane cu = lane_view(const_cast<double*>(custom_state), g, m.n_custom, cl);
+          const double T = cd[0], P = cd[1], rho = cd[2];
+          for (int r = lane; r < m.n_reactions; r += NT)
+          {
+            const double kr = rate_constant(m, r, T, P, rho, cu);
+            L.k[r] = kr;
+            ks[r] = kr;
+          }
+        }
+        else
+        {
+          for (int r = lane; r < m.n_reactions; r += NT)
+            L.k[r] = ks[r];
+        }
       }
       CellControl cc;
       if (resume)
@@ -1146,7 +1181,7 @@ namespace
 
   template<int NT>
   int launch_coop_nt(micm_hip_state* s, const DevParams& p, const CoopPlan& plan, const int32_t* list,
-                     const int* count, int64_t n_fixed, const CellControl* resume, hipStream_t st)
+                     const int* count, int64_t n_fixed, const CellControl* resume, bool update_rates, hipStream_t st)
   {
     const micm_hip_mechanism* m = s->mech;
     HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_coop<NT>), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1161,22 +1196,23 @@ namespace
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, s->coop_next);
     hipLaunchKernelGGL(k_coop<NT>, dim3(static_cast<unsigned>(blocks)), dim3(NT), plan.lds, st, m->dev, m->coop, p,
                        plan.stage, n_fixed, list, count, s->coop_next, resume, s->y, s->k, s->atol, s->status,
-                       s->stats, s->final_time);
+                       s->stats, s->final_time, update_rates ? s->cond : nullptr,
+                       update_rates ? s->custom : nullptr);
     HIP_TRY(hipGetLastError());
     return MICM_HIP_OK;
   }
 
   int launch_coop(micm_hip_state* s, const DevParams& p, const int32_t* list, const int* count, int64_t n_fixed,
-                  const CellControl* resume, hipStream_t st)
+                  const CellControl* resume, bool update_rates, hipStream_t st)
   {
     const CoopPlan plan = plan_coop(s->mech, p.stages, s->coop_stage, s->coop_threads);
     if (plan.lds == 0)
       return fail(MICM_HIP_ERR_UNSUPPORTED, "the cooperative form does not fit this mechanism in LDS");
     switch (plan.threads)
     {
-      case 128: return launch_coop_nt<128>(s, p, plan, list, count, n_fixed, resume, st);
-      case 256: return launch_coop_nt<256>(s, p, plan, list, count, n_fixed, resume, st);
-      default: return launch_coop_nt<kWave>(s, p, plan, list, count, n_fixed, resume, st);
+      case 128: return launch_coop_nt<128>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
+      case 256: return launch_coop_nt<256>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
+      default: return launch_coop_nt<kWave>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
     }
   }
 }  // namespace
@@ -1629,8 +1665,15 @@ extern "C"
     const DevMech& d = m->dev;
     hipStream_t st = s->stream;
     const unsigned grid = grid_for(s->n_cells);
+    const bool update_rates = (form & MICM_HIP_SOLVE_UPDATE_RATES) != 0;
+    form &= ~MICM_HIP_SOLVE_UPDATE_RATES;
     if (form == MICM_HIP_SOLVE_PLAIN || form == MICM_HIP_SOLVE_LOCKSTEP)
     {
+      if (update_rates)
+      {
+        hipLaunchKernelGGL(k_rate_constants, dim3(grid), dim3(kBlock), 0, st, d, s->n_cells, s->cond, s->custom, s->k);
+        HIP_TRY(hipGetLastError());
+      }
       const bool lockstep = form == MICM_HIP_SOLVE_LOCKSTEP;
       Work w = state_work(s);
       hipLaunchKernelGGL(k_plain_init, dim3(grid), dim3(kBlock), 0, st, p, s->n_cells, s->ctl);
@@ -1688,17 +1731,18 @@ extern "C"
       Queue q{ s->queue_next, s->has_order ? s->order : nullptr };
       hipLaunchKernelGGL(k_persistent, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, st, d, pp, s->n_cells, q, w,
                          s->y, s->k, s->atol, s->status, s->stats, s->final_time, s->ctl, s->handoff_list,
-                         s->handoff_count);
+                         s->handoff_count, update_rates ? s->cond : nullptr, update_rates ? s->custom : nullptr);
       HIP_TRY(hipGetLastError());
+      // handed-off cells carry the rate constants the persistent kernel stored
       if (pp.handoff_steps > 0)
-        if (int rc = launch_coop(s, p, s->handoff_list, s->handoff_count, 0, s->ctl, st))
+        if (int rc = launch_coop(s, p, s->handoff_list, s->handoff_count, 0, s->ctl, false, st))
           return rc;
       HIP_TRY(hipStreamSynchronize(st));
       return MICM_HIP_OK;
     }
     if (form == MICM_HIP_SOLVE_COOPERATIVE)
     {
-      if (int rc = launch_coop(s, p, nullptr, nullptr, s->n_cells, nullptr, st))
+      if (int rc = launch_coop(s, p, nullptr, nullptr, s->n_cells, nullptr, update_rates, st))
         return rc;
       HIP_TRY(hipStreamSynchronize(st));
       return MICM_HIP_OK;

@@ -6,6 +6,7 @@
 // would differ from the sequential Doolittle factorisation.
 #include "../micm_amd/csrc/mechanism.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -78,5 +79,27 @@ extern "C"
     run(t.lv_bwd, alu.data(), none, x, false, false);
     std::memcpy(b, x.data(), sizeof(double) * n);
     return 0;
+  }
+}
+
+extern "C"
+{
+  // Per level of schedule `which` (0 LU, 1 forward, 2 backward): tasks and the
+  // largest number of terms of one task; returns the number of levels written.
+  int harness_level_stats(const micm_hip_mechanism_desc* desc, int32_t which, int64_t* tasks, int64_t* max_terms,
+                          int64_t capacity)
+  {
+    const MechanismTables t = micm_hip::compile_mechanism(*desc);
+    const LevelSchedule& s = which == 0 ? t.lv_lu : (which == 1 ? t.lv_fwd : t.lv_bwd);
+    const int32_t nl = s.n_levels();
+    for (int32_t l = 0; l < nl && l < capacity; ++l)
+    {
+      tasks[l] = s.offsets[l + 1] - s.offsets[l];
+      int64_t m = 0;
+      for (int32_t q = s.offsets[l]; q < s.offsets[l + 1]; ++q)
+        m = std::max<int64_t>(m, s.tasks[2 * q + 3] - s.tasks[2 * q + 1]);
+      max_terms[l] = m;
+    }
+    return nl;
   }
 }

@@ -230,6 +230,32 @@ def test_per_cell_forms_agree_bitwise(name):
         assert bits_equal(r[3], ref[3])
 
 
+@pytest.mark.parametrize("form", [_lib.SOLVE_PLAIN, _lib.SOLVE_PERSISTENT, _lib.SOLVE_COOPERATIVE])
+def test_fused_rate_constants_match_separate_update(form):
+    """UpdateStateParameters fused into the Solve launch gives the same rate constants
+    and the same solve as the separate call (hand-off included)."""
+    desc, inputs, tableau, dt = CASES["60_200"]
+    mech = _lib.Mechanism(desc)
+    cells = inputs(desc, 300, seed=17)
+    order = mech.species_order()
+    print(f"fused rates: form={form}", flush=True)
+    ref = _solve(mech, cells, order, tableau, dt, form, handoff=2)
+    state = _lib.DeviceState(mech, cells.n_cells)
+    state.set_handoff(2)
+    state.set_variables(cells.variables[:, order])
+    state.set_conditions(cells.conditions)
+    state.set_custom_parameters(cells.custom)
+    state.solve(dt, _lib.rosenbrock_parameters(tableau), form | _lib.SOLVE_UPDATE_RATES)
+    st, stats, ft = state.results()
+    assert bits_equal(state.get_variables(), ref[0])
+    assert np.array_equal(st, ref[1]) and np.array_equal(stats, ref[2]) and bits_equal(ft, ref[3])
+    other = _lib.DeviceState(mech, cells.n_cells)
+    other.set_conditions(cells.conditions)
+    other.set_custom_parameters(cells.custom)
+    other.update_state_parameters()
+    assert bits_equal(state.get_rate_constants(), other.get_rate_constants())
+
+
 @pytest.mark.parametrize("name", ["all_types", "60_200"])
 def test_lockstep_solve_bitwise(name):
     """One step size for all cells (one multi-cell State): bitwise with the portable-math
