@@ -100,6 +100,100 @@ namespace
                       lane_view(const_cast<double*>(y), g, m.n_species, l), lane_view(jac, g, m.nnz_jac, l));
   }
 
+  // ---- tiled single-phase kernels: one workgroup per 16 cells of a group, the tile's
+  // y and k staged in LDS with 128-byte coalesced loads, outputs gathered per
+  // species / Jacobian element in the reference's accumulation order and written
+  // once.  Each HBM byte of y, k and the output moves once.
+  constexpr int kTileCells = 16;
+  constexpr int kTileThreads = 256;
+
+  __host__ __device__ inline size_t tile_lds_bytes(int n_species, int n_reactions)
+  {
+    return static_cast<size_t>(n_species + n_reactions) * kTileCells * sizeof(double);
+  }
+
+  __device__ inline void load_tile(double* dst, const double* src, size_t g, int n_values, int q0)
+  {
+    for (int i = threadIdx.x; i < n_values * kTileCells; i += kTileThreads)
+    {
+      const int v = i / kTileCells, cc = i % kTileCells;
+      dst[i] = src[(g * n_values + v) * kGroup + q0 + cc];
+    }
+  }
+
+  // f += f(y): rates of the tile's cells into LDS (in place over k), then per-species gathers
+  __global__ void __launch_bounds__(kTileThreads) k_add_forcing_tiled(DevMech m, DevCoop c, int64_t n_cells,
+                                                                      const double* __restrict__ k,
+                                                                      const double* __restrict__ y, double* __restrict__ f)
+  {
+    extern __shared__ double tile[];
+    const int N = m.n_species, R = m.n_reactions;
+    const size_t g = blockIdx.x / (kGroup / kTileCells);
+    const int q0 = (blockIdx.x % (kGroup / kTileCells)) * kTileCells;
+    double* yt = tile;
+    double* rt = tile + static_cast<size_t>(N) * kTileCells;
+    load_tile(yt, y, g, N, q0);
+    load_tile(rt, k, g, R, q0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < R * kTileCells; i += kTileThreads)
+    {
+      const int r = i / kTileCells, cc = i % kTileCells;
+      double rate = rt[i];
+      for (int a = m.reactant_offsets[r]; a < m.reactant_offsets[r + 1]; ++a)
+        rate *= yt[m.reactant_ids[a] * kTileCells + cc];
+      rt[i] = rate;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < N * kTileCells; i += kTileThreads)
+    {
+      const int sp = i / kTileCells, cc = i % kTileCells;
+      const int64_t cell = static_cast<int64_t>(g) * kGroup + q0 + cc;
+      if (cell >= n_cells)
+        continue;
+      double* out = f + (g * N + sp) * kGroup + q0 + cc;
+      double v = *out;
+      for (int q = c.spec_offsets[sp]; q < c.spec_offsets[sp + 1]; ++q)
+        v += c.spec_coef[q] * rt[c.spec_rxn[q] * kTileCells + cc];
+      *out = v;
+    }
+  }
+
+  // jac -= J(y): per Jacobian element, the products of its terms in the reference's order
+  __global__ void __launch_bounds__(kTileThreads) k_subtract_jacobian_tiled(DevMech m, DevCoop c, int64_t n_cells,
+                                                                            const double* __restrict__ k,
+                                                                            const double* __restrict__ y,
+                                                                            double* __restrict__ jac)
+  {
+    extern __shared__ double tile[];
+    const int N = m.n_species, R = m.n_reactions;
+    const size_t g = blockIdx.x / (kGroup / kTileCells);
+    const int q0 = (blockIdx.x % (kGroup / kTileCells)) * kTileCells;
+    double* yt = tile;
+    double* kt = tile + static_cast<size_t>(N) * kTileCells;
+    load_tile(yt, y, g, N, q0);
+    load_tile(kt, k, g, R, q0);
+    __syncthreads();
+    const int nnz = m.nnz_jac;
+    for (int i = threadIdx.x; i < nnz * kTileCells; i += kTileThreads)
+    {
+      const int e = i / kTileCells, cc = i % kTileCells;
+      const int64_t cell = static_cast<int64_t>(g) * kGroup + q0 + cc;
+      if (cell >= n_cells)
+        continue;
+      double* out = jac + (g * nnz + e) * kGroup + q0 + cc;
+      double v = *out;
+      for (int q = c.jel_offsets[e]; q < c.jel_offsets[e + 1]; ++q)
+      {
+        const int t = c.jel_term[q];
+        double d = kt[m.term_process[t] * kTileCells + cc];
+        for (int a = m.term_dep_offsets[t]; a < m.term_dep_offsets[t + 1]; ++a)
+          d *= yt[m.term_dep_ids[a] * kTileCells + cc];
+        v += c.jel_coef[q] * d;
+      }
+      *out = v;
+    }
+  }
+
   __global__ void __launch_bounds__(kBlock) k_alpha_minus_jacobian(DevMech m, int64_t n_cells, double* __restrict__ jac,
                                                                    double alpha)
   {
@@ -1387,7 +1481,18 @@ extern "C"
     KERNEL_PRELUDE();
     if (!k || !y || !f)
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null device pointer");
-    hipLaunchKernelGGL(k_add_forcing, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, k, y, f);
+    const size_t lds = tile_lds_bytes(m->dev.n_species, m->dev.n_reactions);
+    if (lds <= lds_limit())
+    {
+      HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_add_forcing_tiled),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+      hipLaunchKernelGGL(k_add_forcing_tiled, dim3(static_cast<unsigned>(n_groups_of(n_cells) * (kGroup / kTileCells))),
+                         dim3(kTileThreads), lds, st, m->dev, m->coop, n_cells, k, y, f);
+    }
+    else
+    {
+      hipLaunchKernelGGL(k_add_forcing, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, k, y, f);
+    }
     HIP_TRY(hipGetLastError());
     return MICM_HIP_OK;
   }
@@ -1398,7 +1503,19 @@ extern "C"
     KERNEL_PRELUDE();
     if (!k || !y || !jac)
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null device pointer");
-    hipLaunchKernelGGL(k_subtract_jacobian, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, k, y, jac);
+    const size_t lds = tile_lds_bytes(m->dev.n_species, m->dev.n_reactions);
+    if (lds <= lds_limit())
+    {
+      HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_subtract_jacobian_tiled),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+      hipLaunchKernelGGL(k_subtract_jacobian_tiled,
+                         dim3(static_cast<unsigned>(n_groups_of(n_cells) * (kGroup / kTileCells))), dim3(kTileThreads),
+                         lds, st, m->dev, m->coop, n_cells, k, y, jac);
+    }
+    else
+    {
+      hipLaunchKernelGGL(k_subtract_jacobian, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, k, y, jac);
+    }
     HIP_TRY(hipGetLastError());
     return MICM_HIP_OK;
   }
