@@ -72,6 +72,8 @@ namespace micm_hip
     int32_t n_terms;
     const int32_t* jac_to_alu;
     const int32_t* alu_diag;
+    const int32_t* alu_lower;  // L storage index -> ALU position (-1: the unit diagonal)
+    const int32_t* alu_upper;  // U storage index -> ALU position
     DevLevels lu, fwd, bwd;
     const int32_t* spec_offsets;
     const int32_t* spec_rxn;

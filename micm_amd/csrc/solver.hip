@@ -194,6 +194,144 @@ namespace
     }
   }
 
+  // Tiled LU and substitution: TC cells of a group per workgroup, their in-place
+  // factors in LDS (position p of cell cc at p * TC + cc), the level schedules of the
+  // cooperative form run over (task, cell) pairs.  L and U leave in the reference's
+  // separate storage.
+  template<int TC>
+  __global__ void __launch_bounds__(kTileThreads) k_lu_decompose_tiled(DevMech m, DevCoop c, int64_t n_cells,
+                                                                       const double* __restrict__ jac,
+                                                                       double* __restrict__ lower, double* __restrict__ upper)
+  {
+    extern __shared__ double alu[];
+    const size_t g = blockIdx.x / (kGroup / TC);
+    const int q0 = (blockIdx.x % (kGroup / TC)) * TC;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < c.nnz_alu * TC; i += kTileThreads)
+      alu[i] = 0.0;
+    __syncthreads();
+    for (int i = tid; i < m.nnz_jac * TC; i += kTileThreads)
+    {
+      const int e = i / TC, cc = i % TC;
+      alu[c.jac_to_alu[e] * TC + cc] = jac[(g * m.nnz_jac + e) * kGroup + q0 + cc];
+    }
+    __syncthreads();
+    const DevLevels& lv = c.lu;
+    for (int l = 0; l < lv.n_levels; ++l)
+    {
+      const int tb = lv.offsets[l], n_items = (lv.offsets[l + 1] - tb) * TC;
+      for (int w = tid; w < n_items; w += kTileThreads)
+      {
+        const int t = tb + w / TC, cc = w % TC;
+        const uint2 task = lv.tasks[t];
+        const uint32_t qe = lv.tasks[t + 1].y;
+        const uint32_t o = task.x & 0xffffu, dg = task.x >> 16;
+        double v = alu[o * TC + cc];
+        for (uint32_t q = task.y; q < qe; ++q)
+        {
+          const uint32_t pr = lv.terms[q];
+          v -= alu[(pr & 0xffffu) * TC + cc] * alu[(pr >> 16) * TC + cc];
+        }
+        if (dg != 0xffffu)
+          v = v / alu[dg * TC + cc];
+        alu[o * TC + cc] = v;
+      }
+      __syncthreads();
+    }
+    for (int i = tid; i < m.nnz_lower * TC; i += kTileThreads)
+    {
+      const int p = i / TC, cc = i % TC;
+      if (static_cast<int64_t>(g) * kGroup + q0 + cc < n_cells)
+      {
+        const int a = c.alu_lower[p];
+        lower[(g * m.nnz_lower + p) * kGroup + q0 + cc] = a >= 0 ? alu[a * TC + cc] : 1.0;
+      }
+    }
+    for (int i = tid; i < m.nnz_upper * TC; i += kTileThreads)
+    {
+      const int p = i / TC, cc = i % TC;
+      if (static_cast<int64_t>(g) * kGroup + q0 + cc < n_cells)
+        upper[(g * m.nnz_upper + p) * kGroup + q0 + cc] = alu[c.alu_upper[p] * TC + cc];
+    }
+  }
+
+  // x <- U^-1 L^-1 x (linear_solver.inl:235-294), dividing by L[i][i] as the reference does
+  template<int TC>
+  __global__ void __launch_bounds__(kTileThreads) k_lu_solve_tiled(DevMech m, DevCoop c, int64_t n_cells,
+                                                                   const double* __restrict__ lower,
+                                                                   const double* __restrict__ upper, double* __restrict__ x)
+  {
+    extern __shared__ double lds_solve[];
+    const int N = m.n_species;
+    double* alu = lds_solve;
+    double* xt = alu + static_cast<size_t>(c.nnz_alu) * TC;
+    double* ld = xt + static_cast<size_t>(N) * TC;
+    const size_t g = blockIdx.x / (kGroup / TC);
+    const int q0 = (blockIdx.x % (kGroup / TC)) * TC;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < m.nnz_lower * TC; i += kTileThreads)
+    {
+      const int p = i / TC, cc = i % TC;
+      const double v = lower[(g * m.nnz_lower + p) * kGroup + q0 + cc];
+      const int a = c.alu_lower[p];
+      if (a >= 0)
+        alu[a * TC + cc] = v;
+    }
+    for (int i = tid; i < N * TC; i += kTileThreads)
+    {
+      const int r = i / TC, cc = i % TC;
+      ld[i] = lower[(g * m.nnz_lower + m.lower_diag[r]) * kGroup + q0 + cc];
+      xt[i] = x[(g * N + r) * kGroup + q0 + cc];
+    }
+    for (int i = tid; i < m.nnz_upper * TC; i += kTileThreads)
+    {
+      const int p = i / TC, cc = i % TC;
+      alu[c.alu_upper[p] * TC + cc] = upper[(g * m.nnz_upper + p) * kGroup + q0 + cc];
+    }
+    __syncthreads();
+    // rows without off-diagonal L entries only divide; they are inputs of every level
+    for (int i = tid; i < N * TC; i += kTileThreads)
+    {
+      const int r = i / TC;
+      if (m.fwd_offsets[r] == m.fwd_offsets[r + 1])
+        xt[i] = xt[i] / ld[i];
+    }
+    __syncthreads();
+    for (int pass = 0; pass < 2; ++pass)
+    {
+      const DevLevels& lv = pass == 0 ? c.fwd : c.bwd;
+      for (int l = 0; l < lv.n_levels; ++l)
+      {
+        const int tb = lv.offsets[l], n_items = (lv.offsets[l + 1] - tb) * TC;
+        for (int w = tid; w < n_items; w += kTileThreads)
+        {
+          const int t = tb + w / TC, cc = w % TC;
+          const uint2 task = lv.tasks[t];
+          const uint32_t qe = lv.tasks[t + 1].y;
+          const uint32_t o = task.x & 0xffffu, dg = task.x >> 16;
+          double v = xt[o * TC + cc];
+          for (uint32_t q = task.y; q < qe; ++q)
+          {
+            const uint32_t pr = lv.terms[q];
+            v -= alu[(pr & 0xffffu) * TC + cc] * xt[(pr >> 16) * TC + cc];
+          }
+          if (pass == 0)
+            v = v / ld[o * TC + cc];
+          else
+            v = v / alu[dg * TC + cc];
+          xt[o * TC + cc] = v;
+        }
+        __syncthreads();
+      }
+    }
+    for (int i = tid; i < N * TC; i += kTileThreads)
+    {
+      const int r = i / TC, cc = i % TC;
+      if (static_cast<int64_t>(g) * kGroup + q0 + cc < n_cells)
+        x[(g * N + r) * kGroup + q0 + cc] = xt[i];
+    }
+  }
+
   __global__ void __launch_bounds__(kBlock) k_alpha_minus_jacobian(DevMech m, int64_t n_cells, double* __restrict__ jac,
                                                                    double alpha)
   {
@@ -1143,7 +1281,8 @@ namespace
              padded_bytes(t.lu_term_offsets) + padded_bytes(t.lu_term_pairs) +
              padded_bytes(t.lower_diag) + padded_bytes(t.upper_diag) + padded_bytes(t.fwd_offsets) +
              padded_bytes(t.fwd_pairs) + padded_bytes(t.bwd_offsets) + padded_bytes(t.bwd_pairs) +
-             padded_bytes(t.jac_to_alu) + padded_bytes(t.alu_diag) + level_bytes(t.lv_lu) + level_bytes(t.lv_fwd) +
+             padded_bytes(t.jac_to_alu) + padded_bytes(t.alu_diag) + padded_bytes(t.alu_lower) +
+             padded_bytes(t.alu_upper) + level_bytes(t.lv_lu) + level_bytes(t.lv_fwd) +
              level_bytes(t.lv_bwd) +
              padded_bytes(t.spec_offsets) + padded_bytes(t.spec_rxn) +
              padded_bytes(t.spec_coef) + padded_bytes(t.jel_offsets) + padded_bytes(t.jel_term) +
@@ -1192,6 +1331,8 @@ namespace
     c.n_terms = static_cast<int32_t>(t.term_process.size());
     upload(m->blob, off, t.jac_to_alu, c.jac_to_alu, staging);
     upload(m->blob, off, t.alu_diag, c.alu_diag, staging);
+    upload(m->blob, off, t.alu_lower, c.alu_lower, staging);
+    upload(m->blob, off, t.alu_upper, c.alu_upper, staging);
     upload_levels(m->blob, off, t.lv_lu, c.lu, staging);
     upload_levels(m->blob, off, t.lv_fwd, c.fwd, staging);
     upload_levels(m->blob, off, t.lv_bwd, c.bwd, staging);
@@ -1225,6 +1366,28 @@ namespace
     (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev);
     return static_cast<size_t>(std::max(max_lds, optin)) - 64;  // static __shared__ of the kernel
   }
+
+  // Cells per workgroup of the tiled LU / substitution kernels: the largest power of
+  // two up to 16 whose tile of `doubles_per_cell` fits one workgroup's LDS; 0 when the
+  // level schedules are unavailable or one cell does not fit.
+  int tile_cells(const micm_hip_mechanism* m, size_t doubles_per_cell)
+  {
+    if (!m->t.coop_levels_ok)
+      return 0;
+    const size_t limit = lds_limit();
+    for (int tc = 16; tc >= 1; tc /= 2)
+      if (doubles_per_cell * tc * sizeof(double) <= limit)
+        return tc;
+    return 0;
+  }
+
+#define LAUNCH_TILED(kernel, blocks, lds, stream, ...)                                                               \
+  do                                                                                                                \
+  {                                                                                                                 \
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                static_cast<int>(lds)));                                                            \
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kTileThreads), lds, stream, __VA_ARGS__);                        \
+  } while (0)
 
   // stage_request < 0: stage the substitution tables when they fit and the LU tables
   // when that costs no resident cell per CU; threads_request 0: one wavefront.
@@ -1536,7 +1699,24 @@ extern "C"
     KERNEL_PRELUDE();
     if (!jac || !lower || !upper)
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null device pointer");
-    hipLaunchKernelGGL(k_lu_decompose, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, jac, lower, upper);
+    const int tc = tile_cells(m, static_cast<size_t>(m->coop.nnz_alu));
+    if (tc > 0)
+    {
+      const size_t lds = static_cast<size_t>(m->coop.nnz_alu) * tc * sizeof(double);
+      const unsigned blocks = static_cast<unsigned>(n_groups_of(n_cells) * (kGroup / tc));
+      switch (tc)
+      {
+        case 16: LAUNCH_TILED(k_lu_decompose_tiled<16>, blocks, lds, st, m->dev, m->coop, n_cells, jac, lower, upper); break;
+        case 8: LAUNCH_TILED(k_lu_decompose_tiled<8>, blocks, lds, st, m->dev, m->coop, n_cells, jac, lower, upper); break;
+        case 4: LAUNCH_TILED(k_lu_decompose_tiled<4>, blocks, lds, st, m->dev, m->coop, n_cells, jac, lower, upper); break;
+        case 2: LAUNCH_TILED(k_lu_decompose_tiled<2>, blocks, lds, st, m->dev, m->coop, n_cells, jac, lower, upper); break;
+        default: LAUNCH_TILED(k_lu_decompose_tiled<1>, blocks, lds, st, m->dev, m->coop, n_cells, jac, lower, upper); break;
+      }
+    }
+    else
+    {
+      hipLaunchKernelGGL(k_lu_decompose, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, jac, lower, upper);
+    }
     HIP_TRY(hipGetLastError());
     return MICM_HIP_OK;
   }
@@ -1547,7 +1727,25 @@ extern "C"
     KERNEL_PRELUDE();
     if (!lower || !upper || !x)
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null device pointer");
-    hipLaunchKernelGGL(k_lu_solve, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, lower, upper, x);
+    const int tc = tile_cells(m, static_cast<size_t>(m->coop.nnz_alu) + 2 * static_cast<size_t>(m->dev.n_species));
+    if (tc > 0)
+    {
+      const size_t lds = (static_cast<size_t>(m->coop.nnz_alu) + 2 * static_cast<size_t>(m->dev.n_species)) * tc *
+                         sizeof(double);
+      const unsigned blocks = static_cast<unsigned>(n_groups_of(n_cells) * (kGroup / tc));
+      switch (tc)
+      {
+        case 16: LAUNCH_TILED(k_lu_solve_tiled<16>, blocks, lds, st, m->dev, m->coop, n_cells, lower, upper, x); break;
+        case 8: LAUNCH_TILED(k_lu_solve_tiled<8>, blocks, lds, st, m->dev, m->coop, n_cells, lower, upper, x); break;
+        case 4: LAUNCH_TILED(k_lu_solve_tiled<4>, blocks, lds, st, m->dev, m->coop, n_cells, lower, upper, x); break;
+        case 2: LAUNCH_TILED(k_lu_solve_tiled<2>, blocks, lds, st, m->dev, m->coop, n_cells, lower, upper, x); break;
+        default: LAUNCH_TILED(k_lu_solve_tiled<1>, blocks, lds, st, m->dev, m->coop, n_cells, lower, upper, x); break;
+      }
+    }
+    else
+    {
+      hipLaunchKernelGGL(k_lu_solve, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, lower, upper, x);
+    }
     HIP_TRY(hipGetLastError());
     return MICM_HIP_OK;
   }
