@@ -298,6 +298,23 @@ int micm_hip_update_state_parameters(const micm_hip_mechanism* mechanism, micm_h
 int micm_hip_solve(const micm_hip_mechanism* mechanism, micm_hip_state* state, double time_step,
                    const micm_hip_rosenbrock_parameters* parameters, int32_t form);
 
+/* ---------------------------------------------------------------- packed solves
+ * Several states -- of one mechanism or of different ones -- solved together.  The
+ * per-cell Solve of every state (MICM_HIP_SOLVE_PERSISTENT or _COOPERATIVE) is
+ * captured once into a hipGraph with one concurrent branch per state, and later
+ * solves with the same time step, parameters and launch shapes replay it.  The
+ * states must outlive the packed object and must not be used by another call while
+ * a packed solve runs.  No counterpart in the reference, whose States are solved one
+ * at a time; for callers that integrate several mechanisms (or several cell sets)
+ * per time step. */
+typedef struct micm_hip_packed micm_hip_packed;
+int micm_hip_packed_create(int32_t n_states, micm_hip_state* const* states, int32_t form, micm_hip_packed** out);
+/* Blocks until every state has finished; per-cell results via micm_hip_state_get_results. */
+int micm_hip_packed_solve(micm_hip_packed* packed, double time_step, const micm_hip_rosenbrock_parameters* params);
+/* 1 if the last micm_hip_packed_solve replayed a graph captured earlier, else 0. */
+int micm_hip_packed_replayed(const micm_hip_packed* packed);
+void micm_hip_packed_destroy(micm_hip_packed* packed);
+
 #ifdef __cplusplus
 }
 #endif

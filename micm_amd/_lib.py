@@ -88,11 +88,15 @@ _SIGNATURES = {
     "micm_hip_state_set_stream": [_vp, _vp],
     "micm_hip_state_set_handoff": [_vp, _i64],
     "micm_hip_state_set_cooperative_shape": [_vp, _i32, _i32],
+    "micm_hip_packed_create": [_i32, ctypes.POINTER(_vp), _i32, ctypes.POINTER(_vp)],
+    "micm_hip_packed_solve": [_vp, ctypes.c_double, ctypes.POINTER(RosenbrockParameters)],
+    "micm_hip_packed_replayed": [_vp],
+    "micm_hip_packed_destroy": [_vp],
     "micm_hip_mechanism_cooperative_supported": [_vp, ctypes.c_int32, _i32p],
     "micm_hip_update_state_parameters": [_vp, _vp],
     "micm_hip_solve": [_vp, _vp, ctypes.c_double, ctypes.POINTER(RosenbrockParameters), ctypes.c_int32],
 }
-_VOID = {"micm_hip_mechanism_destroy", "micm_hip_state_destroy"}
+_VOID = {"micm_hip_mechanism_destroy", "micm_hip_state_destroy", "micm_hip_packed_destroy"}
 
 _lib = None
 
@@ -304,3 +308,29 @@ class DeviceState:
 
     def solve(self, time_step, parameters, form=SOLVE_PLAIN):
         check(lib().micm_hip_solve(self.mechanism.h, self.h, float(time_step), ctypes.byref(parameters), int(form)))
+
+
+class PackedSolve:
+    """Several DeviceStates (any mechanisms) solved together by one graph-captured
+    launch with a concurrent branch per state (micm_hip_packed_*).  Keeps the states
+    alive while it exists."""
+
+    def __init__(self, states, form=SOLVE_COOPERATIVE):
+        self.states = list(states)
+        arr = (_vp * len(self.states))(*[s.h for s in self.states])
+        h = _vp()
+        check(lib().micm_hip_packed_create(len(self.states), arr, int(form), ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().micm_hip_packed_destroy(self.h)
+            self.h = None
+
+    def solve(self, time_step, parameters):
+        check(lib().micm_hip_packed_solve(self.h, float(time_step), ctypes.byref(parameters)))
+
+    @property
+    def replayed(self):
+        return bool(lib().micm_hip_packed_replayed(self.h))
+

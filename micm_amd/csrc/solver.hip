@@ -1067,6 +1067,23 @@ struct micm_hip_state
   int coop_threads = 0;
 };
 
+struct micm_hip_packed
+{
+  std::vector<micm_hip_state*> states;
+  int32_t form = MICM_HIP_SOLVE_COOPERATIVE;
+  hipStream_t origin = nullptr;
+  std::vector<hipStream_t> branch;
+  std::vector<hipEvent_t> done;
+  hipEvent_t fork = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  // what the captured graph was built for
+  bool captured = false, replayed = false;
+  double dt = 0.0;
+  micm_hip_rosenbrock_parameters params{};
+  std::vector<int64_t> shapes;  // per state: handoff, cooperative threads, staging
+};
+
 namespace
 {
   template<class T>
@@ -1471,6 +1488,63 @@ namespace
       case 256: return launch_coop_nt<256>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
       default: return launch_coop_nt<kWave>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
     }
+  }
+
+  // Device memory a per-cell form needs beyond the state's arrays, allocated before any
+  // launch (and outside any stream capture).
+  int persistent_blocks(const micm_hip_state* s, int64_t& blocks)
+  {
+    int dev = 0, cus = 0, blocks_per_cu = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, k_persistent, kBlock, 0));
+    if (blocks_per_cu < 1)
+      blocks_per_cu = 1;
+    blocks = static_cast<int64_t>(cus) * blocks_per_cu;
+    const int64_t needed = (s->n_groups + (kBlock / kGroup) - 1) / (kBlock / kGroup);
+    blocks = std::max<int64_t>(1, std::min(blocks, needed));
+    return MICM_HIP_OK;
+  }
+
+  int prepare_solve(micm_hip_state* s, int32_t form, int stages)
+  {
+    if (form != MICM_HIP_SOLVE_PERSISTENT)
+      return MICM_HIP_OK;
+    int64_t blocks = 0;
+    if (int rc = persistent_blocks(s, blocks))
+      return rc;
+    return ensure_slots(s, blocks * (kBlock / kGroup), stages);
+  }
+
+  // The launches of one Solve of a per-cell form (persistent or cooperative), enqueued
+  // on `st` without waiting; prepare_solve must have run.
+  int enqueue_solve(micm_hip_state* s, const DevParams& p, int32_t form, bool update_rates, hipStream_t st)
+  {
+    const micm_hip_mechanism* m = s->mech;
+    const DevMech& d = m->dev;
+    if (form == MICM_HIP_SOLVE_COOPERATIVE)
+      return launch_coop(s, p, nullptr, nullptr, s->n_cells, nullptr, update_rates, st);
+    int64_t blocks = 0;
+    if (int rc = persistent_blocks(s, blocks))
+      return rc;
+    if (s->n_slots < blocks * (kBlock / kGroup))
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "persistent scratch not prepared");
+    Work w = slot_work(s);
+    DevParams pp = p;
+    const bool coop_ok = coop_lds_bytes(m, p.stages) > 0;
+    pp.handoff_steps = !coop_ok ? 0 : (s->handoff_steps >= 0 ? s->handoff_steps : 48);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, s->queue_next);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, s->handoff_count);
+    Queue q{ s->queue_next, s->has_order ? s->order : nullptr };
+    hipLaunchKernelGGL(k_persistent, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, st, d, pp, s->n_cells, q, w,
+                       s->y, s->k, s->atol, s->status, s->stats, s->final_time, s->ctl, s->handoff_list,
+                       s->handoff_count, update_rates ? s->cond : nullptr, update_rates ? s->custom : nullptr);
+    HIP_TRY(hipGetLastError());
+    // handed-off cells carry the rate constants the persistent kernel stored
+    if (pp.handoff_steps > 0)
+      if (int rc = launch_coop(s, p, s->handoff_list, s->handoff_count, 0, s->ctl, false, st))
+        return rc;
+    return MICM_HIP_OK;
   }
 }  // namespace
 
@@ -2023,45 +2097,134 @@ extern "C"
       HIP_TRY(hipStreamSynchronize(st));
       return MICM_HIP_OK;
     }
-    if (form == MICM_HIP_SOLVE_PERSISTENT)
+    if (form == MICM_HIP_SOLVE_PERSISTENT || form == MICM_HIP_SOLVE_COOPERATIVE)
     {
-      int dev = 0, cus = 0;
-      HIP_TRY(hipGetDevice(&dev));
-      HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-      int blocks_per_cu = 0;
-      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, k_persistent, kBlock, 0));
-      if (blocks_per_cu < 1)
-        blocks_per_cu = 1;
-      int64_t blocks = static_cast<int64_t>(cus) * blocks_per_cu;
-      const int64_t needed = (s->n_groups + (kBlock / kGroup) - 1) / (kBlock / kGroup);
-      blocks = std::max<int64_t>(1, std::min(blocks, needed));
-      if (int rc = ensure_slots(s, blocks * (kBlock / kGroup), p.stages))
+      if (int rc = prepare_solve(s, form, p.stages))
         return rc;
-      Work w = slot_work(s);
-      DevParams pp = p;
-      const bool coop_ok = coop_lds_bytes(m, p.stages) > 0;
-      pp.handoff_steps = !coop_ok ? 0 : (s->handoff_steps >= 0 ? s->handoff_steps : 48);
-      hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, s->queue_next);
-      hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, s->handoff_count);
-      Queue q{ s->queue_next, s->has_order ? s->order : nullptr };
-      hipLaunchKernelGGL(k_persistent, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, st, d, pp, s->n_cells, q, w,
-                         s->y, s->k, s->atol, s->status, s->stats, s->final_time, s->ctl, s->handoff_list,
-                         s->handoff_count, update_rates ? s->cond : nullptr, update_rates ? s->custom : nullptr);
-      HIP_TRY(hipGetLastError());
-      // handed-off cells carry the rate constants the persistent kernel stored
-      if (pp.handoff_steps > 0)
-        if (int rc = launch_coop(s, p, s->handoff_list, s->handoff_count, 0, s->ctl, false, st))
-          return rc;
-      HIP_TRY(hipStreamSynchronize(st));
-      return MICM_HIP_OK;
-    }
-    if (form == MICM_HIP_SOLVE_COOPERATIVE)
-    {
-      if (int rc = launch_coop(s, p, nullptr, nullptr, s->n_cells, nullptr, update_rates, st))
+      if (int rc = enqueue_solve(s, p, form, update_rates, st))
         return rc;
       HIP_TRY(hipStreamSynchronize(st));
       return MICM_HIP_OK;
     }
     return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "unknown solve form");
   }
+
+  int micm_hip_packed_create(int32_t n_states, micm_hip_state* const* states, int32_t form, micm_hip_packed** out)
+  {
+    if (!out || n_states < 1 || !states)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "need at least one state");
+    if (form != MICM_HIP_SOLVE_PERSISTENT && form != MICM_HIP_SOLVE_COOPERATIVE)
+      return fail(MICM_HIP_ERR_UNSUPPORTED, "packed solves use the persistent or the cooperative form");
+    for (int32_t i = 0; i < n_states; ++i)
+      if (!states[i])
+        return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null state");
+    auto p = std::make_unique<micm_hip_packed>();
+    p->states.assign(states, states + n_states);
+    p->form = form;
+    HIP_TRY(hipStreamCreate(&p->origin));
+    HIP_TRY(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
+    for (int32_t i = 0; i < n_states; ++i)
+    {
+      hipStream_t b = nullptr;
+      hipEvent_t e = nullptr;
+      HIP_TRY(hipStreamCreate(&b));
+      p->branch.push_back(b);
+      HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      p->done.push_back(e);
+    }
+    *out = p.release();
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_packed_solve(micm_hip_packed* pk, double time_step, const micm_hip_rosenbrock_parameters* params)
+  {
+    if (!pk || !params)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "bad packed object or parameters");
+    if (params->stages < 1 || params->stages > 6)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "stages must be 1..6");
+    std::vector<int64_t> shapes;
+    for (micm_hip_state* s : pk->states)
+    {
+      if (int rc = prepare_solve(s, pk->form, static_cast<int>(params->stages)))
+        return rc;
+      shapes.push_back(s->handoff_steps);
+      shapes.push_back(s->coop_threads);
+      shapes.push_back(s->coop_stage);
+      shapes.push_back(static_cast<int64_t>(s->n_slots));
+    }
+    const bool same = pk->captured && pk->dt == time_step && pk->shapes == shapes &&
+                      std::memcmp(&pk->params, params, sizeof *params) == 0;
+    if (!same)
+    {
+      if (pk->exec)
+        (void)hipGraphExecDestroy(pk->exec);
+      if (pk->graph)
+        (void)hipGraphDestroy(pk->graph);
+      pk->exec = nullptr;
+      pk->graph = nullptr;
+      pk->captured = false;
+      HIP_TRY(hipStreamBeginCapture(pk->origin, hipStreamCaptureModeThreadLocal));
+      int rc = MICM_HIP_OK;
+      if (hipEventRecord(pk->fork, pk->origin) != hipSuccess)
+        rc = fail(MICM_HIP_ERR_DEVICE, "event record failed during capture");
+      for (size_t i = 0; rc == MICM_HIP_OK && i < pk->states.size(); ++i)
+      {
+        micm_hip_state* s = pk->states[i];
+        const DevParams p = make_params(*params, time_step, s->rtol);
+        if (hipStreamWaitEvent(pk->branch[i], pk->fork, 0) != hipSuccess)
+          rc = fail(MICM_HIP_ERR_DEVICE, "stream wait failed during capture");
+        else if ((rc = enqueue_solve(s, p, pk->form, false, pk->branch[i])) == MICM_HIP_OK)
+        {
+          if (hipEventRecord(pk->done[i], pk->branch[i]) != hipSuccess ||
+              hipStreamWaitEvent(pk->origin, pk->done[i], 0) != hipSuccess)
+            rc = fail(MICM_HIP_ERR_DEVICE, "join failed during capture");
+        }
+      }
+      hipGraph_t graph = nullptr;
+      const hipError_t end = hipStreamEndCapture(pk->origin, &graph);
+      if (rc != MICM_HIP_OK)
+      {
+        if (graph)
+          (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      if (end != hipSuccess || !graph)
+        return fail(MICM_HIP_ERR_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(end));
+      pk->graph = graph;
+      HIP_TRY(hipGraphInstantiate(&pk->exec, pk->graph, nullptr, nullptr, 0));
+      pk->captured = true;
+      pk->dt = time_step;
+      pk->params = *params;
+      pk->shapes = shapes;
+    }
+    pk->replayed = same;
+    HIP_TRY(hipGraphLaunch(pk->exec, pk->origin));
+    HIP_TRY(hipStreamSynchronize(pk->origin));
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_packed_replayed(const micm_hip_packed* pk)
+  {
+    return pk && pk->replayed ? 1 : 0;
+  }
+
+  void micm_hip_packed_destroy(micm_hip_packed* pk)
+  {
+    if (!pk)
+      return;
+    if (pk->exec)
+      (void)hipGraphExecDestroy(pk->exec);
+    if (pk->graph)
+      (void)hipGraphDestroy(pk->graph);
+    for (hipEvent_t e : pk->done)
+      (void)hipEventDestroy(e);
+    for (hipStream_t b : pk->branch)
+      (void)hipStreamDestroy(b);
+    if (pk->fork)
+      (void)hipEventDestroy(pk->fork);
+    if (pk->origin)
+      (void)hipStreamDestroy(pk->origin);
+    delete pk;
+  }
+
 }
