@@ -165,6 +165,11 @@ def workload(name, n_cells):
     raise SystemExit(f"unknown workload {name}")
 
 
+def log(msg):
+    """Progress on stderr; stdout carries only the result line."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,6 +207,7 @@ def main():
     n_cells = args.cells or (100000 if args.workload == "210_560" else 10000)
     torch.cuda.set_device(0)
     desc, cells, tableau, dt, label = workload(args.workload, n_cells)
+    log(f"workload {args.workload}: {n_cells} cells generated")
     mech = _lib.Mechanism(desc)
     order = mech.species_order()
     params = _lib.rosenbrock_parameters(tableau)
@@ -223,8 +229,9 @@ def main():
     torch.cuda.synchronize()
     update_ms = (time.perf_counter() - t0) * 1e3
 
+    log(f"inputs uploaded ({upload_s:.2f} s), rate constants {update_ms:.1f} ms")
     if args.form == "autotune":
-        choice = autotune.autotune(mech, state, dt, params)
+        choice = autotune.autotune(mech, state, dt, params, verbose=True)
         form = autotune.apply(state, choice)
         form_label = f"autotuned: {choice['label']}"
     else:
@@ -242,9 +249,11 @@ def main():
         _copy_device(ptr.variables, y_initial.data_ptr(), n_y * 8, stream)
         state.solve(dt, params, form)
 
-    for _ in range(args.warmup):
+    log(f"form: {form_label}")
+    for i in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        log(f"warm-up step {i + 1} done")
     start_ev, end_ev = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     start_ev.record(stream)
@@ -253,6 +262,7 @@ def main():
     end_ev.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    log(f"{args.steps} timed steps: {wall:.2f} s")
     kernel_ms = start_ev.elapsed_time(end_ev) / args.steps
     status, stats, _ = state.results()
     ms_per_step = wall * 1e3 / args.steps
@@ -313,11 +323,13 @@ def main():
             t0 = time.perf_counter()
             probe.solve(dt, params, f)
             cmp[name] = m / (time.perf_counter() - t0)
+        log("plain form compared")
         result["plain_form"] = {"value": round(cmp["plain"], 2), "unit": "cell-solves/s", "cells": m,
                                 "chosen_form_same_cells": round(cmp["chosen"], 2),
                                 "speedup": round(cmp["chosen"] / cmp["plain"], 3)}
     if not args.no_kernels:
         result["kernels"] = kernel_rooflines(mech, state, stream, n_cells)
+        log("single-phase kernels timed")
     if args.cpu_sample > 0:
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
         sample = min(args.cpu_sample, n_cells)

@@ -115,7 +115,22 @@ def lib():
             f.argtypes = args
             f.restype = None if name in _VOID else ctypes.c_int
         _lib = L
+        _init_device_runtime()
     return _lib
+
+
+def _init_device_runtime():
+    """The library shares the process's HIP runtime with PyTorch when PyTorch is
+    loaded; initialise it through PyTorch first so both see the same device state."""
+    import sys
+    torch = sys.modules.get("torch")
+    if torch is None:
+        return
+    try:
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except Exception:
+        pass
 
 
 class MicmHipError(RuntimeError):

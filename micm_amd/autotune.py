@@ -11,6 +11,7 @@ candidate produces bit-identical results, so the choice only affects speed.
 Results are cached per (mechanism layout, tableau, time step, sample) in this
 process.
 """
+import sys
 import time
 
 import numpy as np
@@ -45,40 +46,54 @@ def apply(state, choice):
     return choice["form"]
 
 
-def autotune(mech, state, dt, params, sample=2048, repeats=1, candidates=CANDIDATES, verbose=False):
-    """Pick the fastest per-cell form for `state` (whose inputs and rate constants
-    are set) on a strided sample of its cells.  Returns a dict with keys label,
-    form, handoff, threads, stage, seconds (per candidate: label -> seconds)."""
-    n = state.n_cells
-    m = min(sample, n)
-    key = _key(mech, params.stages, dt, m)
-    if key in _cache:
-        return dict(_cache[key])
-    idx = np.linspace(0, n - 1, m).round().astype(np.int64)
-    y = state.get_variables()[idx]
-    k = state.get_rate_constants()[idx]
-    probe = _lib.DeviceState(mech, m)
+def _time_candidates(mech, y, k, dt, params, candidates, repeats, verbose):
+    probe = _lib.DeviceState(mech, y.shape[0])
     probe.set_rate_constants(k)
-    supported = mech.cooperative_supported(params.stages)
     times = {}
-    best = None
     for label, form, handoff, threads, stage in candidates:
-        if form == _lib.SOLVE_COOPERATIVE and not supported:
-            continue
         probe.set_handoff(handoff)
         probe.set_cooperative_shape(threads, stage)
         el = []
-        for _ in range(repeats + 1):  # the first run compiles / warms up
+        for _ in range(repeats + 1):  # the first run warms up
             probe.set_variables(y)
             t0 = time.perf_counter()
             probe.solve(dt, params, form)
             el.append(time.perf_counter() - t0)
         times[label] = min(el[1:])
         if verbose:
-            print(f"autotune: {label:32s} {times[label] * 1e3:9.2f} ms", flush=True)
-        if best is None or times[label] < times[best[0]]:
-            best = (label, form, handoff, threads, stage)
-    label, form, handoff, threads, stage = best
+            print(f"autotune: {label:32s} {y.shape[0]:6d} cells {times[label] * 1e3:9.2f} ms", file=sys.stderr,
+                  flush=True)
+    return times
+
+
+def autotune(mech, state, dt, params, sample=2048, first_sample=512, finalists=3, repeats=1, candidates=CANDIDATES,
+             verbose=False):
+    """Pick the fastest per-cell form for `state` (whose inputs and rate constants
+    are set): every candidate is timed on a strided sample of `first_sample` cells,
+    the `finalists` fastest again on `sample` cells.  Returns a dict with keys label,
+    form, handoff, threads, stage, sample, seconds (label -> seconds on the larger
+    sample, or on the first one for candidates that did not go on)."""
+    n = state.n_cells
+    m = min(sample, n)
+    key = _key(mech, params.stages, dt, m)
+    if key in _cache:
+        return dict(_cache[key])
+    supported = mech.cooperative_supported(params.stages)
+    cands = [c for c in candidates if c[1] != _lib.SOLVE_COOPERATIVE or supported]
+    y_all = state.get_variables()
+    k_all = state.get_rate_constants()
+
+    def pick(count):
+        idx = np.linspace(0, n - 1, count).round().astype(np.int64)
+        return y_all[idx], k_all[idx]
+
+    m1 = min(first_sample, m)
+    times = _time_candidates(mech, *pick(m1), dt, params, cands, repeats, verbose)
+    if m > m1:
+        best = sorted(cands, key=lambda c: times[c[0]])[:finalists]
+        times.update(_time_candidates(mech, *pick(m), dt, params, best, repeats, verbose))
+        cands = best
+    label, form, handoff, threads, stage = min(cands, key=lambda c: times[c[0]])
     choice = {"label": label, "form": form, "handoff": handoff, "threads": threads, "stage": stage,
               "sample": m, "seconds": times}
     _cache[key] = dict(choice)

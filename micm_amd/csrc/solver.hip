@@ -1287,6 +1287,7 @@ namespace
     if (m->blob)
       return MICM_HIP_OK;
     const MechanismTables& t = m->t;
+    HIP_TRY(hipInit(0));
     HIP_TRY(hipGetDevice(&m->device));
     size_t bytes = 0;
     bytes += padded_bytes(t.rate_type) + padded_bytes(t.rate_params) + padded_bytes(t.custom_index) +

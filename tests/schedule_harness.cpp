@@ -103,3 +103,35 @@ extern "C"
     return nl;
   }
 }
+
+extern "C"
+{
+  // Longest dependency chain of schedule `which`, weighting each task by its number
+  // of terms plus `base`: the lower bound any level assignment can reach.
+  double harness_critical_path(const micm_hip_mechanism_desc* desc, int32_t which, double base)
+  {
+    const MechanismTables t = micm_hip::compile_mechanism(*desc);
+    const LevelSchedule& s = which == 0 ? t.lv_lu : (which == 1 ? t.lv_fwd : t.lv_bwd);
+    const size_t n_pos = which == 0 ? t.alu_col.size() : static_cast<size_t>(t.n_species);
+    std::vector<double> finish(n_pos, 0.0);
+    double best = 0.0;
+    for (int32_t l = 0; l < s.n_levels(); ++l)
+      for (int32_t q = s.offsets[l]; q < s.offsets[l + 1]; ++q)
+      {
+        const uint32_t o = s.tasks[2 * q] & 0xffffu, d = s.tasks[2 * q] >> 16;
+        double start = 0.0;
+        for (uint32_t k = s.tasks[2 * q + 1]; k < s.tasks[2 * q + 3]; ++k)
+        {
+          if (which == 0)
+            start = std::max(start, std::max(finish[s.terms[k] & 0xffffu], finish[s.terms[k] >> 16]));
+          else
+            start = std::max(start, finish[s.terms[k] >> 16]);
+        }
+        if (which == 0 && d != 0xffffu)
+          start = std::max(start, finish[d]);
+        finish[o] = start + base + (s.tasks[2 * q + 3] - s.tasks[2 * q + 1]);
+        best = std::max(best, finish[o]);
+      }
+    return best;
+  }
+}
