@@ -107,6 +107,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        # PyTorch bundles its own HIP runtime under the same soname as the system one this
+        # library links; whichever loads first serves the whole process, and PyTorch only
+        # works with its own.  Load it first so that both share it.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         L.micm_hip_last_error.restype = ctypes.c_char_p
         L.micm_hip_last_error.argtypes = []

@@ -126,29 +126,49 @@ namespace micm_hip
   }
 
   // v - sum_q A[lo(terms[q])] * X[hi(terms[q])] for q in [b, e), subtracting in order;
-  // the loads of eight terms are issued before their subtractions.
+  // with TC cells interleaved in A and X, position p of cell cc is p * TC + cc.
+  // Terms go in batches of eight: the eight table entries, then the sixteen operands
+  // are loaded before the eight subtractions.  A short batch reads past the task's last
+  // term (the tables end in eight padding entries) and skips those subtractions.
+  template<int TC = 1>
   __device__ __forceinline__ double coop_dot(double v, const uint32_t* terms, uint32_t b, uint32_t e, const double* A,
-                                             const double* X)
+                                             const double* X, int cc = 0)
   {
     uint32_t q = b;
     for (; q + 8 <= e; q += 8)
     {
+      uint32_t t[8];
       double a[8], x[8];
 #pragma unroll
       for (int r = 0; r < 8; ++r)
+        t[r] = terms[q + r];
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
       {
-        const uint32_t t = terms[q + r];
-        a[r] = A[t & 0xffffu];
-        x[r] = X[t >> 16];
+        a[r] = A[(t[r] & 0xffffu) * TC + cc];
+        x[r] = X[(t[r] >> 16) * TC + cc];
       }
 #pragma unroll
       for (int r = 0; r < 8; ++r)
         v -= a[r] * x[r];
     }
-    for (; q < e; ++q)
+    if (q < e)
     {
-      const uint32_t t = terms[q];
-      v -= A[t & 0xffffu] * X[t >> 16];
+      uint32_t t[8];
+      double a[8], x[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        t[r] = terms[q + r];
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+      {
+        a[r] = A[(t[r] & 0xffffu) * TC + cc];
+        x[r] = X[(t[r] >> 16) * TC + cc];
+      }
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        if (q + r < e)
+          v -= a[r] * x[r];
     }
     return v;
   }

@@ -173,6 +173,8 @@ namespace micm_hip
         s.offsets.push_back(static_cast<int32_t>(v.size()));
       s.tasks.push_back(0xffffffffu);
       s.tasks.push_back(static_cast<uint32_t>(s.terms.size()));
+      // eight entries naming position 0 let a batch of eight read past a task's last term
+      s.terms.insert(s.terms.end(), 8, 0u);
       return s;
     }
 

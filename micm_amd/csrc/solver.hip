@@ -226,12 +226,7 @@ namespace
         const uint2 task = lv.tasks[t];
         const uint32_t qe = lv.tasks[t + 1].y;
         const uint32_t o = task.x & 0xffffu, dg = task.x >> 16;
-        double v = alu[o * TC + cc];
-        for (uint32_t q = task.y; q < qe; ++q)
-        {
-          const uint32_t pr = lv.terms[q];
-          v -= alu[(pr & 0xffffu) * TC + cc] * alu[(pr >> 16) * TC + cc];
-        }
+        double v = coop_dot<TC>(alu[o * TC + cc], lv.terms, task.y, qe, alu, alu, cc);
         if (dg != 0xffffu)
           v = v / alu[dg * TC + cc];
         alu[o * TC + cc] = v;
@@ -309,12 +304,7 @@ namespace
           const uint2 task = lv.tasks[t];
           const uint32_t qe = lv.tasks[t + 1].y;
           const uint32_t o = task.x & 0xffffu, dg = task.x >> 16;
-          double v = xt[o * TC + cc];
-          for (uint32_t q = task.y; q < qe; ++q)
-          {
-            const uint32_t pr = lv.terms[q];
-            v -= alu[(pr & 0xffffu) * TC + cc] * xt[(pr >> 16) * TC + cc];
-          }
+          double v = coop_dot<TC>(xt[o * TC + cc], lv.terms, task.y, qe, alu, xt, cc);
           if (pass == 0)
             v = v / ld[o * TC + cc];
           else
@@ -826,10 +816,11 @@ namespace
 
   // ---- cooperative form: one workgroup per cell, working set in LDS
 
-  // stage: bit 0 the substitution tables, bit 1 the LU tables, copied into LDS once per
-  // workgroup after the per-cell data
-  template<int NT>
-  __global__ void __launch_bounds__(NT) k_coop(DevMech m, DevCoop c, DevParams p, int stage, int64_t n_fixed,
+  // STAGE: bit 0 the substitution tables, bit 1 the LU tables, copied into LDS once per
+  // workgroup after the per-cell data.  A compile-time choice, so that every table
+  // access is a plain LDS or global access.
+  template<int NT, int STAGE>
+  __global__ void __launch_bounds__(NT) k_coop(DevMech m, DevCoop c, DevParams p, int64_t n_fixed,
                                                const int32_t* __restrict__ list, const int* list_count, int* next,
                                                const CellControl* resume, double* __restrict__ y_state,
                                                double* __restrict__ k_state, const double* __restrict__ atol,
@@ -855,7 +846,7 @@ namespace
         w += (words + 1) & ~size_t(1);  // keep 8-byte alignment for the task pairs
         return dst;
       };
-      if (stage & 1)
+      if constexpr ((STAGE & 1) != 0)
       {
         T.fwd_tasks = reinterpret_cast<const uint2*>(
             stage_copy(reinterpret_cast<const uint32_t*>(c.fwd.tasks), 2 * (static_cast<size_t>(c.fwd.n_tasks) + 1)));
@@ -864,7 +855,7 @@ namespace
             stage_copy(reinterpret_cast<const uint32_t*>(c.bwd.tasks), 2 * (static_cast<size_t>(c.bwd.n_tasks) + 1)));
         T.bwd_terms = stage_copy(c.bwd.terms, c.bwd.n_terms);
       }
-      if (stage & 2)
+      if constexpr ((STAGE & 2) != 0)
       {
         T.lu_tasks = reinterpret_cast<const uint2*>(
             stage_copy(reinterpret_cast<const uint32_t*>(c.lu.tasks), 2 * (static_cast<size_t>(c.lu.n_tasks) + 1)));
@@ -1454,27 +1445,39 @@ namespace
     return plan_coop(m, stages).lds;
   }
 
-  template<int NT>
-  int launch_coop_nt(micm_hip_state* s, const DevParams& p, const CoopPlan& plan, const int32_t* list,
-                     const int* count, int64_t n_fixed, const CellControl* resume, bool update_rates, hipStream_t st)
+  template<int NT, int STAGE>
+  int launch_coop_shape(micm_hip_state* s, const DevParams& p, const CoopPlan& plan, const int32_t* list,
+                        const int* count, int64_t n_fixed, const CellControl* resume, bool update_rates, hipStream_t st)
   {
     const micm_hip_mechanism* m = s->mech;
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_coop<NT>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                static_cast<int>(plan.lds)));
+    const void* fn = reinterpret_cast<const void*>(k_coop<NT, STAGE>);
+    HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(plan.lds)));
     int dev = 0, cus = 0, per_cu = 0;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coop<NT>, NT, plan.lds));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coop<NT, STAGE>, NT, plan.lds));
     int64_t blocks = static_cast<int64_t>(cus) * std::max(per_cu, 1);
     if (!list)
       blocks = std::max<int64_t>(1, std::min(blocks, n_fixed));
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, s->coop_next);
-    hipLaunchKernelGGL(k_coop<NT>, dim3(static_cast<unsigned>(blocks)), dim3(NT), plan.lds, st, m->dev, m->coop, p,
-                       plan.stage, n_fixed, list, count, s->coop_next, resume, s->y, s->k, s->atol, s->status,
-                       s->stats, s->final_time, update_rates ? s->cond : nullptr,
-                       update_rates ? s->custom : nullptr);
+    hipLaunchKernelGGL((k_coop<NT, STAGE>), dim3(static_cast<unsigned>(blocks)), dim3(NT), plan.lds, st, m->dev, m->coop,
+                       p, n_fixed, list, count, s->coop_next, resume, s->y, s->k, s->atol, s->status, s->stats,
+                       s->final_time, update_rates ? s->cond : nullptr, update_rates ? s->custom : nullptr);
     HIP_TRY(hipGetLastError());
     return MICM_HIP_OK;
+  }
+
+  template<int NT>
+  int launch_coop_nt(micm_hip_state* s, const DevParams& p, const CoopPlan& plan, const int32_t* list,
+                     const int* count, int64_t n_fixed, const CellControl* resume, bool update_rates, hipStream_t st)
+  {
+    switch (plan.stage)
+    {
+      case 1: return launch_coop_shape<NT, 1>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
+      case 2: return launch_coop_shape<NT, 2>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
+      case 3: return launch_coop_shape<NT, 3>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
+      default: return launch_coop_shape<NT, 0>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
+    }
   }
 
   int launch_coop(micm_hip_state* s, const DevParams& p, const int32_t* list, const int* count, int64_t n_fixed,

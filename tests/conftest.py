@@ -3,6 +3,12 @@ import sys
 
 import pytest
 
+try:
+    # the HIP library shares PyTorch's HIP runtime; PyTorch must be loaded first
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
