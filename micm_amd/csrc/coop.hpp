@@ -127,48 +127,48 @@ namespace micm_hip
 
   // v - sum_q A[lo(terms[q])] * X[hi(terms[q])] for q in [b, e), subtracting in order;
   // with TC cells interleaved in A and X, position p of cell cc is p * TC + cc.
-  // Terms go in batches of eight: the eight table entries, then the sixteen operands
-  // are loaded before the eight subtractions.  A short batch reads past the task's last
-  // term (the tables end in eight padding entries) and skips those subtractions.
+  // Terms go in batches of eight.  The table entries of the next batch are loaded
+  // together with the operands of the current one, so each batch waits for one round
+  // trip; then its eight subtractions run in order.  A short batch reads past the
+  // task's last term (the tables end in eight padding entries) and skips those
+  // subtractions.
   template<int TC = 1>
   __device__ __forceinline__ double coop_dot(double v, const uint32_t* terms, uint32_t b, uint32_t e, const double* A,
                                              const double* X, int cc = 0)
   {
-    uint32_t q = b;
-    for (; q + 8 <= e; q += 8)
-    {
-      uint32_t t[8];
-      double a[8], x[8];
+    if (b >= e)
+      return v;
+    uint32_t t[8];
 #pragma unroll
-      for (int r = 0; r < 8; ++r)
-        t[r] = terms[q + r];
+    for (int r = 0; r < 8; ++r)
+      t[r] = terms[b + r];
+    for (uint32_t q = b; q < e; q += 8)
+    {
+      double a[8], x[8];
 #pragma unroll
       for (int r = 0; r < 8; ++r)
       {
         a[r] = A[(t[r] & 0xffffu) * TC + cc];
         x[r] = X[(t[r] >> 16) * TC + cc];
       }
+      // the next batch's entries (past the end: padding or the next task's terms)
+      const uint32_t qn = q + 8 < e ? q + 8 : q;
 #pragma unroll
       for (int r = 0; r < 8; ++r)
-        v -= a[r] * x[r];
-    }
-    if (q < e)
-    {
-      uint32_t t[8];
-      double a[8], x[8];
-#pragma unroll
-      for (int r = 0; r < 8; ++r)
-        t[r] = terms[q + r];
-#pragma unroll
-      for (int r = 0; r < 8; ++r)
+        t[r] = terms[qn + r];
+      if (q + 8 <= e)
       {
-        a[r] = A[(t[r] & 0xffffu) * TC + cc];
-        x[r] = X[(t[r] >> 16) * TC + cc];
-      }
 #pragma unroll
-      for (int r = 0; r < 8; ++r)
-        if (q + r < e)
+        for (int r = 0; r < 8; ++r)
           v -= a[r] * x[r];
+      }
+      else
+      {
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          if (q + r < e)
+            v -= a[r] * x[r];
+      }
     }
     return v;
   }
