@@ -47,21 +47,30 @@ namespace micm_hip
   struct CoopLds
   {
     double *alu, *jac, *y, *ynew, *f0, *yerr, *K, *k, *scratch;
-    double* jterm;  // Jacobian partial products: the ALU array when it is large enough
+    double* jterm;     // Jacobian partial products: the ALU array when it is large enough
+    double* products;  // per-level products of the LU and substitution schedules
+    int p_capacity;    // doubles in products
   };
 
-  // Doubles of per-cell LDS data.  The Jacobian's partial products live in the ALU
-  // array (rebuilt from the Jacobian before every factorisation) when it is large enough.
-  __host__ __device__ inline size_t coop_lds_doubles(int nnz_alu, int nnz_jac, int n, int stages, int n_reactions,
-                                                      int n_terms)
+  __host__ __device__ inline int coop_scratch_doubles(int n, int n_reactions, int nnz_alu, int n_terms)
   {
     int scratch = n_reactions > n ? n_reactions : n;
     if (n_terms > nnz_alu && n_terms > scratch)
       scratch = n_terms;
-    return static_cast<size_t>(nnz_alu) + nnz_jac + static_cast<size_t>(4 + stages) * n + n_reactions + scratch;
+    return scratch;
   }
 
-  __device__ inline CoopLds coop_layout(double* lds, const DevMech& m, const DevCoop& c, int stages)
+  // Doubles of per-cell LDS data, the per-level product buffer (p_capacity) included.
+  // The Jacobian's partial products live in the ALU array (rebuilt from the Jacobian
+  // before every factorisation) when it is large enough.
+  __host__ __device__ inline size_t coop_lds_doubles(int nnz_alu, int nnz_jac, int n, int stages, int n_reactions,
+                                                      int n_terms, int p_capacity)
+  {
+    return static_cast<size_t>(nnz_alu) + nnz_jac + static_cast<size_t>(4 + stages) * n + n_reactions +
+           coop_scratch_doubles(n, n_reactions, nnz_alu, n_terms) + p_capacity;
+  }
+
+  __device__ inline CoopLds coop_layout(double* lds, const DevMech& m, const DevCoop& c, int stages, int p_capacity)
   {
     CoopLds L;
     const int n = m.n_species;
@@ -75,6 +84,8 @@ namespace micm_hip
     L.k = L.K + static_cast<size_t>(stages) * n;
     L.scratch = L.k + m.n_reactions;
     L.jterm = c.n_terms <= c.nnz_alu ? L.alu : L.scratch;
+    L.products = L.scratch + coop_scratch_doubles(n, m.n_reactions, c.nnz_alu, c.n_terms);
+    L.p_capacity = p_capacity;
     return L;
   }
 
@@ -127,11 +138,11 @@ namespace micm_hip
 
   // v - sum_q A[lo(terms[q])] * X[hi(terms[q])] for q in [b, e), subtracting in order;
   // with TC cells interleaved in A and X, position p of cell cc is p * TC + cc.
-  // Terms go in batches of eight.  The table entries of the next batch are loaded
-  // together with the operands of the current one, so each batch waits for one round
-  // trip; then its eight subtractions run in order.  A short batch reads past the
-  // task's last term (the tables end in eight padding entries) and skips those
-  // subtractions.
+  // Terms go in batches of eight through a two-deep pipeline: while the subtractions
+  // of batch i run in order, the operands of batch i + 1 and the table entries of
+  // batch i + 2 are in flight.  Batches past the task's end read padding or the next
+  // task's entries (the tables end in sixteen padding entries) and are not
+  // subtracted; a short last batch skips the missing terms.
   template<int TC = 1>
   __device__ __forceinline__ double coop_dot(double v, const uint32_t* terms, uint32_t b, uint32_t e, const double* A,
                                              const double* X, int cc = 0)
@@ -139,23 +150,32 @@ namespace micm_hip
     if (b >= e)
       return v;
     uint32_t t[8];
+    double a[8], x[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r)
       t[r] = terms[b + r];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+    {
+      a[r] = A[(t[r] & 0xffffu) * TC + cc];
+      x[r] = X[(t[r] >> 16) * TC + cc];
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+      t[r] = terms[b + 8 + r];
     for (uint32_t q = b; q < e; q += 8)
     {
-      double a[8], x[8];
+      double an[8], xn[8];
 #pragma unroll
       for (int r = 0; r < 8; ++r)
       {
-        a[r] = A[(t[r] & 0xffffu) * TC + cc];
-        x[r] = X[(t[r] >> 16) * TC + cc];
+        an[r] = A[(t[r] & 0xffffu) * TC + cc];
+        xn[r] = X[(t[r] >> 16) * TC + cc];
       }
-      // the next batch's entries (past the end: padding or the next task's terms)
-      const uint32_t qn = q + 8 < e ? q + 8 : q;
+      const uint32_t q2 = q + 16 < e ? q + 16 : q;
 #pragma unroll
       for (int r = 0; r < 8; ++r)
-        t[r] = terms[qn + r];
+        t[r] = terms[q2 + r];
       if (q + 8 <= e)
       {
 #pragma unroll
@@ -169,29 +189,80 @@ namespace micm_hip
           if (q + r < e)
             v -= a[r] * x[r];
       }
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+      {
+        a[r] = an[r];
+        x[r] = xn[r];
+      }
     }
     return v;
   }
 
   // Runs a level schedule: task t computes out[o] = (out[o] - dot) / A[diag] (no
   // division when diag is 0xffff), reading A and X.
+  //
+  // A level whose products fit the buffer P (p_capacity doubles) runs in two phases:
+  // every thread forms products of the level (the level's terms are contiguous), then
+  // one thread per task subtracts its products from P in order.  Each product is
+  // rounded once either way, so the result is that of subtracting a * x term by
+  // term; the long sequential part reads a contiguous stream instead of two indexed
+  // operands per term.  Larger levels run task by task.
   template<int NT>
   __device__ inline void coop_levels(const DevLevels& lv, const uint2* tasks, const uint32_t* terms, const double* A,
-                                     const double* X, double* out)
+                                     const double* X, double* out, double* P, int p_capacity)
   {
     const int tid = threadIdx.x;
     for (int l = 0; l < lv.n_levels; ++l)
     {
       const int tb = lv.offsets[l], te = lv.offsets[l + 1];
-      for (int t = tb + tid; t < te; t += NT)
+      const uint32_t qb = tasks[tb].y, qe = tasks[te].y;
+      if (static_cast<int>(qe - qb) <= p_capacity)
       {
-        const uint2 task = tasks[t];
-        const uint32_t qe = tasks[t + 1].y;
-        const uint32_t o = task.x & 0xffffu, d = task.x >> 16;
-        double v = coop_dot(out[o], terms, task.y, qe, A, X);
-        if (d != 0xffffu)
-          v = v / A[d];
-        out[o] = v;
+        for (uint32_t q = qb + tid; q < qe; q += NT)
+        {
+          const uint32_t tr = terms[q];
+          P[q - qb] = A[tr & 0xffffu] * X[tr >> 16];
+        }
+        coop_sync<NT>();
+        for (int t = tb + tid; t < te; t += NT)
+        {
+          const uint2 task = tasks[t];
+          const uint32_t e = tasks[t + 1].y;
+          const uint32_t o = task.x & 0xffffu, d = task.x >> 16;
+          double v = out[o];
+          const double* p = P + (task.y - qb);
+          const uint32_t n = e - task.y;
+          uint32_t q = 0;
+          for (; q + 8 <= n; q += 8)
+          {
+            double s[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+              s[r] = p[q + r];
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+              v -= s[r];
+          }
+          for (; q < n; ++q)
+            v -= p[q];
+          if (d != 0xffffu)
+            v = v / A[d];
+          out[o] = v;
+        }
+      }
+      else
+      {
+        for (int t = tb + tid; t < te; t += NT)
+        {
+          const uint2 task = tasks[t];
+          const uint32_t qe2 = tasks[t + 1].y;
+          const uint32_t o = task.x & 0xffffu, d = task.x >> 16;
+          double v = coop_dot(out[o], terms, task.y, qe2, A, X);
+          if (d != 0xffffu)
+            v = v / A[d];
+          out[o] = v;
+        }
       }
       coop_sync<NT>();
     }
@@ -215,15 +286,15 @@ namespace micm_hip
     for (int e = tid; e < m.nnz_jac; e += NT)
       L.alu[c.jac_to_alu[e]] = L.jac[e];
     coop_sync<NT>();
-    coop_levels<NT>(c.lu, T.lu_tasks, T.lu_terms, L.alu, L.alu, L.alu);
+    coop_levels<NT>(c.lu, T.lu_tasks, T.lu_terms, L.alu, L.alu, L.alu, L.products, L.p_capacity);
   }
 
   // x <- U^-1 L^-1 x with the factors in L.alu (unit lower diagonal).
   template<int NT>
   __device__ inline void coop_solve(const DevCoop& c, const CoopLds& L, const CoopTables& T, double* x)
   {
-    coop_levels<NT>(c.fwd, T.fwd_tasks, T.fwd_terms, L.alu, x, x);
-    coop_levels<NT>(c.bwd, T.bwd_tasks, T.bwd_terms, L.alu, x, x);
+    coop_levels<NT>(c.fwd, T.fwd_tasks, T.fwd_terms, L.alu, x, x, L.products, L.p_capacity);
+    coop_levels<NT>(c.bwd, T.bwd_tasks, T.bwd_terms, L.alu, x, x, L.products, L.p_capacity);
   }
 
   // 32-bit words of the tables a launch stages in LDS.

@@ -63,6 +63,7 @@ namespace micm_hip
     const uint2* tasks;      // n_tasks + 1
     const uint32_t* terms;   // two 16-bit positions
     int32_t n_levels, n_tasks, n_terms;
+    int32_t max_level_terms;  // largest number of terms of one level
   };
 
   // Tables of the cooperative (one workgroup per cell) form; see MechanismTables.

@@ -173,8 +173,11 @@ namespace micm_hip
         s.offsets.push_back(static_cast<int32_t>(v.size()));
       s.tasks.push_back(0xffffffffu);
       s.tasks.push_back(static_cast<uint32_t>(s.terms.size()));
-      // eight entries naming position 0 let a batch of eight read past a task's last term
-      s.terms.insert(s.terms.end(), 8, 0u);
+      for (size_t l = 0; l + 1 < s.offsets.size(); ++l)
+        s.max_level_terms = std::max<int32_t>(
+            s.max_level_terms, static_cast<int32_t>(s.tasks[2 * s.offsets[l + 1] + 1] - s.tasks[2 * s.offsets[l] + 1]));
+      // sixteen entries naming position 0 let the batches of eight read past a task's last term
+      s.terms.insert(s.terms.end(), 16, 0u);
       return s;
     }
 

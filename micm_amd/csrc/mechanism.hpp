@@ -17,8 +17,9 @@ namespace micm_hip
   {
     std::vector<int32_t> offsets;  // level -> first task, n_levels + 1
     std::vector<uint32_t> tasks;   // 2 per task + sentinel: out | diag << 16, first term
-    std::vector<uint32_t> terms;   // two 16-bit fields, then eight zero entries of padding
+    std::vector<uint32_t> terms;   // two 16-bit fields, then sixteen zero entries of padding
     int32_t max_terms = 0;
+    int32_t max_level_terms = 0;  // largest number of terms of one level
     int32_t n_levels() const { return offsets.empty() ? 0 : static_cast<int32_t>(offsets.size()) - 1; }
     int32_t n_tasks() const { return static_cast<int32_t>(tasks.size() / 2) - 1; }
   };

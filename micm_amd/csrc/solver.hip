@@ -820,7 +820,7 @@ namespace
   // workgroup after the per-cell data.  A compile-time choice, so that every table
   // access is a plain LDS or global access.
   template<int NT, int STAGE>
-  __global__ void __launch_bounds__(NT) k_coop(DevMech m, DevCoop c, DevParams p, int64_t n_fixed,
+  __global__ void __launch_bounds__(NT) k_coop(DevMech m, DevCoop c, DevParams p, int p_capacity, int64_t n_fixed,
                                                const int32_t* __restrict__ list, const int* list_count, int* next,
                                                const CellControl* resume, double* __restrict__ y_state,
                                                double* __restrict__ k_state, const double* __restrict__ atol,
@@ -833,12 +833,12 @@ namespace
     __shared__ double s_error;
     const int lane = threadIdx.x;
     const int n = m.n_species;
-    const CoopLds L = coop_layout(lds, m, c, p.stages);
+    const CoopLds L = coop_layout(lds, m, c, p.stages, p_capacity);
     const int64_t count = list ? static_cast<int64_t>(*list_count) : n_fixed;
     CoopTables T{ c.lu.tasks, c.fwd.tasks, c.bwd.tasks, c.lu.terms, c.fwd.terms, c.bwd.terms };
     {
       uint32_t* w = reinterpret_cast<uint32_t*>(lds + coop_lds_doubles(c.nnz_alu, m.nnz_jac, n, p.stages,
-                                                                         m.n_reactions, c.n_terms));
+                                                                         m.n_reactions, c.n_terms, p_capacity));
       auto stage_copy = [&](const uint32_t* src, size_t words) {
         for (size_t q = lane; q < words; q += NT)
           w[q] = src[q];
@@ -1108,6 +1108,7 @@ namespace
     d.n_levels = l.n_levels();
     d.n_tasks = l.tasks.empty() ? 0 : l.n_tasks();
     d.n_terms = static_cast<int32_t>(l.terms.size());
+    d.max_level_terms = l.max_level_terms;
   }
 
 
@@ -1364,6 +1365,7 @@ namespace
     size_t lds = 0;
     int stage = 0;
     int threads = kWave;
+    int p_capacity = 0;  // doubles of the per-level product buffer
   };
 
   size_t lds_limit()
@@ -1406,8 +1408,12 @@ namespace
     const DevCoop& c = m->coop;
     if (!m->t.coop_levels_ok)
       return plan;
+    constexpr size_t kLdsPerCu = 160 * 1024;
+    // the product buffer always covers the substitution levels; LU levels too where
+    // that costs no resident cell per CU (larger LU levels run task by task)
+    const int p_solve = std::max(c.fwd.max_level_terms, c.bwd.max_level_terms);
     const size_t base = coop_lds_doubles(c.nnz_alu, m->dev.nnz_jac, m->dev.n_species, stages, m->dev.n_reactions,
-                                         c.n_terms) *
+                                         c.n_terms, p_solve) *
                         sizeof(double);
     const size_t solve = (coop_solve_table_words(c) + 4) * sizeof(uint32_t);
     const size_t lu = (coop_lu_table_words(c) + 2) * sizeof(uint32_t);
@@ -1421,7 +1427,6 @@ namespace
     }
     else
     {
-      constexpr size_t kLdsPerCu = 160 * 1024;
       if (base + solve <= limit)
         stage |= 1;
       const size_t with_solve = base + ((stage & 1) ? solve : 0);
@@ -1433,9 +1438,20 @@ namespace
       stage &= ~2;
     if (base + ((stage & 1) ? solve : 0) > limit)
       stage &= ~1;
-    const size_t bytes = base + ((stage & 1) ? solve : 0) + ((stage & 2) ? lu : 0);
+    size_t bytes = base + ((stage & 1) ? solve : 0) + ((stage & 2) ? lu : 0);
+    int p = p_solve;
+    if (c.lu.max_level_terms > p)
+    {
+      const size_t per_cu = kLdsPerCu / bytes;
+      const size_t room = std::min(limit, per_cu > 0 ? kLdsPerCu / per_cu : limit);
+      const size_t extra = room > bytes ? (room - bytes) / sizeof(double) : 0;
+      const int grow = static_cast<int>(std::min<size_t>(extra, static_cast<size_t>(c.lu.max_level_terms - p)));
+      p += grow;
+      bytes += static_cast<size_t>(grow) * sizeof(double);
+    }
     plan.lds = bytes;
     plan.stage = stage;
+    plan.p_capacity = p;
     plan.threads = (threads_request == 128 || threads_request == 256) ? threads_request : kWave;
     return plan;
   }
@@ -1461,7 +1477,7 @@ namespace
       blocks = std::max<int64_t>(1, std::min(blocks, n_fixed));
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, s->coop_next);
     hipLaunchKernelGGL((k_coop<NT, STAGE>), dim3(static_cast<unsigned>(blocks)), dim3(NT), plan.lds, st, m->dev, m->coop,
-                       p, n_fixed, list, count, s->coop_next, resume, s->y, s->k, s->atol, s->status, s->stats,
+                       p, plan.p_capacity, n_fixed, list, count, s->coop_next, resume, s->y, s->k, s->atol, s->status, s->stats,
                        s->final_time, update_rates ? s->cond : nullptr, update_rates ? s->custom : nullptr);
     HIP_TRY(hipGetLastError());
     return MICM_HIP_OK;
