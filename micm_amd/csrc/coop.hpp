@@ -219,10 +219,30 @@ namespace micm_hip
       const uint32_t qb = tasks[tb].y, qe = tasks[te].y;
       if (static_cast<int>(qe - qb) <= p_capacity)
       {
-        for (uint32_t q = qb + tid; q < qe; q += NT)
+        // eight products per thread per round: table entries, then operands, then products
+        for (uint32_t q0 = qb + tid; q0 < qe; q0 += 8 * NT)
         {
-          const uint32_t tr = terms[q];
-          P[q - qb] = A[tr & 0xffffu] * X[tr >> 16];
+          uint32_t tr[8];
+          double a[8], x[8];
+#pragma unroll
+          for (int r = 0; r < 8; ++r)
+          {
+            const uint32_t q = q0 + r * NT;
+            tr[r] = terms[q < qe ? q : qb];
+          }
+#pragma unroll
+          for (int r = 0; r < 8; ++r)
+          {
+            a[r] = A[tr[r] & 0xffffu];
+            x[r] = X[tr[r] >> 16];
+          }
+#pragma unroll
+          for (int r = 0; r < 8; ++r)
+          {
+            const uint32_t q = q0 + r * NT;
+            if (q < qe)
+              P[q - qb] = a[r] * x[r];
+          }
         }
         coop_sync<NT>();
         for (int t = tb + tid; t < te; t += NT)
