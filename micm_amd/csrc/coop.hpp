@@ -219,6 +219,20 @@ namespace micm_hip
       const uint32_t qb = tasks[tb].y, qe = tasks[te].y;
       if (static_cast<int>(qe - qb) <= p_capacity)
       {
+        // the first task of each thread: descriptor, starting value and divisor are
+        // loaded before the products, which they do not depend on
+        const int t0 = tb + tid;
+        uint2 task0 = make_uint2(0u, 0u);
+        uint32_t e0 = 0;
+        double v0 = 0.0, d0 = 1.0;
+        if (t0 < te)
+        {
+          task0 = tasks[t0];
+          e0 = tasks[t0 + 1].y;
+          v0 = out[task0.x & 0xffffu];
+          if ((task0.x >> 16) != 0xffffu)
+            d0 = A[task0.x >> 16];
+        }
         // eight products per thread per round: table entries, then operands, then products
         for (uint32_t q0 = qb + tid; q0 < qe; q0 += 8 * NT)
         {
@@ -245,30 +259,47 @@ namespace micm_hip
           }
         }
         coop_sync<NT>();
-        for (int t = tb + tid; t < te; t += NT)
+        for (int t = t0; t < te; t += NT)
         {
-          const uint2 task = tasks[t];
-          const uint32_t e = tasks[t + 1].y;
-          const uint32_t o = task.x & 0xffffu, d = task.x >> 16;
-          double v = out[o];
+          uint2 task = task0;
+          uint32_t e = e0;
+          double v = v0, dv = d0;
+          if (t != t0)
+          {
+            task = tasks[t];
+            e = tasks[t + 1].y;
+            v = out[task.x & 0xffffu];
+            if ((task.x >> 16) != 0xffffu)
+              dv = A[task.x >> 16];
+          }
           const double* p = P + (task.y - qb);
           const uint32_t n = e - task.y;
           uint32_t q = 0;
-          for (; q + 8 <= n; q += 8)
+          for (; q + 16 <= n; q += 16)
           {
-            double s[8];
+            double s[16];
 #pragma unroll
-            for (int r = 0; r < 8; ++r)
+            for (int r = 0; r < 16; ++r)
               s[r] = p[q + r];
 #pragma unroll
-            for (int r = 0; r < 8; ++r)
+            for (int r = 0; r < 16; ++r)
+              v -= s[r];
+          }
+          for (; q + 4 <= n; q += 4)
+          {
+            double s[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              s[r] = p[q + r];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
               v -= s[r];
           }
           for (; q < n; ++q)
             v -= p[q];
-          if (d != 0xffffu)
-            v = v / A[d];
-          out[o] = v;
+          if ((task.x >> 16) != 0xffffu)
+            v = v / dv;
+          out[task.x & 0xffffu] = v;
         }
       }
       else
