@@ -89,27 +89,94 @@ namespace micm_hip
     return L;
   }
 
+  // product of up to three packed operands, in list order, onto v; 0xffff counts
+  // fall back to the index lists
+  __device__ __forceinline__ double packed_product(double v, uint64_t pk, double y0, double y1, double y2)
+  {
+    const uint32_t count = static_cast<uint32_t>(pk >> 48);
+    if (count >= 1)
+      v *= y0;
+    if (count >= 2)
+      v *= y1;
+    if (count >= 3)
+      v *= y2;
+    return v;
+  }
+
+  // v + sum_q coef[q] * src[idx[q]] for q in [b, e), added in order, in batches of eight
+  // (the lists end in eight padding entries)
+  __device__ __forceinline__ double gather_sum(double v, const int32_t* idx, const double* coef, int b, int e,
+                                               const double* src)
+  {
+    for (int q = b; q < e; q += 8)
+    {
+      int ix[8];
+      double cf[8], sv[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+      {
+        ix[r] = idx[q + r];
+        cf[r] = coef[q + r];
+      }
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        sv[r] = src[ix[r]];
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        if (q + r < e)
+          v += cf[r] * sv[r];
+    }
+    return v;
+  }
+
   // out[s] = f(yv)[s]
   template<int NT>
   __device__ inline void coop_forcing(const DevMech& m, const DevCoop& c, const CoopLds& L, const double* yv,
                                       double* out)
   {
     const int tid = threadIdx.x;
-    for (int r = tid; r < m.n_reactions; r += NT)
+    const int R = m.n_reactions;
+    for (int r0 = tid; r0 < R; r0 += 4 * NT)
     {
-      double rate = L.k[r];
-      for (int i = m.reactant_offsets[r]; i < m.reactant_offsets[r + 1]; ++i)
-        rate *= yv[m.reactant_ids[i]];
-      L.scratch[r] = rate;
+      uint64_t pk[4];
+      double kk[4], y0[4], y1[4], y2[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+      {
+        const int r = r0 + j * NT < R ? r0 + j * NT : 0;
+        pk[j] = c.rxn_pack[r];
+        kk[j] = L.k[r];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+      {
+        y0[j] = yv[pk[j] & 0xffffu];
+        y1[j] = yv[(pk[j] >> 16) & 0xffffu];
+        y2[j] = yv[(pk[j] >> 32) & 0xffffu];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+      {
+        const int r = r0 + j * NT;
+        if (r < R)
+        {
+          double rate = kk[j];
+          if ((pk[j] >> 48) == 0xffffu)
+          {
+            for (int i = m.reactant_offsets[r]; i < m.reactant_offsets[r + 1]; ++i)
+              rate *= yv[m.reactant_ids[i]];
+          }
+          else
+          {
+            rate = packed_product(rate, pk[j], y0[j], y1[j], y2[j]);
+          }
+          L.scratch[r] = rate;
+        }
+      }
     }
     coop_sync<NT>();
     for (int s = tid; s < m.n_species; s += NT)
-    {
-      double v = 0.0;
-      for (int q = c.spec_offsets[s]; q < c.spec_offsets[s + 1]; ++q)
-        v += c.spec_coef[q] * L.scratch[c.spec_rxn[q]];
-      out[s] = v;
-    }
+      out[s] = gather_sum(0.0, c.spec_rxn, c.spec_coef, c.spec_offsets[s], c.spec_offsets[s + 1], L.scratch);
     coop_sync<NT>();
   }
 
@@ -118,21 +185,47 @@ namespace micm_hip
   __device__ inline void coop_jacobian(const DevMech& m, const DevCoop& c, const CoopLds& L)
   {
     const int tid = threadIdx.x;
-    for (int t = tid; t < c.n_terms; t += NT)
+    const int nt = c.n_terms;
+    for (int t0 = tid; t0 < nt; t0 += 4 * NT)
     {
-      double d = L.k[m.term_process[t]];
-      for (int i = m.term_dep_offsets[t]; i < m.term_dep_offsets[t + 1]; ++i)
-        d *= L.y[m.term_dep_ids[i]];
-      L.jterm[t] = d;
+      uint64_t pk[4];
+      double kk[4], y0[4], y1[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+      {
+        const int t = t0 + j * NT < nt ? t0 + j * NT : 0;
+        pk[j] = c.term_pack[t];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+      {
+        kk[j] = L.k[pk[j] & 0xffffu];
+        y0[j] = L.y[(pk[j] >> 16) & 0xffffu];
+        y1[j] = L.y[(pk[j] >> 32) & 0xffffu];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+      {
+        const int t = t0 + j * NT;
+        if (t < nt)
+        {
+          double d = kk[j];
+          if ((pk[j] >> 48) == 0xffffu)
+          {
+            for (int i = m.term_dep_offsets[t]; i < m.term_dep_offsets[t + 1]; ++i)
+              d *= L.y[m.term_dep_ids[i]];
+          }
+          else
+          {
+            d = packed_product(d, pk[j], y0[j], y1[j], 0.0);
+          }
+          L.jterm[t] = d;
+        }
+      }
     }
     coop_sync<NT>();
     for (int e = tid; e < m.nnz_jac; e += NT)
-    {
-      double v = 0.0;
-      for (int q = c.jel_offsets[e]; q < c.jel_offsets[e + 1]; ++q)
-        v += c.jel_coef[q] * L.jterm[c.jel_term[q]];
-      L.jac[e] = v;
-    }
+      L.jac[e] = gather_sum(0.0, c.jel_term, c.jel_coef, c.jel_offsets[e], c.jel_offsets[e + 1], L.jterm);
     coop_sync<NT>();
   }
 
@@ -208,7 +301,14 @@ namespace micm_hip
   // rounded once either way, so the result is that of subtracting a * x term by
   // term; the long sequential part reads a contiguous stream instead of two indexed
   // operands per term.  Larger levels run task by task.
-  template<int NT>
+  //
+  // V selects code shapes that give identical results (bit 0: products eight per
+  // thread per round; bit 1: each thread's first task loaded before the products;
+  // bit 2: sums in batches of sixteen, else eight; bit 3: the tail of a sum in
+  // batches of four).
+  constexpr int kLevelShape = 1 | 8;
+
+  template<int NT, int V = kLevelShape>
   __device__ inline void coop_levels(const DevLevels& lv, const uint2* tasks, const uint32_t* terms, const double* A,
                                      const double* X, double* out, double* P, int p_capacity)
   {
@@ -219,52 +319,70 @@ namespace micm_hip
       const uint32_t qb = tasks[tb].y, qe = tasks[te].y;
       if (static_cast<int>(qe - qb) <= p_capacity)
       {
-        // the first task of each thread: descriptor, starting value and divisor are
-        // loaded before the products, which they do not depend on
         const int t0 = tb + tid;
         uint2 task0 = make_uint2(0u, 0u);
         uint32_t e0 = 0;
         double v0 = 0.0, d0 = 1.0;
-        if (t0 < te)
+        if constexpr ((V & 2) != 0)
         {
-          task0 = tasks[t0];
-          e0 = tasks[t0 + 1].y;
-          v0 = out[task0.x & 0xffffu];
-          if ((task0.x >> 16) != 0xffffu)
-            d0 = A[task0.x >> 16];
+          if (t0 < te)
+          {
+            task0 = tasks[t0];
+            e0 = tasks[t0 + 1].y;
+            v0 = out[task0.x & 0xffffu];
+            if ((task0.x >> 16) != 0xffffu)
+              d0 = A[task0.x >> 16];
+          }
         }
-        // eight products per thread per round: table entries, then operands, then products
-        for (uint32_t q0 = qb + tid; q0 < qe; q0 += 8 * NT)
+        if constexpr ((V & 1) != 0)
         {
-          uint32_t tr[8];
-          double a[8], x[8];
-#pragma unroll
-          for (int r = 0; r < 8; ++r)
+          for (uint32_t q0 = qb + tid; q0 < qe; q0 += 8 * NT)
           {
-            const uint32_t q = q0 + r * NT;
-            tr[r] = terms[q < qe ? q : qb];
+            uint32_t tr[8];
+            double a[8], x[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+            {
+              const uint32_t q = q0 + r * NT;
+              tr[r] = terms[q < qe ? q : qb];
+            }
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+            {
+              a[r] = A[tr[r] & 0xffffu];
+              x[r] = X[tr[r] >> 16];
+            }
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+            {
+              const uint32_t q = q0 + r * NT;
+              if (q < qe)
+                P[q - qb] = a[r] * x[r];
+            }
           }
-#pragma unroll
-          for (int r = 0; r < 8; ++r)
+        }
+        else
+        {
+          for (uint32_t q = qb + tid; q < qe; q += NT)
           {
-            a[r] = A[tr[r] & 0xffffu];
-            x[r] = X[tr[r] >> 16];
-          }
-#pragma unroll
-          for (int r = 0; r < 8; ++r)
-          {
-            const uint32_t q = q0 + r * NT;
-            if (q < qe)
-              P[q - qb] = a[r] * x[r];
+            const uint32_t tr = terms[q];
+            P[q - qb] = A[tr & 0xffffu] * X[tr >> 16];
           }
         }
         coop_sync<NT>();
         for (int t = t0; t < te; t += NT)
         {
-          uint2 task = task0;
-          uint32_t e = e0;
-          double v = v0, dv = d0;
-          if (t != t0)
+          uint2 task;
+          uint32_t e;
+          double v, dv = 1.0;
+          if ((V & 2) != 0 && t == t0)
+          {
+            task = task0;
+            e = e0;
+            v = v0;
+            dv = d0;
+          }
+          else
           {
             task = tasks[t];
             e = tasks[t + 1].y;
@@ -275,25 +393,29 @@ namespace micm_hip
           const double* p = P + (task.y - qb);
           const uint32_t n = e - task.y;
           uint32_t q = 0;
-          for (; q + 16 <= n; q += 16)
+          constexpr int kB = (V & 4) != 0 ? 16 : 8;
+          for (; q + kB <= n; q += kB)
           {
-            double s[16];
+            double s[kB];
 #pragma unroll
-            for (int r = 0; r < 16; ++r)
+            for (int r = 0; r < kB; ++r)
               s[r] = p[q + r];
 #pragma unroll
-            for (int r = 0; r < 16; ++r)
+            for (int r = 0; r < kB; ++r)
               v -= s[r];
           }
-          for (; q + 4 <= n; q += 4)
+          if constexpr ((V & 8) != 0)
           {
-            double s[4];
+            for (; q + 4 <= n; q += 4)
+            {
+              double s[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-              s[r] = p[q + r];
+              for (int r = 0; r < 4; ++r)
+                s[r] = p[q + r];
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-              v -= s[r];
+              for (int r = 0; r < 4; ++r)
+                v -= s[r];
+            }
           }
           for (; q < n; ++q)
             v -= p[q];

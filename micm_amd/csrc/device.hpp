@@ -76,6 +76,8 @@ namespace micm_hip
     const int32_t* alu_lower;  // L storage index -> ALU position (-1: the unit diagonal)
     const int32_t* alu_upper;  // U storage index -> ALU position
     DevLevels lu, fwd, bwd;
+    const uint64_t* rxn_pack;   // reactant ids of each reaction (see MechanismTables)
+    const uint64_t* term_pack;  // process and dependent ids of each Jacobian term
     const int32_t* spec_offsets;
     const int32_t* spec_rxn;
     const double* spec_coef;

@@ -214,7 +214,7 @@ namespace micm_hip
           t.alu_upper[p] = alu(i, t.upper_col[p]);
       }
       // level schedules (16-bit positions; 0xffff marks "no divisor")
-      t.coop_levels_ok = t.alu_col.size() < 0xffff && n < 0xffff;
+      t.coop_levels_ok = t.alu_col.size() < 0xffff && n < 0xffff && t.n_reactions < 0xffff;
       t.lv_lu = LevelSchedule{};
       t.lv_fwd = LevelSchedule{};
       t.lv_bwd = LevelSchedule{};
@@ -342,6 +342,32 @@ namespace micm_hip
           t.jel_coef.push_back(cf);
         }
         t.jel_offsets.push_back(static_cast<int32_t>(t.jel_term.size()));
+      }
+      // eight padding entries let the gathers read in batches of eight past a list's end
+      t.spec_rxn.insert(t.spec_rxn.end(), 8, 0);
+      t.spec_coef.insert(t.spec_coef.end(), 8, 0.0);
+      t.jel_term.insert(t.jel_term.end(), 8, 0);
+      t.jel_coef.insert(t.jel_coef.end(), 8, 0.0);
+      // packed operand lists: up to three 16-bit ids and their count (0xffff: use the lists)
+      auto pack = [](const int32_t* ids, int32_t count, int32_t max_ids) {
+        if (count > max_ids)
+          return static_cast<uint64_t>(0xffff) << 48;
+        uint64_t v = static_cast<uint64_t>(count) << 48;
+        for (int32_t i = 0; i < count; ++i)
+          v |= static_cast<uint64_t>(static_cast<uint16_t>(ids[i])) << (16 * i);
+        return v;
+      };
+      t.rxn_pack.clear();
+      for (int32_t r = 0; r < (int32_t)t.n_reactions; ++r)
+        t.rxn_pack.push_back(pack(t.reactant_ids.data() + t.reactant_offsets[r],
+                                  t.reactant_offsets[r + 1] - t.reactant_offsets[r], 3));
+      t.term_pack.clear();
+      for (size_t term = 0; term < t.term_process.size(); ++term)
+      {
+        const int32_t b = t.term_dep_offsets[term], e = t.term_dep_offsets[term + 1];
+        uint64_t v = pack(t.term_dep_ids.data() + b, e - b, 2);
+        v = (v & 0xffff000000000000ull) | ((v & 0xffffffffull) << 16) | static_cast<uint16_t>(t.term_process[term]);
+        t.term_pack.push_back(v);
       }
     }
   }  // namespace

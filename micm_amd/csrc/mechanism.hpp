@@ -88,6 +88,9 @@ namespace micm_hip
     LevelSchedule lv_lu;   // out = ALU position, diag = U[i][i] position (L elements), terms (l | u << 16)
     LevelSchedule lv_fwd;  // out = row, terms (ALU position | column << 16)
     LevelSchedule lv_bwd;  // out = row, diag = U[i][i] position, terms (ALU position | column << 16)
+    // packed operands: reaction -> reactant ids, Jacobian term -> (process, dependent ids);
+    // three / two 16-bit ids and the count in bits 48-63 (0xffff: more, use the lists)
+    std::vector<uint64_t> rxn_pack, term_pack;
     // forcing as a per-species gather: f[s] += c * rate[r] over (r, c) in reaction order
     std::vector<int32_t> spec_offsets, spec_rxn;
     std::vector<double> spec_coef;

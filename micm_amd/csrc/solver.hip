@@ -1294,6 +1294,7 @@ namespace
              padded_bytes(t.jac_to_alu) + padded_bytes(t.alu_diag) + padded_bytes(t.alu_lower) +
              padded_bytes(t.alu_upper) + level_bytes(t.lv_lu) + level_bytes(t.lv_fwd) +
              level_bytes(t.lv_bwd) +
+             padded_bytes(t.rxn_pack) + padded_bytes(t.term_pack) +
              padded_bytes(t.spec_offsets) + padded_bytes(t.spec_rxn) +
              padded_bytes(t.spec_coef) + padded_bytes(t.jel_offsets) + padded_bytes(t.jel_term) +
              padded_bytes(t.jel_coef) + 256;
@@ -1346,6 +1347,8 @@ namespace
     upload_levels(m->blob, off, t.lv_lu, c.lu, staging);
     upload_levels(m->blob, off, t.lv_fwd, c.fwd, staging);
     upload_levels(m->blob, off, t.lv_bwd, c.bwd, staging);
+    upload(m->blob, off, t.rxn_pack, c.rxn_pack, staging);
+    upload(m->blob, off, t.term_pack, c.term_pack, staging);
     upload(m->blob, off, t.spec_offsets, c.spec_offsets, staging);
     upload(m->blob, off, t.spec_rxn, c.spec_rxn, staging);
     upload(m->blob, off, t.spec_coef, c.spec_coef, staging);
