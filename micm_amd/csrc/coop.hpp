@@ -303,10 +303,11 @@ namespace micm_hip
   // operands per term.  Larger levels run task by task.
   //
   // V selects code shapes that give identical results (bit 0: products eight per
-  // thread per round; bit 1: each thread's first task loaded before the products;
-  // bit 2: sums in batches of sixteen, else eight; bit 3: the tail of a sum in
-  // batches of four).
-  constexpr int kLevelShape = 1 | 8;
+  // thread per round on levels of more than 2 NT terms -- the small levels of the
+  // substitutions run faster one product per round; bit 1: each thread's first task
+  // loaded before the products; bit 2: sums in batches of sixteen, else eight;
+  // bit 3: the tail of a sum in batches of four).
+  constexpr int kLevelShape = 1;
 
   template<int NT, int V = kLevelShape>
   __device__ inline void coop_levels(const DevLevels& lv, const uint2* tasks, const uint32_t* terms, const double* A,
@@ -334,7 +335,7 @@ namespace micm_hip
               d0 = A[task0.x >> 16];
           }
         }
-        if constexpr ((V & 1) != 0)
+        if ((V & 1) != 0 && qe - qb > 2u * NT)
         {
           for (uint32_t q0 = qb + tid; q0 < qe; q0 += 8 * NT)
           {
