@@ -178,6 +178,8 @@ def main():
     ap.add_argument("--workload", default="210_560", choices=["210_560", "ts1"])
     ap.add_argument("--cells", type=int, default=0, help="cells (0: 100000 for 210_560, 10000 for ts1)")
     ap.add_argument("--form", default="autotune", choices=["autotune", "persistent", "plain", "cooperative"])
+    ap.add_argument("--coop-threads", type=int, default=0, help="threads per cell of the cooperative form (0: library)")
+    ap.add_argument("--handoff", type=int, default=-1, help="persistent form: hand-off step count (-1: library)")
     ap.add_argument("--compare-sample", type=int, default=8192,
                     help="cells on which the plain form and the chosen form are timed side by side (0: skip)")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel roofline of the single-phase kernels")
@@ -237,7 +239,9 @@ def main():
     else:
         form = {"persistent": _lib.SOLVE_PERSISTENT, "plain": _lib.SOLVE_PLAIN,
                 "cooperative": _lib.SOLVE_COOPERATIVE}[args.form]
-        form_label = args.form
+        state.set_cooperative_shape(args.coop_threads, -1)
+        state.set_handoff(args.handoff)
+        form_label = args.form + (f" ({args.coop_threads} threads per cell)" if args.coop_threads else "")
         choice = None
 
     ptr = state.pointers()
