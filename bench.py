@@ -135,6 +135,22 @@ def kernel_rooflines(mech, state, stream, n_cells, repeats=5):
     return out
 
 
+def measured_traffic(workload_name, n_cells, threads):
+    """HBM bytes per Solve launch from a committed rocprofv3 PMC summary
+    (profiles/*_hbm_traffic.md, FETCH_SIZE and WRITE_SIZE runs scaled by the same
+    counters on a 1 GiB copy) when one covers this workload, cell count and
+    cooperative shape; None otherwise."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_hbm_traffic.md"))):
+        text = open(path).read()
+        for block in text.split("```json")[1:]:
+            rec = json.loads(block.split("```")[0])
+            if (rec.get("workload"), rec.get("cells"), rec.get("kernel")) == \
+                    (workload_name, n_cells, f"k_coop<{threads}, 1>"):
+                return rec["traffic_bytes"], os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -236,12 +252,14 @@ def main():
         choice = autotune.autotune(mech, state, dt, params, verbose=True)
         form = autotune.apply(state, choice)
         form_label = f"autotuned: {choice['label']}"
+        coop_threads = choice["threads"] if form == _lib.SOLVE_COOPERATIVE else 0
     else:
         form = {"persistent": _lib.SOLVE_PERSISTENT, "plain": _lib.SOLVE_PLAIN,
                 "cooperative": _lib.SOLVE_COOPERATIVE}[args.form]
         state.set_cooperative_shape(args.coop_threads, -1)
         state.set_handoff(args.handoff)
         form_label = args.form + (f" ({args.coop_threads} threads per cell)" if args.coop_threads else "")
+        coop_threads = args.coop_threads if form == _lib.SOLVE_COOPERATIVE else 0
         choice = None
 
     ptr = state.pointers()
@@ -279,6 +297,7 @@ def main():
 
     bytes_per_step = algorithmic_bytes(mech.info, mech.n_species, mech.n_reactions, stats)
     achieved = bytes_per_step / (kernel_ms * 1e-3) / 1e9
+    traffic, traffic_source = measured_traffic(args.workload, n_cells, coop_threads) if coop_threads else (None, None)
     result = {
         "metric": "cell-solves/s, 210-species x 100k cells, batched Rosenbrock (FP64)"
         if args.workload == "210_560" else "cell-solves/s, TS1 x 10k cells (reference benchmark mechanism)",
@@ -297,7 +316,8 @@ def main():
         "config": {"workload": label, "form": form_label, "cells": n_cells, "tableau_stages": int(params.stages),
                    "time_step_s": dt},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "traffic_source": traffic_source,
                      "kernel": "the Solve launch (one persistent or cooperative kernel per step); algorithmic bytes "
                                f"per step {bytes_per_step} summed over the attempts the cells took"},
         "attempts_per_step": int(stats[:, 2].sum()),
