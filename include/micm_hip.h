@@ -277,7 +277,7 @@ int micm_hip_state_set_stream(micm_hip_state* state, void* stream);
 /* Step count at which the persistent form hands a cell to the cooperative form
  * (0: never; negative: the library's default).  Results do not depend on it. */
 int micm_hip_state_set_handoff(micm_hip_state* state, int64_t steps);
-/* Launch shape of the cooperative form: threads per cell (64, 128 or 256; 0: the
+/* Launch shape of the cooperative form: threads per cell (64, 128, 256, 512 or 1024; 0: the
  * library's choice) and which constant tables are staged in LDS (bit 0 the
  * substitution tables, bit 1 the LU tables; -1: the library's choice; tables that
  * do not fit next to the cell's data stay in global memory).  A tuning

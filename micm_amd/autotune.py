@@ -3,7 +3,7 @@
 The fastest way to run a mechanism depends on its size and on how unevenly its
 cells behave: the per-lane persistent form (one cell per lane, with or without
 handing slow cells to the cooperative form), or the cooperative form (one
-workgroup per cell) with 64, 128 or 256 threads and with its constant tables
+workgroup per cell) with 64, 128, 256 or 512 threads and with its constant tables
 staged in LDS or read from global memory.  `autotune` times every candidate on a
 strided sample of the state's own cells and returns the fastest.  Every
 candidate produces bit-identical results, so the choice only affects speed.
@@ -26,6 +26,7 @@ CANDIDATES = (
     ("cooperative64", _lib.SOLVE_COOPERATIVE, -1, 64, -1),
     ("cooperative128", _lib.SOLVE_COOPERATIVE, -1, 128, -1),
     ("cooperative256", _lib.SOLVE_COOPERATIVE, -1, 256, -1),
+    ("cooperative512", _lib.SOLVE_COOPERATIVE, -1, 512, -1),
     ("cooperative64-global-tables", _lib.SOLVE_COOPERATIVE, -1, 64, 0),
     ("cooperative128-global-tables", _lib.SOLVE_COOPERATIVE, -1, 128, 0),
 )

@@ -18,6 +18,7 @@
 #pragma once
 
 #include "device.hpp"
+#include "mechanism.hpp"
 #include "path.hpp"
 
 namespace micm_hip
@@ -103,10 +104,11 @@ namespace micm_hip
     return v;
   }
 
-  // v + sum_q coef[q] * src[idx[q]] for q in [b, e), added in order, in batches of eight
-  // (the lists end in eight padding entries)
+  // v + sum_q coef[q] * src[idx[q] * TC + cc] for q in [b, e), added in order, in batches
+  // of eight (the lists end in eight padding entries)
+  template<int TC = 1>
   __device__ __forceinline__ double gather_sum(double v, const int32_t* idx, const double* coef, int b, int e,
-                                               const double* src)
+                                               const double* src, int cc = 0)
   {
     for (int q = b; q < e; q += 8)
     {
@@ -120,7 +122,7 @@ namespace micm_hip
       }
 #pragma unroll
       for (int r = 0; r < 8; ++r)
-        sv[r] = src[ix[r]];
+        sv[r] = src[ix[r] * TC + cc];
 #pragma unroll
       for (int r = 0; r < 8; ++r)
         if (q + r < e)
@@ -292,6 +294,60 @@ namespace micm_hip
     return v;
   }
 
+  // coop_dot for one task; a task of up to S terms reads its term entries, then all its
+  // operands, at once
+  template<int TC = 1, int S = 4>
+  __device__ __forceinline__ double task_dot(double v, const uint32_t* terms, uint32_t b, uint32_t e, const double* A,
+                                             const double* X, int cc = 0)
+  {
+    if (e - b > static_cast<uint32_t>(S))
+      return coop_dot<TC>(v, terms, b, e, A, X, cc);
+    uint32_t ts[S];
+    double a[S], x[S];
+#pragma unroll
+    for (int r = 0; r < S; ++r)
+      ts[r] = terms[b + r];
+#pragma unroll
+    for (int r = 0; r < S; ++r)
+    {
+      a[r] = A[(ts[r] & 0xffffu) * TC + cc];
+      x[r] = X[(ts[r] >> 16) * TC + cc];
+    }
+#pragma unroll
+    for (int r = 0; r < S; ++r)
+      if (static_cast<uint32_t>(r) < e - b)
+        v -= a[r] * x[r];
+    return v;
+  }
+
+  // One-phase executor for schedules whose sums are split into short segments: each
+  // thread runs whole tasks (task_dot).  The LU schedule is segmented into runs of at
+  // least four products and the forward substitution's segments are short, so this
+  // beats the two-phase form there; the back substitution keeps long sums and runs
+  // two-phase.
+  template<int NT>
+  __device__ inline void coop_levels_direct(const DevLevels& lv, const uint2* tasks, const uint32_t* terms,
+                                            const double* A, const double* X, double* out)
+  {
+    const int tid = threadIdx.x;
+    for (int l = 0; l < lv.n_levels; ++l)
+    {
+      const int tb = lv.offsets[l], te = lv.offsets[l + 1];
+      for (int t = tb + tid; t < te; t += NT)
+      {
+        const uint2 task = tasks[t];
+        const uint32_t e = tasks[t + 1].y;
+        const uint32_t o = task.x & 0xffffu, d = task.x >> 16;
+        const double dv = d < kUnitDivisor ? A[d] : 1.0;
+        double v = task_dot(out[o], terms, task.y, e, A, X);
+        if (d < kUnitDivisor)
+          v = v / dv;
+        out[o] = v;
+      }
+      coop_sync<NT>();
+    }
+  }
+
   // Runs a level schedule: task t computes out[o] = (out[o] - dot) / A[diag] (no
   // division when diag is 0xffff), reading A and X.
   //
@@ -331,7 +387,7 @@ namespace micm_hip
             task0 = tasks[t0];
             e0 = tasks[t0 + 1].y;
             v0 = out[task0.x & 0xffffu];
-            if ((task0.x >> 16) != 0xffffu)
+            if ((task0.x >> 16) < kUnitDivisor)
               d0 = A[task0.x >> 16];
           }
         }
@@ -388,7 +444,7 @@ namespace micm_hip
             task = tasks[t];
             e = tasks[t + 1].y;
             v = out[task.x & 0xffffu];
-            if ((task.x >> 16) != 0xffffu)
+            if ((task.x >> 16) < kUnitDivisor)
               dv = A[task.x >> 16];
           }
           const double* p = P + (task.y - qb);
@@ -420,7 +476,7 @@ namespace micm_hip
           }
           for (; q < n; ++q)
             v -= p[q];
-          if ((task.x >> 16) != 0xffffu)
+          if ((task.x >> 16) < kUnitDivisor)
             v = v / dv;
           out[task.x & 0xffffu] = v;
         }
@@ -433,7 +489,7 @@ namespace micm_hip
           const uint32_t qe2 = tasks[t + 1].y;
           const uint32_t o = task.x & 0xffffu, d = task.x >> 16;
           double v = coop_dot(out[o], terms, task.y, qe2, A, X);
-          if (d != 0xffffu)
+          if (d < kUnitDivisor)
             v = v / A[d];
           out[o] = v;
         }
@@ -460,14 +516,14 @@ namespace micm_hip
     for (int e = tid; e < m.nnz_jac; e += NT)
       L.alu[c.jac_to_alu[e]] = L.jac[e];
     coop_sync<NT>();
-    coop_levels<NT>(c.lu, T.lu_tasks, T.lu_terms, L.alu, L.alu, L.alu, L.products, L.p_capacity);
+    coop_levels_direct<NT>(c.lu, T.lu_tasks, T.lu_terms, L.alu, L.alu, L.alu);
   }
 
   // x <- U^-1 L^-1 x with the factors in L.alu (unit lower diagonal).
   template<int NT>
   __device__ inline void coop_solve(const DevCoop& c, const CoopLds& L, const CoopTables& T, double* x)
   {
-    coop_levels<NT>(c.fwd, T.fwd_tasks, T.fwd_terms, L.alu, x, x, L.products, L.p_capacity);
+    coop_levels_direct<NT>(c.fwd, T.fwd_tasks, T.fwd_terms, L.alu, x, x);
     coop_levels<NT>(c.bwd, T.bwd_tasks, T.bwd_terms, L.alu, x, x, L.products, L.p_capacity);
   }
 

@@ -84,6 +84,7 @@ namespace micm_hip
     const int32_t* jel_offsets;
     const int32_t* jel_term;
     const double* jel_coef;
+    const uint64_t* jel_pack;   // term_pack of each Jacobian gather entry
   };
 
   // One lane's view of a grouped array: element v of this lane's cell.

@@ -181,6 +181,57 @@ namespace micm_hip
       return s;
     }
 
+    // Splits one element's ordered products into segments by the level at which
+    // their operands become readable: term q runs at level max(level of term q - 1,
+    // ready[q]).  Each segment is a task that continues the element's partial sum,
+    // kept in the element's own position between levels, so every subtraction happens
+    // in the same order and with the same rounding as in one long task, but the sum
+    // starts as soon as its first operands exist instead of when its last one does.
+    // The last segment runs no earlier than `final_min` and carries the divisor; an
+    // element without products gets one task that only divides.  Returns the level
+    // of the last segment.
+    // Shortest segment (except an element's last) of the LU and substitution sums: the
+    // LU's many short segments cost more in per-task overhead than they save in chain
+    // length, the substitutions' do not (measured on MI355X for 60/200, TS1 and
+    // 210/560).
+    constexpr size_t kLuSegmentMin = 4;
+    constexpr size_t kSolveSegmentMin = 1;
+
+    int32_t add_segments(std::vector<LevelTask>& out, uint32_t pos, uint32_t diag, const std::vector<uint32_t>& terms,
+                         const std::vector<int32_t>& ready, int32_t final_min, size_t min_len)
+    {
+      const size_t first = out.size();
+      int32_t lv = 0;
+      for (size_t q = 0; q < terms.size();)
+      {
+        lv = std::max(lv, ready[q]);
+        size_t e = q + 1;
+        while (e < terms.size() && (ready[e] <= lv || e - q < min_len))
+        {
+          lv = std::max(lv, ready[e]);
+          ++e;
+        }
+        LevelTask seg;
+        seg.out = pos;
+        seg.diag = kNoDivisor;
+        seg.level = lv;
+        seg.terms.assign(terms.begin() + static_cast<std::ptrdiff_t>(q), terms.begin() + static_cast<std::ptrdiff_t>(e));
+        out.push_back(std::move(seg));
+        q = e;
+      }
+      if (out.size() == first)
+      {
+        LevelTask seg;
+        seg.out = pos;
+        seg.level = 0;
+        out.push_back(std::move(seg));
+      }
+      LevelTask& last = out.back();
+      last.level = std::max(last.level, final_min);
+      last.diag = diag;
+      return last.level;
+    }
+
     void build_cooperative_schedule(MechanismTables& t)
     {
       const int32_t n = static_cast<int32_t>(t.n_species);
@@ -214,17 +265,17 @@ namespace micm_hip
           t.alu_upper[p] = alu(i, t.upper_col[p]);
       }
       // level schedules (16-bit positions; 0xffff marks "no divisor")
-      t.coop_levels_ok = t.alu_col.size() < 0xffff && n < 0xffff && t.n_reactions < 0xffff;
+      t.coop_levels_ok = t.alu_col.size() < kUnitDivisor && n < 0xffff && t.n_reactions < 0xffff;
       t.lv_lu = LevelSchedule{};
       t.lv_fwd = LevelSchedule{};
       t.lv_bwd = LevelSchedule{};
       if (!t.coop_levels_ok)
         return;
-      const uint32_t kNone = 0xffff;
       // LU: one task per Doolittle element that has products to subtract or a divisor;
       // the level of an ALU position is one more than the highest level it reads
       {
-        std::vector<int32_t> level(t.alu_col.size(), 0);
+        std::vector<int32_t> level(t.alu_col.size(), 0), ready;
+        std::vector<uint32_t> terms;
         std::vector<LevelTask> tasks;
         const size_t n_el = t.lu_out.size();
         for (size_t e = 0; e < n_el; ++e)
@@ -234,66 +285,59 @@ namespace micm_hip
           const int32_t tb = t.lu_term_offsets[e], te = t.lu_term_offsets[e + 1];
           if (tb == te && !lower)
             continue;  // U element equal to its Jacobian entry
-          LevelTask task;
-          task.out = static_cast<uint32_t>(out);
-          task.diag = lower ? static_cast<uint32_t>(t.alu_upper[t.lu_div[e]]) : kNone;
-          int32_t lv = lower ? level[task.diag] : 0;
+          const uint32_t diag = lower ? static_cast<uint32_t>(t.alu_upper[t.lu_div[e]]) : kNoDivisor;
+          terms.clear();
+          ready.clear();
           for (int32_t q = tb; q < te; ++q)
           {
             const int32_t l = t.alu_lower[t.lu_term_l[q]], u = t.alu_upper[t.lu_term_u[q]];
-            task.terms.push_back(static_cast<uint32_t>(l) | (static_cast<uint32_t>(u) << 16));
-            lv = std::max(lv, std::max(level[l], level[u]));
+            terms.push_back(static_cast<uint32_t>(l) | (static_cast<uint32_t>(u) << 16));
+            ready.push_back(std::max(level[l], level[u]));
           }
-          task.level = lv;
-          level[out] = lv + 1;
-          tasks.push_back(std::move(task));
+          level[out] = add_segments(tasks, static_cast<uint32_t>(out), diag, terms, ready, lower ? level[diag] : 0,
+                                    kLuSegmentMin) + 1;
         }
         t.lv_lu = pack_levels(tasks);
       }
       // forward substitution: row i subtracts L[i][j] x[j], j ascending (unit diagonal)
       {
-        std::vector<int32_t> level(n, 0);
+        std::vector<int32_t> level(n, 0), ready;
+        std::vector<uint32_t> terms;
         std::vector<LevelTask> tasks;
         for (int32_t i = 0; i < n; ++i)
         {
           const int32_t b = t.fwd_offsets[i], e = t.fwd_offsets[i + 1];
           if (b == e)
             continue;
-          LevelTask task;
-          task.out = static_cast<uint32_t>(i);
-          task.diag = kNone;
-          int32_t lv = 0;
+          terms.clear();
+          ready.clear();
           for (int32_t q = b; q < e; ++q)
           {
             const int32_t j = t.fwd_col[q];
-            task.terms.push_back(static_cast<uint32_t>(t.alu_lower[t.fwd_mat[q]]) | (static_cast<uint32_t>(j) << 16));
-            lv = std::max(lv, level[j]);
+            terms.push_back(static_cast<uint32_t>(t.alu_lower[t.fwd_mat[q]]) | (static_cast<uint32_t>(j) << 16));
+            ready.push_back(level[j]);
           }
-          task.level = lv;
-          level[i] = lv + 1;
-          tasks.push_back(std::move(task));
+          level[i] = add_segments(tasks, static_cast<uint32_t>(i), kUnitDivisor, terms, ready, 0, kSolveSegmentMin) + 1;
         }
         t.lv_fwd = pack_levels(tasks);
       }
       // back substitution: row i subtracts U[i][j] x[j], j ascending, then divides by U[i][i]
       {
-        std::vector<int32_t> level(n, 0);
+        std::vector<int32_t> level(n, 0), ready;
+        std::vector<uint32_t> terms;
         std::vector<LevelTask> tasks;
         for (int32_t i = n - 1; i >= 0; --i)
         {
-          LevelTask task;
-          task.out = static_cast<uint32_t>(i);
-          task.diag = static_cast<uint32_t>(t.alu_diag[i]);
-          int32_t lv = 0;
+          terms.clear();
+          ready.clear();
           for (int32_t q = t.bwd_offsets[i]; q < t.bwd_offsets[i + 1]; ++q)
           {
             const int32_t j = t.bwd_col[q];
-            task.terms.push_back(static_cast<uint32_t>(t.alu_upper[t.bwd_mat[q]]) | (static_cast<uint32_t>(j) << 16));
-            lv = std::max(lv, level[j]);
+            terms.push_back(static_cast<uint32_t>(t.alu_upper[t.bwd_mat[q]]) | (static_cast<uint32_t>(j) << 16));
+            ready.push_back(level[j]);
           }
-          task.level = lv;
-          level[i] = lv + 1;
-          tasks.push_back(std::move(task));
+          level[i] = add_segments(tasks, static_cast<uint32_t>(i), static_cast<uint32_t>(t.alu_diag[i]), terms, ready, 0,
+                                kSolveSegmentMin) + 1;
         }
         t.lv_bwd = pack_levels(tasks);
       }
@@ -369,6 +413,11 @@ namespace micm_hip
         v = (v & 0xffff000000000000ull) | ((v & 0xffffffffull) << 16) | static_cast<uint16_t>(t.term_process[term]);
         t.term_pack.push_back(v);
       }
+      // the operands of each Jacobian gather entry, so that a gather reads one word per entry
+      t.jel_pack.clear();
+      for (size_t q = 0; q + 8 < t.jel_term.size(); ++q)
+        t.jel_pack.push_back(t.term_pack[t.jel_term[q]]);
+      t.jel_pack.insert(t.jel_pack.end(), 8, 0);
     }
   }  // namespace
 

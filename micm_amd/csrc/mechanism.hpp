@@ -12,6 +12,10 @@
 
 namespace micm_hip
 {
+  // Divisor field of a level task: an ALU position, or one of these markers.
+  constexpr uint32_t kNoDivisor = 0xffff;    // no division
+  constexpr uint32_t kUnitDivisor = 0xfffe;  // forward substitution: divide by the row's L[i][i]
+
   // Independent tasks grouped by dependency level (see MechanismTables::lv_lu).
   struct LevelSchedule
   {
@@ -81,12 +85,14 @@ namespace micm_hip
     std::vector<int32_t> alu_row_ptr, alu_col, alu_diag;
     std::vector<int32_t> jac_to_alu;                       // Jacobian element -> ALU position
     std::vector<int32_t> alu_lower, alu_upper;             // L / U storage index -> ALU position
-    // Level schedules: every task reads only positions written at lower levels, and
+    // Level schedules: every task reads only final values written at lower levels and
     // subtracts its products in the reference's order, so the tasks of one level run
-    // in parallel without changing any rounding.
+    // in parallel without changing any rounding.  An element's sum may be split into
+    // segments at increasing levels (its partial sum kept in its own position); only
+    // the last segment divides.
     bool coop_levels_ok = false;                           // positions fit the 16-bit fields
     LevelSchedule lv_lu;   // out = ALU position, diag = U[i][i] position (L elements), terms (l | u << 16)
-    LevelSchedule lv_fwd;  // out = row, terms (ALU position | column << 16)
+    LevelSchedule lv_fwd;  // out = row, diag = kUnitDivisor on a row's last segment, terms (ALU position | column << 16)
     LevelSchedule lv_bwd;  // out = row, diag = U[i][i] position, terms (ALU position | column << 16)
     // packed operands: reaction -> reactant ids, Jacobian term -> (process, dependent ids);
     // three / two 16-bit ids and the count in bits 48-63 (0xffff: more, use the lists)
@@ -97,6 +103,7 @@ namespace micm_hip
     // -J as a per-element gather: J[e] += c * d[t] over (t, c) in the reference's order
     std::vector<int32_t> jel_offsets, jel_term;
     std::vector<double> jel_coef;
+    std::vector<uint64_t> jel_pack;                        // term_pack of each gather entry
 
     int64_t nnz_alu() const { return (int64_t)alu_col.size(); }
     int64_t nnz_jacobian() const { return (int64_t)jac_col.size(); }

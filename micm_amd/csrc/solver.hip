@@ -48,7 +48,7 @@ namespace
 
   constexpr int kBlock = 256;
 
-  inline int64_t n_groups_of(int64_t n_cells)
+  __host__ __device__ inline int64_t n_groups_of(int64_t n_cells)
   {
     return (n_cells + kGroup - 1) / kGroup;
   }
@@ -58,8 +58,20 @@ namespace
     return static_cast<unsigned>((n_groups_of(n_cells) * kGroup + kBlock - 1) / kBlock);
   }
 
+  // reaction chunks of the rate-constant kernel: at least 32 reactions each, and enough
+  // chunks for about 8192 waves in all, at most 16
+  inline dim3 rate_grid(int64_t n_cells, int n_reactions)
+  {
+    const int64_t waves = n_groups_of(n_cells);
+    int chunks = static_cast<int>(std::min<int64_t>(16, std::max<int64_t>(1, (8192 + waves - 1) / waves)));
+    chunks = std::max(1, std::min(chunks, (n_reactions + 31) / 32));
+    return dim3(grid_for(n_cells), static_cast<unsigned>(chunks));
+  }
+
   // ------------------------------------------------------------------ kernel-level entry points
 
+  // one cell per lane; the reactions are split into gridDim.y contiguous chunks so that a
+  // launch over few cells still has enough waves to hide the transcendental latency
   __global__ void __launch_bounds__(kBlock) k_rate_constants(DevMech m, int64_t n_cells, const double* __restrict__ cond,
                                                              const double* __restrict__ custom, double* __restrict__ k)
   {
@@ -72,7 +84,9 @@ namespace
     Lane cu = lane_view(const_cast<double*>(custom), g, m.n_custom, l);
     Lane kk = lane_view(k, g, m.n_reactions, l);
     const double T = cd[0], P = cd[1], rho = cd[2];
-    for (int r = 0; r < m.n_reactions; ++r)
+    const int per = (m.n_reactions + gridDim.y - 1) / gridDim.y;
+    const int rb = blockIdx.y * per, re = min(m.n_reactions, rb + per);
+    for (int r = rb; r < re; ++r)
       kk[r] = rate_constant(m, r, T, P, rho, cu);
   }
 
@@ -100,97 +114,242 @@ namespace
                       lane_view(const_cast<double*>(y), g, m.n_species, l), lane_view(jac, g, m.nnz_jac, l));
   }
 
-  // ---- tiled single-phase kernels: one workgroup per 16 cells of a group, the tile's
-  // y and k staged in LDS with 128-byte coalesced loads, outputs gathered per
-  // species / Jacobian element in the reference's accumulation order and written
-  // once.  Each HBM byte of y, k and the output moves once.
-  constexpr int kTileCells = 16;
+  // ---- tiled single-phase kernels: one workgroup per TC cells of a group, the tile's
+  // y and k staged in LDS, outputs gathered per species / Jacobian element in the
+  // reference's accumulation order and written once.  Each HBM byte of y, k and the
+  // output moves once.
+  //
+  // Workgroups are dealt to the eight XCDs round-robin (workgroup b runs on XCD b % 8),
+  // and each XCD has its own L2.  The tile of workgroup b is chosen so that the 64 / TC
+  // tiles of one group run on one XCD one after another and share the group's
+  // 128-byte lines in that L2.
+  constexpr int kXcds = 8;
   constexpr int kTileThreads = 256;
 
-  __host__ __device__ inline size_t tile_lds_bytes(int n_species, int n_reactions)
+  template<int TC>
+  __host__ __device__ constexpr int tiles_per_group()
   {
-    return static_cast<size_t>(n_species + n_reactions) * kTileCells * sizeof(double);
+    return kGroup / TC;
   }
 
-  __device__ inline void load_tile(double* dst, const double* src, size_t g, int n_values, int q0)
+  template<int TC>
+  __device__ inline bool tile_of_block(int64_t n_cells, size_t& g, int& q0)
   {
-    for (int i = threadIdx.x; i < n_values * kTileCells; i += kTileThreads)
-    {
-      const int v = i / kTileCells, cc = i % kTileCells;
-      dst[i] = src[(g * n_values + v) * kGroup + q0 + cc];
-    }
+    constexpr int S = tiles_per_group<TC>();
+    const unsigned b = blockIdx.x, x = b % kXcds, j = b / kXcds;
+    const int64_t grp = static_cast<int64_t>(j / S) * kXcds + x;
+    g = static_cast<size_t>(grp);
+    q0 = static_cast<int>(j % S) * TC;
+    return grp < n_groups_of(n_cells);
   }
 
-  // f += f(y): rates of the tile's cells into LDS (in place over k), then per-species gathers
-  __global__ void __launch_bounds__(kTileThreads) k_add_forcing_tiled(DevMech m, DevCoop c, int64_t n_cells,
-                                                                      const double* __restrict__ k,
-                                                                      const double* __restrict__ y, double* __restrict__ f)
+  // dst[v * TC + cc] = src[(g * n_values + v) * 64 + q0 + cc], eight loads in flight per thread
+  template<int TC, int NT>
+  __device__ inline void load_tile(double* dst, const double* __restrict__ src, size_t g, int n_values, int q0)
   {
-    extern __shared__ double tile[];
-    const int N = m.n_species, R = m.n_reactions;
-    const size_t g = blockIdx.x / (kGroup / kTileCells);
-    const int q0 = (blockIdx.x % (kGroup / kTileCells)) * kTileCells;
-    double* yt = tile;
-    double* rt = tile + static_cast<size_t>(N) * kTileCells;
-    load_tile(yt, y, g, N, q0);
-    load_tile(rt, k, g, R, q0);
-    __syncthreads();
-    for (int i = threadIdx.x; i < R * kTileCells; i += kTileThreads)
+    const int total = n_values * TC;
+    const double* base = src + g * static_cast<size_t>(n_values) * kGroup + q0;
+    for (int i0 = threadIdx.x; i0 < total; i0 += 8 * NT)
     {
-      const int r = i / kTileCells, cc = i % kTileCells;
-      double rate = rt[i];
-      for (int a = m.reactant_offsets[r]; a < m.reactant_offsets[r + 1]; ++a)
-        rate *= yt[m.reactant_ids[a] * kTileCells + cc];
-      rt[i] = rate;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < N * kTileCells; i += kTileThreads)
-    {
-      const int sp = i / kTileCells, cc = i % kTileCells;
-      const int64_t cell = static_cast<int64_t>(g) * kGroup + q0 + cc;
-      if (cell >= n_cells)
-        continue;
-      double* out = f + (g * N + sp) * kGroup + q0 + cc;
-      double v = *out;
-      for (int q = c.spec_offsets[sp]; q < c.spec_offsets[sp + 1]; ++q)
-        v += c.spec_coef[q] * rt[c.spec_rxn[q] * kTileCells + cc];
-      *out = v;
-    }
-  }
-
-  // jac -= J(y): per Jacobian element, the products of its terms in the reference's order
-  __global__ void __launch_bounds__(kTileThreads) k_subtract_jacobian_tiled(DevMech m, DevCoop c, int64_t n_cells,
-                                                                            const double* __restrict__ k,
-                                                                            const double* __restrict__ y,
-                                                                            double* __restrict__ jac)
-  {
-    extern __shared__ double tile[];
-    const int N = m.n_species, R = m.n_reactions;
-    const size_t g = blockIdx.x / (kGroup / kTileCells);
-    const int q0 = (blockIdx.x % (kGroup / kTileCells)) * kTileCells;
-    double* yt = tile;
-    double* kt = tile + static_cast<size_t>(N) * kTileCells;
-    load_tile(yt, y, g, N, q0);
-    load_tile(kt, k, g, R, q0);
-    __syncthreads();
-    const int nnz = m.nnz_jac;
-    for (int i = threadIdx.x; i < nnz * kTileCells; i += kTileThreads)
-    {
-      const int e = i / kTileCells, cc = i % kTileCells;
-      const int64_t cell = static_cast<int64_t>(g) * kGroup + q0 + cc;
-      if (cell >= n_cells)
-        continue;
-      double* out = jac + (g * nnz + e) * kGroup + q0 + cc;
-      double v = *out;
-      for (int q = c.jel_offsets[e]; q < c.jel_offsets[e + 1]; ++q)
+      double v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
       {
-        const int t = c.jel_term[q];
-        double d = kt[m.term_process[t] * kTileCells + cc];
-        for (int a = m.term_dep_offsets[t]; a < m.term_dep_offsets[t + 1]; ++a)
-          d *= yt[m.term_dep_ids[a] * kTileCells + cc];
-        v += c.jel_coef[q] * d;
+        const int i = i0 + j * NT < total ? i0 + j * NT : i0;
+        v[j] = base[(i / TC) * kGroup + i % TC];
       }
-      *out = v;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (i0 + j * NT < total)
+          dst[i0 + j * NT] = v[j];
+    }
+  }
+
+  // rates of the tile's cells, in place over their rate constants, four reactions per
+  // thread at a time with packed reactant ids
+  template<int TC, int NT>
+  __device__ inline void tile_rates(const DevMech& m, const DevCoop& c, const double* yt, double* rt)
+  {
+    const int nr = m.n_reactions * TC;
+    const int cc = threadIdx.x % TC;  // NT is a multiple of TC
+    for (int i0 = threadIdx.x; i0 < nr; i0 += 4 * NT)
+    {
+      uint64_t pk[4];
+      double kk[4], y0[4], y1[4], y2[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+      {
+        const int i = i0 + j * NT < nr ? i0 + j * NT : i0;
+        pk[j] = c.rxn_pack[i / TC];
+        kk[j] = rt[i];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+      {
+        y0[j] = yt[(pk[j] & 0xffffu) * TC + cc];
+        y1[j] = yt[((pk[j] >> 16) & 0xffffu) * TC + cc];
+        y2[j] = yt[((pk[j] >> 32) & 0xffffu) * TC + cc];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+      {
+        const int i = i0 + j * NT;
+        if (i < nr)
+        {
+          double rate = kk[j];
+          if ((pk[j] >> 48) == 0xffffu)
+          {
+            const int r = i / TC;
+            for (int a = m.reactant_offsets[r]; a < m.reactant_offsets[r + 1]; ++a)
+              rate *= yt[m.reactant_ids[a] * TC + cc];
+          }
+          else
+          {
+            rate = packed_product(rate, pk[j], y0[j], y1[j], y2[j]);
+          }
+          rt[i] = rate;
+        }
+      }
+    }
+  }
+
+  // f += f(y): rates of the tile's cells into LDS (in place over k), then per-species
+  // gathers; the old value of the next item is loaded while the current one is summed
+  template<int TC, int NT>
+  __global__ void __launch_bounds__(NT) k_add_forcing_tiled(DevMech m, DevCoop c, int64_t n_cells,
+                                                            const double* __restrict__ k,
+                                                            const double* __restrict__ y, double* __restrict__ f)
+  {
+    extern __shared__ double tile[];
+    size_t g;
+    int q0;
+    if (!tile_of_block<TC>(n_cells, g, q0))
+      return;
+    const int N = m.n_species, R = m.n_reactions;
+    double* yt = tile;
+    double* rt = tile + static_cast<size_t>(N) * TC;
+    load_tile<TC, NT>(yt, y, g, N, q0);
+    load_tile<TC, NT>(rt, k, g, R, q0);
+    __syncthreads();
+    tile_rates<TC, NT>(m, c, yt, rt);
+    __syncthreads();
+    const int cc = threadIdx.x % TC;
+    if (static_cast<int64_t>(g) * kGroup + q0 + cc >= n_cells)
+      return;
+    const int ns = N * TC;
+    double* out = f + g * static_cast<size_t>(N) * kGroup + q0 + cc;
+    int i = threadIdx.x;
+    double v_next = i < ns ? out[(i / TC) * kGroup] : 0.0;
+    for (; i < ns; i += NT)
+    {
+      const int sp = i / TC;
+      double v = v_next;
+      if (i + NT < ns)
+        v_next = out[((i + NT) / TC) * kGroup];
+      v = gather_sum<TC>(v, c.spec_rxn, c.spec_coef, c.spec_offsets[sp], c.spec_offsets[sp + 1], rt, cc);
+      out[sp * kGroup] = v;
+    }
+  }
+
+  // jac -= J(y): per Jacobian element, its terms' products k * y... in the reference's
+  // order, four entries at a time with packed operands
+  template<int TC, int NT>
+  __global__ void __launch_bounds__(NT) k_subtract_jacobian_tiled(DevMech m, DevCoop c, int64_t n_cells,
+                                                                  const double* __restrict__ k,
+                                                                  const double* __restrict__ y,
+                                                                  double* __restrict__ jac)
+  {
+    extern __shared__ double tile[];
+    size_t g;
+    int q0;
+    if (!tile_of_block<TC>(n_cells, g, q0))
+      return;
+    const int N = m.n_species, R = m.n_reactions;
+    double* yt = tile;
+    double* kt = tile + static_cast<size_t>(N) * TC;
+    load_tile<TC, NT>(yt, y, g, N, q0);
+    load_tile<TC, NT>(kt, k, g, R, q0);
+    __syncthreads();
+    const int cc = threadIdx.x % TC;
+    if (static_cast<int64_t>(g) * kGroup + q0 + cc >= n_cells)
+      return;
+    const int nnz = m.nnz_jac;
+    const int ne = nnz * TC;
+    double* out = jac + g * static_cast<size_t>(nnz) * kGroup + q0 + cc;
+    int i = threadIdx.x;
+    double v_next = i < ne ? out[(i / TC) * kGroup] : 0.0;
+    for (; i < ne; i += NT)
+    {
+      const int e = i / TC;
+      double v = v_next;
+      if (i + NT < ne)
+        v_next = out[((i + NT) / TC) * kGroup];
+      const int qe = c.jel_offsets[e + 1];
+      for (int q = c.jel_offsets[e]; q < qe; q += 4)
+      {
+        uint64_t pk[4];
+        double cf[4], kk[4], y0[4], y1[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+          pk[r] = c.jel_pack[q + r];
+          cf[r] = c.jel_coef[q + r];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+          kk[r] = kt[(pk[r] & 0xffffu) * TC + cc];
+          y0[r] = yt[((pk[r] >> 16) & 0xffffu) * TC + cc];
+          y1[r] = yt[((pk[r] >> 32) & 0xffffu) * TC + cc];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+          if (q + r < qe)
+          {
+            double d;
+            if ((pk[r] >> 48) == 0xffffu)
+            {
+              const int t = c.jel_term[q + r];
+              d = kt[m.term_process[t] * TC + cc];
+              for (int a = m.term_dep_offsets[t]; a < m.term_dep_offsets[t + 1]; ++a)
+                d *= yt[m.term_dep_ids[a] * TC + cc];
+            }
+            else
+            {
+              d = packed_product(kk[r], pk[r], y0[r], y1[r], 0.0);
+            }
+            v += cf[r] * d;
+          }
+        }
+      }
+      out[e * kGroup] = v;
+    }
+  }
+
+  // dst[map[v] * TC + cc] = src[(g * n_values + v) * 64 + q0 + cc] for map[v] >= 0,
+  // eight loads in flight per thread
+  template<int TC, int NT>
+  __device__ inline void load_tile_scatter(double* dst, const int32_t* map, const double* __restrict__ src, size_t g,
+                                           int n_values, int q0)
+  {
+    const int total = n_values * TC;
+    const double* base = src + g * static_cast<size_t>(n_values) * kGroup + q0;
+    for (int i0 = threadIdx.x; i0 < total; i0 += 8 * NT)
+    {
+      double v[8];
+      int to[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+      {
+        const int i = i0 + j * NT < total ? i0 + j * NT : i0;
+        v[j] = base[(i / TC) * kGroup + i % TC];
+        to[j] = map[i / TC];
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (i0 + j * NT < total && to[j] >= 0)
+          dst[to[j] * TC + (i0 + j * NT) % TC] = v[j];
     }
   }
 
@@ -204,17 +363,15 @@ namespace
                                                                        double* __restrict__ lower, double* __restrict__ upper)
   {
     extern __shared__ double alu[];
-    const size_t g = blockIdx.x / (kGroup / TC);
-    const int q0 = (blockIdx.x % (kGroup / TC)) * TC;
+    size_t g;
+    int q0;
+    if (!tile_of_block<TC>(n_cells, g, q0))
+      return;
     const int tid = threadIdx.x;
     for (int i = tid; i < c.nnz_alu * TC; i += kTileThreads)
       alu[i] = 0.0;
     __syncthreads();
-    for (int i = tid; i < m.nnz_jac * TC; i += kTileThreads)
-    {
-      const int e = i / TC, cc = i % TC;
-      alu[c.jac_to_alu[e] * TC + cc] = jac[(g * m.nnz_jac + e) * kGroup + q0 + cc];
-    }
+    load_tile_scatter<TC, kTileThreads>(alu, c.jac_to_alu, jac, g, m.nnz_jac, q0);
     __syncthreads();
     const DevLevels& lv = c.lu;
     for (int l = 0; l < lv.n_levels; ++l)
@@ -226,8 +383,8 @@ namespace
         const uint2 task = lv.tasks[t];
         const uint32_t qe = lv.tasks[t + 1].y;
         const uint32_t o = task.x & 0xffffu, dg = task.x >> 16;
-        double v = coop_dot<TC>(alu[o * TC + cc], lv.terms, task.y, qe, alu, alu, cc);
-        if (dg != 0xffffu)
+        double v = task_dot<TC>(alu[o * TC + cc], lv.terms, task.y, qe, alu, alu, cc);
+        if (dg < kUnitDivisor)
           v = v / alu[dg * TC + cc];
         alu[o * TC + cc] = v;
       }
@@ -261,27 +418,18 @@ namespace
     double* alu = lds_solve;
     double* xt = alu + static_cast<size_t>(c.nnz_alu) * TC;
     double* ld = xt + static_cast<size_t>(N) * TC;
-    const size_t g = blockIdx.x / (kGroup / TC);
-    const int q0 = (blockIdx.x % (kGroup / TC)) * TC;
+    size_t g;
+    int q0;
+    if (!tile_of_block<TC>(n_cells, g, q0))
+      return;
     const int tid = threadIdx.x;
-    for (int i = tid; i < m.nnz_lower * TC; i += kTileThreads)
-    {
-      const int p = i / TC, cc = i % TC;
-      const double v = lower[(g * m.nnz_lower + p) * kGroup + q0 + cc];
-      const int a = c.alu_lower[p];
-      if (a >= 0)
-        alu[a * TC + cc] = v;
-    }
+    load_tile_scatter<TC, kTileThreads>(alu, c.alu_lower, lower, g, m.nnz_lower, q0);
+    load_tile_scatter<TC, kTileThreads>(alu, c.alu_upper, upper, g, m.nnz_upper, q0);
+    load_tile<TC, kTileThreads>(xt, x, g, N, q0);
     for (int i = tid; i < N * TC; i += kTileThreads)
     {
       const int r = i / TC, cc = i % TC;
       ld[i] = lower[(g * m.nnz_lower + m.lower_diag[r]) * kGroup + q0 + cc];
-      xt[i] = x[(g * N + r) * kGroup + q0 + cc];
-    }
-    for (int i = tid; i < m.nnz_upper * TC; i += kTileThreads)
-    {
-      const int p = i / TC, cc = i % TC;
-      alu[c.alu_upper[p] * TC + cc] = upper[(g * m.nnz_upper + p) * kGroup + q0 + cc];
     }
     __syncthreads();
     // rows without off-diagonal L entries only divide; they are inputs of every level
@@ -304,11 +452,9 @@ namespace
           const uint2 task = lv.tasks[t];
           const uint32_t qe = lv.tasks[t + 1].y;
           const uint32_t o = task.x & 0xffffu, dg = task.x >> 16;
-          double v = coop_dot<TC>(xt[o * TC + cc], lv.terms, task.y, qe, alu, xt, cc);
-          if (pass == 0)
-            v = v / ld[o * TC + cc];
-          else
-            v = v / alu[dg * TC + cc];
+          double v = task_dot<TC>(xt[o * TC + cc], lv.terms, task.y, qe, alu, xt, cc);
+          if (pass == 0 ? dg == kUnitDivisor : dg < kUnitDivisor)
+            v = v / (pass == 0 ? ld[o * TC + cc] : alu[dg * TC + cc]);
           xt[o * TC + cc] = v;
         }
         __syncthreads();
@@ -1297,7 +1443,7 @@ namespace
              padded_bytes(t.rxn_pack) + padded_bytes(t.term_pack) +
              padded_bytes(t.spec_offsets) + padded_bytes(t.spec_rxn) +
              padded_bytes(t.spec_coef) + padded_bytes(t.jel_offsets) + padded_bytes(t.jel_term) +
-             padded_bytes(t.jel_coef) + 256;
+             padded_bytes(t.jel_coef) + padded_bytes(t.jel_pack) + 256;
     HIP_TRY(hipMalloc(&m->blob, bytes));
     std::vector<char> staging;
     size_t off = 0;
@@ -1355,6 +1501,7 @@ namespace
     upload(m->blob, off, t.jel_offsets, c.jel_offsets, staging);
     upload(m->blob, off, t.jel_term, c.jel_term, staging);
     upload(m->blob, off, t.jel_coef, c.jel_coef, staging);
+    upload(m->blob, off, t.jel_pack, c.jel_pack, staging);
     if (off > bytes)
       return fail(MICM_HIP_ERR_DEVICE, "internal table sizing error");
     HIP_TRY(hipMemcpy(m->blob, staging.data(), off, hipMemcpyHostToDevice));
@@ -1395,6 +1542,41 @@ namespace
     return 0;
   }
 
+  // Cells per workgroup of the tiled forcing / Jacobian kernels: the largest power of two
+  // from 16 down to 2 whose tile of y and k leaves room for three workgroups per CU
+  // (two for the smallest tile); 0 when the gather tables are unavailable or nothing fits.
+  constexpr int kGatherThreads = 512;
+
+  size_t gather_tile_bytes(const micm_hip_mechanism* m, int tc)
+  {
+    return (static_cast<size_t>(m->dev.n_species) + static_cast<size_t>(m->dev.n_reactions)) * tc * sizeof(double);
+  }
+
+  int gather_tile_cells(const micm_hip_mechanism* m)
+  {
+    if (!m->t.coop_levels_ok)
+      return 0;
+    const size_t limit = lds_limit();
+    for (int tc = 16; tc >= 4; tc /= 2)
+      if (3 * gather_tile_bytes(m, tc) <= limit)
+        return tc;
+    return 2 * gather_tile_bytes(m, 2) <= limit ? 2 : 0;
+  }
+
+  inline unsigned tiled_blocks(int tc, int64_t n_cells)
+  {
+    return static_cast<unsigned>((n_groups_of(n_cells) + kXcds - 1) / kXcds * kXcds * (kGroup / tc));
+  }
+
+#define LAUNCH_GATHER(kernel, tc, lds, stream, ...)                                                                  \
+  do                                                                                                                \
+  {                                                                                                                 \
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel<tc, kGatherThreads>),                          \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));                \
+    hipLaunchKernelGGL((kernel<tc, kGatherThreads>), dim3(tiled_blocks(tc, n_cells)), dim3(kGatherThreads), lds,    \
+                       stream, __VA_ARGS__);                                                                        \
+  } while (0)
+
 #define LAUNCH_TILED(kernel, blocks, lds, stream, ...)                                                               \
   do                                                                                                                \
   {                                                                                                                 \
@@ -1412,9 +1594,8 @@ namespace
     if (!m->t.coop_levels_ok)
       return plan;
     constexpr size_t kLdsPerCu = 160 * 1024;
-    // the product buffer always covers the substitution levels; LU levels too where
-    // that costs no resident cell per CU (larger LU levels run task by task)
-    const int p_solve = std::max(c.fwd.max_level_terms, c.bwd.max_level_terms);
+    // the product buffer of the two-phase back substitution
+    const int p_solve = c.bwd.max_level_terms;
     const size_t base = coop_lds_doubles(c.nnz_alu, m->dev.nnz_jac, m->dev.n_species, stages, m->dev.n_reactions,
                                          c.n_terms, p_solve) *
                         sizeof(double);
@@ -1441,21 +1622,13 @@ namespace
       stage &= ~2;
     if (base + ((stage & 1) ? solve : 0) > limit)
       stage &= ~1;
-    size_t bytes = base + ((stage & 1) ? solve : 0) + ((stage & 2) ? lu : 0);
-    int p = p_solve;
-    if (c.lu.max_level_terms > p)
-    {
-      const size_t per_cu = kLdsPerCu / bytes;
-      const size_t room = std::min(limit, per_cu > 0 ? kLdsPerCu / per_cu : limit);
-      const size_t extra = room > bytes ? (room - bytes) / sizeof(double) : 0;
-      const int grow = static_cast<int>(std::min<size_t>(extra, static_cast<size_t>(c.lu.max_level_terms - p)));
-      p += grow;
-      bytes += static_cast<size_t>(grow) * sizeof(double);
-    }
-    plan.lds = bytes;
+    plan.lds = base + ((stage & 1) ? solve : 0) + ((stage & 2) ? lu : 0);
     plan.stage = stage;
-    plan.p_capacity = p;
-    plan.threads = (threads_request == 128 || threads_request == 256) ? threads_request : kWave;
+    plan.p_capacity = p_solve;
+    plan.threads = (threads_request == 128 || threads_request == 256 || threads_request == 512 ||
+                    threads_request == 1024)
+                       ? threads_request
+                       : kWave;
     return plan;
   }
 
@@ -1509,6 +1682,8 @@ namespace
     {
       case 128: return launch_coop_nt<128>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
       case 256: return launch_coop_nt<256>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
+      case 512: return launch_coop_nt<512>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
+      case 1024: return launch_coop_nt<1024>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
       default: return launch_coop_nt<kWave>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
     }
   }
@@ -1730,7 +1905,8 @@ extern "C"
     KERNEL_PRELUDE();
     if (!cond || !k || (m->dev.n_custom > 0 && !custom))
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null device pointer");
-    hipLaunchKernelGGL(k_rate_constants, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, cond, custom, k);
+    hipLaunchKernelGGL(k_rate_constants, rate_grid(n_cells, m->dev.n_reactions), dim3(kBlock), 0, st, m->dev, n_cells,
+                       cond, custom, k);
     HIP_TRY(hipGetLastError());
     return MICM_HIP_OK;
   }
@@ -1741,13 +1917,17 @@ extern "C"
     KERNEL_PRELUDE();
     if (!k || !y || !f)
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null device pointer");
-    const size_t lds = tile_lds_bytes(m->dev.n_species, m->dev.n_reactions);
-    if (lds <= lds_limit())
+    const int tc = gather_tile_cells(m);
+    if (tc > 0)
     {
-      HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_add_forcing_tiled),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-      hipLaunchKernelGGL(k_add_forcing_tiled, dim3(static_cast<unsigned>(n_groups_of(n_cells) * (kGroup / kTileCells))),
-                         dim3(kTileThreads), lds, st, m->dev, m->coop, n_cells, k, y, f);
+      const size_t lds = gather_tile_bytes(m, tc);
+      switch (tc)
+      {
+        case 16: LAUNCH_GATHER(k_add_forcing_tiled, 16, lds, st, m->dev, m->coop, n_cells, k, y, f); break;
+        case 8: LAUNCH_GATHER(k_add_forcing_tiled, 8, lds, st, m->dev, m->coop, n_cells, k, y, f); break;
+        case 4: LAUNCH_GATHER(k_add_forcing_tiled, 4, lds, st, m->dev, m->coop, n_cells, k, y, f); break;
+        default: LAUNCH_GATHER(k_add_forcing_tiled, 2, lds, st, m->dev, m->coop, n_cells, k, y, f); break;
+      }
     }
     else
     {
@@ -1763,14 +1943,17 @@ extern "C"
     KERNEL_PRELUDE();
     if (!k || !y || !jac)
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null device pointer");
-    const size_t lds = tile_lds_bytes(m->dev.n_species, m->dev.n_reactions);
-    if (lds <= lds_limit())
+    const int tc = gather_tile_cells(m);
+    if (tc > 0)
     {
-      HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_subtract_jacobian_tiled),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-      hipLaunchKernelGGL(k_subtract_jacobian_tiled,
-                         dim3(static_cast<unsigned>(n_groups_of(n_cells) * (kGroup / kTileCells))), dim3(kTileThreads),
-                         lds, st, m->dev, m->coop, n_cells, k, y, jac);
+      const size_t lds = gather_tile_bytes(m, tc);
+      switch (tc)
+      {
+        case 16: LAUNCH_GATHER(k_subtract_jacobian_tiled, 16, lds, st, m->dev, m->coop, n_cells, k, y, jac); break;
+        case 8: LAUNCH_GATHER(k_subtract_jacobian_tiled, 8, lds, st, m->dev, m->coop, n_cells, k, y, jac); break;
+        case 4: LAUNCH_GATHER(k_subtract_jacobian_tiled, 4, lds, st, m->dev, m->coop, n_cells, k, y, jac); break;
+        default: LAUNCH_GATHER(k_subtract_jacobian_tiled, 2, lds, st, m->dev, m->coop, n_cells, k, y, jac); break;
+      }
     }
     else
     {
@@ -1800,7 +1983,7 @@ extern "C"
     if (tc > 0)
     {
       const size_t lds = static_cast<size_t>(m->coop.nnz_alu) * tc * sizeof(double);
-      const unsigned blocks = static_cast<unsigned>(n_groups_of(n_cells) * (kGroup / tc));
+      const unsigned blocks = tiled_blocks(tc, n_cells);
       switch (tc)
       {
         case 16: LAUNCH_TILED(k_lu_decompose_tiled<16>, blocks, lds, st, m->dev, m->coop, n_cells, jac, lower, upper); break;
@@ -1829,7 +2012,7 @@ extern "C"
     {
       const size_t lds = (static_cast<size_t>(m->coop.nnz_alu) + 2 * static_cast<size_t>(m->dev.n_species)) * tc *
                          sizeof(double);
-      const unsigned blocks = static_cast<unsigned>(n_groups_of(n_cells) * (kGroup / tc));
+      const unsigned blocks = tiled_blocks(tc, n_cells);
       switch (tc)
       {
         case 16: LAUNCH_TILED(k_lu_solve_tiled<16>, blocks, lds, st, m->dev, m->coop, n_cells, lower, upper, x); break;
@@ -2033,8 +2216,8 @@ extern "C"
   {
     if (!s)
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null state");
-    if (threads != 0 && threads != 64 && threads != 128 && threads != 256)
-      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "threads per cell must be 0, 64, 128 or 256");
+    if (threads != 0 && threads != 64 && threads != 128 && threads != 256 && threads != 512 && threads != 1024)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "threads per cell must be 0, 64, 128, 256, 512 or 1024");
     if (stage < -1 || stage > 3)
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "stage must be -1 .. 3");
     s->coop_threads = threads;
@@ -2058,8 +2241,8 @@ extern "C"
       return rc;
     if (!s || s->mech != m)
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "state does not belong to this mechanism");
-    hipLaunchKernelGGL(k_rate_constants, dim3(grid_for(s->n_cells)), dim3(kBlock), 0, s->stream, m->dev, s->n_cells,
-                       s->cond, s->custom, s->k);
+    hipLaunchKernelGGL(k_rate_constants, rate_grid(s->n_cells, m->dev.n_reactions), dim3(kBlock), 0, s->stream, m->dev,
+                       s->n_cells, s->cond, s->custom, s->k);
     HIP_TRY(hipGetLastError());
     return MICM_HIP_OK;
   }
@@ -2083,7 +2266,8 @@ extern "C"
     {
       if (update_rates)
       {
-        hipLaunchKernelGGL(k_rate_constants, dim3(grid), dim3(kBlock), 0, st, d, s->n_cells, s->cond, s->custom, s->k);
+        hipLaunchKernelGGL(k_rate_constants, rate_grid(s->n_cells, d.n_reactions), dim3(kBlock), 0, st, d, s->n_cells,
+                           s->cond, s->custom, s->k);
         HIP_TRY(hipGetLastError());
       }
       const bool lockstep = form == MICM_HIP_SOLVE_LOCKSTEP;

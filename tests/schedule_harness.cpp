@@ -30,7 +30,7 @@ namespace
         double v = out_is_a ? A0[o] : X0[o];
         for (uint32_t q = s.tasks[2 * t + 1]; q < s.tasks[2 * t + 3]; ++q)
           v -= A[s.terms[q] & 0xffffu] * X[s.terms[q] >> 16];
-        if (d != 0xffffu)
+        if (d < micm_hip::kUnitDivisor)
           v = v / A[d];
         (out_is_a ? A_state : X_state)[o] = v;
       }
@@ -119,7 +119,7 @@ extern "C"
       for (int32_t q = s.offsets[l]; q < s.offsets[l + 1]; ++q)
       {
         const uint32_t o = s.tasks[2 * q] & 0xffffu, d = s.tasks[2 * q] >> 16;
-        double start = 0.0;
+        double start = finish[o];  // an earlier segment of the same sum
         for (uint32_t k = s.tasks[2 * q + 1]; k < s.tasks[2 * q + 3]; ++k)
         {
           if (which == 0)
@@ -127,7 +127,7 @@ extern "C"
           else
             start = std::max(start, finish[s.terms[k] >> 16]);
         }
-        if (which == 0 && d != 0xffffu)
+        if (which == 0 && d < micm_hip::kUnitDivisor)
           start = std::max(start, finish[d]);
         finish[o] = start + base + (s.tasks[2 * q + 3] - s.tasks[2 * q + 1]);
         best = std::max(best, finish[o]);
