@@ -90,6 +90,51 @@ namespace micm_hip
     return L;
   }
 
+  // Compact layout: only what the gathers read stays in LDS -- the in-place factor, y
+  // and the stage vectors (the current stage's forcing input and its solve vector) --
+  // plus one scratch area shared in time by the reaction rates, the Jacobian partial
+  // products (when they do not fit over the stage vectors), the back substitution's
+  // products and the error terms.  The Jacobian is assembled straight into the factor
+  // at every attempt (its diagonal, which accumulates the attempts' alpha shifts, and
+  // f(y) are kept in registers, one species per thread) and the rate constants are
+  // read from a per-workgroup copy in global memory.  About 20 % smaller than the
+  // full layout, which fits a second cell per CU for mechanisms of 210/560's size.
+  __host__ __device__ inline size_t coop_compact_scratch_doubles(int n, int n_reactions, int stages, int n_terms,
+                                                                 int p_capacity)
+  {
+    int scratch = n_reactions > n ? n_reactions : n;
+    if (p_capacity > scratch)
+      scratch = p_capacity;
+    if (n_terms > stages * n && n_terms > scratch)
+      scratch = n_terms;
+    return static_cast<size_t>(scratch);
+  }
+  __host__ __device__ inline size_t coop_compact_lds_doubles(int nnz_alu, int n, int stages, int n_reactions,
+                                                             int n_terms, int p_capacity)
+  {
+    return static_cast<size_t>(nnz_alu) + static_cast<size_t>(1 + stages) * n +
+           coop_compact_scratch_doubles(n, n_reactions, stages, n_terms, p_capacity);
+  }
+  __device__ inline CoopLds coop_layout_compact(double* lds, const DevMech& m, const DevCoop& c, int stages,
+                                                int p_capacity, double* k_global)
+  {
+    CoopLds L;
+    const int n = m.n_species;
+    L.alu = lds;
+    L.y = L.alu + c.nnz_alu;
+    L.K = L.y + n;
+    L.scratch = L.K + static_cast<size_t>(stages) * n;
+    L.jac = nullptr;
+    L.ynew = nullptr;
+    L.f0 = nullptr;
+    L.yerr = nullptr;
+    L.k = k_global;
+    L.jterm = c.n_terms <= stages * n ? L.K : L.scratch;
+    L.products = L.scratch;
+    L.p_capacity = p_capacity;
+    return L;
+  }
+
   // product of up to three packed operands, in list order, onto v; 0xffff counts
   // fall back to the index lists
   __device__ __forceinline__ double packed_product(double v, uint64_t pk, double y0, double y1, double y2)
@@ -182,8 +227,8 @@ namespace micm_hip
     coop_sync<NT>();
   }
 
-  // L.jac = -J(y)
-  template<int NT>
+  // L.jac = -J(y); with TO_ALU the elements go straight to their factor positions
+  template<int NT, bool TO_ALU = false>
   __device__ inline void coop_jacobian(const DevMech& m, const DevCoop& c, const CoopLds& L)
   {
     const int tid = threadIdx.x;
@@ -227,7 +272,13 @@ namespace micm_hip
     }
     coop_sync<NT>();
     for (int e = tid; e < m.nnz_jac; e += NT)
-      L.jac[e] = gather_sum(0.0, c.jel_term, c.jel_coef, c.jel_offsets[e], c.jel_offsets[e + 1], L.jterm);
+    {
+      const double v = gather_sum(0.0, c.jel_term, c.jel_coef, c.jel_offsets[e], c.jel_offsets[e + 1], L.jterm);
+      if constexpr (TO_ALU)
+        L.alu[c.jac_to_alu[e]] = v;
+      else
+        L.jac[e] = v;
+    }
     coop_sync<NT>();
   }
 
@@ -320,31 +371,162 @@ namespace micm_hip
     return v;
   }
 
-  // One-phase executor for schedules whose sums are split into short segments: each
-  // thread runs whole tasks (task_dot).  The LU schedule is segmented into runs of at
-  // least four products and the forward substitution's segments are short, so this
-  // beats the two-phase form there; the back substitution keeps long sums and runs
-  // two-phase.
+  // Workgroup barrier that waits for this wave's LDS operations only: global loads
+  // issued before it stay in flight across it.
   template<int NT>
+  __device__ __forceinline__ void lds_barrier()
+  {
+    if constexpr (NT == kWave)
+    {
+      coop_sync<kWave>();
+    }
+    else
+    {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_s_waitcnt(0xc07f);  // vmcnt(63) expcnt(7) lgkmcnt(0)
+      __builtin_amdgcn_s_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
+
+  // One-phase executor for schedules whose sums are split into short segments: each
+  // thread runs whole tasks, a task of up to four terms reading all its operands at
+  // once (longer tasks go through coop_dot).  The LU schedule is segmented into runs
+  // of at least four products and the forward substitution's segments are short, so
+  // this beats the two-phase form there; the back substitution keeps long sums and
+  // runs two-phase.
+  //
+  // With GT (tables in global memory) the reads of a thread's first task are
+  // pipelined over levels: while level l reads its operands, the same round of reads
+  // fetches the term entries of the thread's task at level l + 1 and the task record
+  // at level l + 2 (both constant), and the level barrier waits for LDS operations
+  // only, so the table reads may take up to two levels to arrive.
+  template<int NT, bool GT = false>
   __device__ inline void coop_levels_direct(const DevLevels& lv, const uint2* tasks, const uint32_t* terms,
                                             const double* A, const double* X, double* out)
   {
     const int tid = threadIdx.x;
-    for (int l = 0; l < lv.n_levels; ++l)
+    const int nl = lv.n_levels;
+    if constexpr (!GT)
     {
-      const int tb = lv.offsets[l], te = lv.offsets[l + 1];
-      for (int t = tb + tid; t < te; t += NT)
+      // tables in LDS: reading them level by level is as fast (measured) and keeps
+      // fewer registers live
+      for (int l = 0; l < nl; ++l)
       {
-        const uint2 task = tasks[t];
-        const uint32_t e = tasks[t + 1].y;
+        const int tb = lv.offsets[l], te = lv.offsets[l + 1];
+        for (int t = tb + tid; t < te; t += NT)
+        {
+          const uint2 task = tasks[t];
+          const uint32_t e = tasks[t + 1].y;
+          const uint32_t o = task.x & 0xffffu, d = task.x >> 16;
+          const double dv = d < kUnitDivisor ? A[d] : 1.0;
+          double v = task_dot(out[o], terms, task.y, e, A, X);
+          if (d < kUnitDivisor)
+            v = v / dv;
+          out[o] = v;
+        }
+        coop_sync<NT>();
+      }
+      return;
+    }
+    if (nl == 0)
+      return;
+    auto bound = [&](int i) { return lv.offsets[i < nl ? i : nl]; };
+    int b0 = bound(0), b1 = bound(1), b2 = bound(2), b3 = bound(3);
+    // the thread's first task of the current level (record and terms) and of the next
+    // level (record)
+    bool c_has = b0 + tid < b1, n_has = b1 + tid < b2;
+    uint2 c_task = make_uint2(0u, 0u), n_task = make_uint2(0u, 0u);
+    uint32_t c_e = 0, n_e = 0, c_ts[4] = {0u, 0u, 0u, 0u};
+    if (c_has)
+    {
+      c_task = tasks[b0 + tid];
+      c_e = tasks[b0 + tid + 1].y;
+    }
+    if (n_has)
+    {
+      n_task = tasks[b1 + tid];
+      n_e = tasks[b1 + tid + 1].y;
+    }
+    if (c_has)
+    {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        c_ts[r] = terms[c_task.y + r];
+    }
+    for (int l = 0; l < nl; ++l)
+    {
+      const int b4 = bound(l + 4);
+      // constant reads for the next two levels, issued with this level's operand reads
+      const bool nn_has = b2 + tid < b3;
+      uint2 nn_task = make_uint2(0u, 0u);
+      uint32_t nn_e = 0, n_ts[4] = {0u, 0u, 0u, 0u};
+      if (nn_has)
+      {
+        nn_task = tasks[b2 + tid];
+        nn_e = tasks[b2 + tid + 1].y;
+      }
+      if (n_has)
+      {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          n_ts[r] = terms[n_task.y + r];
+      }
+      for (int t = b0 + tid; t < b1; t += NT)
+      {
+        uint2 task = c_task;
+        uint32_t e = c_e, ts[4] = {c_ts[0], c_ts[1], c_ts[2], c_ts[3]};
+        if (t != b0 + tid)
+        {
+          task = tasks[t];
+          e = tasks[t + 1].y;
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            ts[r] = terms[task.y + r];
+        }
         const uint32_t o = task.x & 0xffffu, d = task.x >> 16;
+        const uint32_t n = e - task.y;
+        double v = out[o];
         const double dv = d < kUnitDivisor ? A[d] : 1.0;
-        double v = task_dot(out[o], terms, task.y, e, A, X);
+        if (n <= 4u)
+        {
+          double a[4], x[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+          {
+            a[r] = A[ts[r] & 0xffffu];
+            x[r] = X[ts[r] >> 16];
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (static_cast<uint32_t>(r) < n)
+              v -= a[r] * x[r];
+        }
+        else
+        {
+          v = coop_dot(v, terms, task.y, e, A, X);
+        }
         if (d < kUnitDivisor)
           v = v / dv;
         out[o] = v;
       }
-      coop_sync<NT>();
+      c_has = n_has;
+      c_task = n_task;
+      c_e = n_e;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        c_ts[r] = n_ts[r];
+      n_has = nn_has;
+      n_task = nn_task;
+      n_e = nn_e;
+      b0 = b1;
+      b1 = b2;
+      b2 = b3;
+      b3 = b4;
+      if constexpr (GT)
+        lds_barrier<NT>();
+      else
+        coop_sync<NT>();
     }
   }
 
@@ -506,7 +688,7 @@ namespace micm_hip
   };
 
   // In-place LU of L.jac into L.alu.
-  template<int NT>
+  template<int NT, bool GT = false>
   __device__ inline void coop_decompose(const DevMech& m, const DevCoop& c, const CoopLds& L, const CoopTables& T)
   {
     const int tid = threadIdx.x;
@@ -516,14 +698,14 @@ namespace micm_hip
     for (int e = tid; e < m.nnz_jac; e += NT)
       L.alu[c.jac_to_alu[e]] = L.jac[e];
     coop_sync<NT>();
-    coop_levels_direct<NT>(c.lu, T.lu_tasks, T.lu_terms, L.alu, L.alu, L.alu);
+    coop_levels_direct<NT, GT>(c.lu, T.lu_tasks, T.lu_terms, L.alu, L.alu, L.alu);
   }
 
   // x <- U^-1 L^-1 x with the factors in L.alu (unit lower diagonal).
-  template<int NT>
+  template<int NT, bool GT = false>
   __device__ inline void coop_solve(const DevCoop& c, const CoopLds& L, const CoopTables& T, double* x)
   {
-    coop_levels_direct<NT>(c.fwd, T.fwd_tasks, T.fwd_terms, L.alu, x, x);
+    coop_levels_direct<NT, GT>(c.fwd, T.fwd_tasks, T.fwd_terms, L.alu, x, x);
     coop_levels<NT>(c.bwd, T.bwd_tasks, T.bwd_terms, L.alu, x, x, L.products, L.p_capacity);
   }
 

@@ -964,22 +964,30 @@ namespace
 
   // STAGE: bit 0 the substitution tables, bit 1 the LU tables, copied into LDS once per
   // workgroup after the per-cell data.  A compile-time choice, so that every table
-  // access is a plain LDS or global access.
-  template<int NT, int STAGE>
+  // access is a plain LDS or global access.  COMPACT (with STAGE 0 and at least one
+  // thread per species): the compact layout of coop_layout_compact, with k_wg holding
+  // gridDim.x rate-constant rows of n_reactions doubles.
+  template<int NT, int STAGE, bool COMPACT = false>
   __global__ void __launch_bounds__(NT) k_coop(DevMech m, DevCoop c, DevParams p, int p_capacity, int64_t n_fixed,
                                                const int32_t* __restrict__ list, const int* list_count, int* next,
                                                const CellControl* resume, double* __restrict__ y_state,
                                                double* __restrict__ k_state, const double* __restrict__ atol,
                                                int32_t* status_out, int64_t* stats_out, double* final_time_out,
                                                const double* __restrict__ cond_state,
-                                               const double* __restrict__ custom_state)
+                                               const double* __restrict__ custom_state, double* __restrict__ k_wg)
   {
+    static_assert(!COMPACT || STAGE == 0, "the compact layout reads the tables from global memory");
     extern __shared__ double lds[];
     __shared__ int s_idx;
     __shared__ double s_error;
     const int lane = threadIdx.x;
     const int n = m.n_species;
-    const CoopLds L = coop_layout(lds, m, c, p.stages, p_capacity);
+    const CoopLds L = COMPACT ? coop_layout_compact(lds, m, c, p.stages, p_capacity,
+                                                    k_wg + static_cast<size_t>(blockIdx.x) * m.n_reactions)
+                              : coop_layout(lds, m, c, p.stages, p_capacity);
+    // compact layout: this thread's species' f(y), accumulated Jacobian diagonal and
+    // new value (threadIdx.x < n)
+    double f0_r = 0.0, jdiag_r = 0.0, ynew_r = 0.0;
     const int64_t count = list ? static_cast<int64_t>(*list_count) : n_fixed;
     CoopTables T{ c.lu.tasks, c.fwd.tasks, c.bwd.tasks, c.lu.terms, c.fwd.terms, c.bwd.terms };
     {
@@ -1053,22 +1061,55 @@ namespace
       coop_sync<NT>();
       while (true)
       {
+        bool fresh = false;
         if (cc.flags & kNewStep)
         {
           if (!begin_step(p, cc))
             break;
-          coop_forcing<NT>(m, c, L, L.y, L.f0);
+          if constexpr (COMPACT)
+          {
+            coop_forcing<NT>(m, c, L, L.y, L.K);
+            if (lane < n)
+              f0_r = L.K[lane];
+          }
+          else
+          {
+            coop_forcing<NT>(m, c, L, L.y, L.f0);
+            coop_jacobian<NT>(m, c, L);
+          }
           cc.stats[MICM_HIP_STAT_FUNCTION_CALLS] += 1;
-          coop_jacobian<NT>(m, c, L);
           cc.stats[MICM_HIP_STAT_JACOBIAN_UPDATES] += 1;
           cc.last_alpha = 0.0;
           cc.flags &= ~kNewStep;
+          fresh = true;
         }
         const double alpha = next_alpha(p, cc);
-        for (int i = lane; i < n; i += NT)
-          L.jac[m.jac_diag[i]] += alpha * 1.0;
-        coop_sync<NT>();
-        coop_decompose<NT>(m, c, L, T);
+        if constexpr (COMPACT)
+        {
+          // -J(y) again (y and k are those of the step), its diagonal replaced by the
+          // accumulated one: the same values as keeping -J and adding alpha in place
+          for (int q = lane; q < c.nnz_alu; q += NT)
+            L.alu[q] = 0.0;
+          coop_sync<NT>();
+          coop_jacobian<NT, true>(m, c, L);
+          if (lane < n)
+          {
+            const int d = c.alu_diag[lane];
+            if (fresh)
+              jdiag_r = L.alu[d];
+            jdiag_r += alpha * 1.0;
+            L.alu[d] = jdiag_r;
+          }
+          coop_sync<NT>();
+          coop_levels_direct<NT, true>(c.lu, T.lu_tasks, T.lu_terms, L.alu, L.alu, L.alu);
+        }
+        else
+        {
+          for (int i = lane; i < n; i += NT)
+            L.jac[m.jac_diag[i]] += alpha * 1.0;
+          coop_sync<NT>();
+          coop_decompose<NT, (STAGE & 2) == 0>(m, c, L, T);
+        }
         cc.stats[MICM_HIP_STAT_DECOMPOSITIONS] += 1;
         for (int s = 0; s < p.stages; ++s)
         {
@@ -1077,19 +1118,22 @@ namespace
           if (s == 0)
           {
             for (int i = lane; i < n; i += NT)
-              Ks[i] = L.f0[i];
+              Ks[i] = COMPACT ? f0_r : L.f0[i];
           }
           else if (p.new_f[s])
           {
+            // compact: the forcing's input goes into its output vector (the forcing
+            // reads its input before it writes)
+            double* yin = COMPACT ? Ks : L.ynew;
             for (int i = lane; i < n; i += NT)
             {
               double v = L.y[i];
               for (int j = 0; j < s; ++j)
                 v += p.a[sc + j] * L.K[static_cast<size_t>(j) * n + i];
-              L.ynew[i] = v;
+              yin[i] = v;
             }
             coop_sync<NT>();
-            coop_forcing<NT>(m, c, L, L.ynew, Ks);
+            coop_forcing<NT>(m, c, L, yin, Ks);
             cc.stats[MICM_HIP_STAT_FUNCTION_CALLS] += 1;
           }
           coop_sync<NT>();
@@ -1110,7 +1154,7 @@ namespace
             }
           }
           coop_sync<NT>();
-          coop_solve<NT>(c, L, T, Ks);
+          coop_solve<NT, COMPACT || (STAGE & 1) == 0>(c, L, T, Ks);
           cc.stats[MICM_HIP_STAT_SOLVES] += 1;
         }
         for (int i = lane; i < n; i += NT)
@@ -1120,8 +1164,15 @@ namespace
             v += p.m[s] * L.K[static_cast<size_t>(s) * n + i];
           for (int s = 0; s < p.stages; ++s)
             e += p.e[s] * L.K[static_cast<size_t>(s) * n + i];
-          L.ynew[i] = v;
-          L.yerr[i] = e;
+          if constexpr (COMPACT)
+          {
+            ynew_r = v;
+          }
+          else
+          {
+            L.ynew[i] = v;
+            L.yerr[i] = e;
+          }
           const double a = fabs(L.y[i]), b = fabs(v);
           const double q = e / (atol[i] + p.rtol * (a > b ? a : b));
           L.scratch[i] = q * q;
@@ -1138,7 +1189,7 @@ namespace
         const double error = s_error;
         if (control_update(p, cc, error))
           for (int i = lane; i < n; i += NT)
-            L.y[i] = L.ynew[i];
+            L.y[i] = COMPACT ? ynew_r : L.ynew[i];
         coop_sync<NT>();
         if (cc.status != MICM_HIP_STATE_RUNNING)
           break;
@@ -1199,6 +1250,8 @@ struct micm_hip_state
   int32_t* handoff_list = nullptr;
   int* handoff_count = nullptr;
   int* coop_next = nullptr;
+  double* coop_k = nullptr;  // rate-constant rows of the compact cooperative layout
+  int64_t coop_k_rows = 0;
   // cooperative launch shape (-1 / 0: chosen from the mechanism)
   int coop_stage = -1;
   int coop_threads = 0;
@@ -1515,8 +1568,40 @@ namespace
     size_t lds = 0;
     int stage = 0;
     int threads = kWave;
-    int p_capacity = 0;  // doubles of the per-level product buffer
+    int p_capacity = 0;     // doubles of the per-level product buffer
+    bool compact = false;   // coop_layout_compact
   };
+
+  // Workgroups of the cooperative kernel that fit one CU (registers and LDS).
+  template<int NT>
+  int coop_occupancy_nt(int stage, bool compact, size_t lds)
+  {
+    int per_cu = 0;
+    hipError_t e = hipSuccess;
+    if (compact)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coop<NT, 0, true>, NT, lds);
+    else if (stage == 1)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coop<NT, 1>, NT, lds);
+    else if (stage == 2)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coop<NT, 2>, NT, lds);
+    else if (stage == 3)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coop<NT, 3>, NT, lds);
+    else
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coop<NT, 0>, NT, lds);
+    return e == hipSuccess ? per_cu : 0;
+  }
+
+  int coop_occupancy(int threads, int stage, bool compact, size_t lds)
+  {
+    switch (threads)
+    {
+      case 128: return coop_occupancy_nt<128>(stage, compact, lds);
+      case 256: return coop_occupancy_nt<256>(stage, compact, lds);
+      case 512: return coop_occupancy_nt<512>(stage, compact, lds);
+      case 1024: return coop_occupancy_nt<1024>(stage, compact, lds);
+      default: return coop_occupancy_nt<kWave>(stage, compact, lds);
+    }
+  }
 
   size_t lds_limit()
   {
@@ -1585,9 +1670,44 @@ namespace
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kTileThreads), lds, stream, __VA_ARGS__);                        \
   } while (0)
 
+  CoopPlan plan_coop_full(const micm_hip_mechanism* m, int stages, int stage_request, int threads_request);
+
   // stage_request < 0: stage the substitution tables when they fit and the LU tables
-  // when that costs no resident cell per CU; threads_request 0: one wavefront.
+  // when that costs no resident cell per CU, or take the compact layout when that fits
+  // more cells per CU; 4: the compact layout; threads_request 0: one wavefront.
   CoopPlan plan_coop(const micm_hip_mechanism* m, int stages, int stage_request = -1, int threads_request = 0)
+  {
+    CoopPlan plan = plan_coop_full(m, stages, stage_request == 4 ? -1 : stage_request, threads_request);
+    const DevCoop& c = m->coop;
+    if (!m->t.coop_levels_ok || (stage_request >= 0 && stage_request != 4))
+      return plan;
+    // the compact layout keeps one species per thread in registers and publishes the
+    // rate constants through global memory, so it needs a workgroup barrier
+    const int threads = (threads_request == 128 || threads_request == 256 || threads_request == 512 ||
+                         threads_request == 1024)
+                            ? threads_request
+                            : kWave;
+    if (threads <= kWave || threads < m->dev.n_species)
+      return plan;
+    const size_t compact = coop_compact_lds_doubles(c.nnz_alu, m->dev.n_species, stages, m->dev.n_reactions,
+                                                    c.n_terms, c.bwd.max_level_terms) *
+                           sizeof(double);
+    if (compact > lds_limit())
+      return plan;
+    const int occ_compact = coop_occupancy(threads, 0, true, compact);
+    const int occ_full = plan.lds > 0 ? coop_occupancy(threads, plan.stage, false, plan.lds) : 0;
+    if (stage_request == 4 ? occ_compact > 0 : occ_compact > occ_full)
+    {
+      plan.lds = compact;
+      plan.stage = 0;
+      plan.threads = threads;
+      plan.p_capacity = c.bwd.max_level_terms;
+      plan.compact = true;
+    }
+    return plan;
+  }
+
+  CoopPlan plan_coop_full(const micm_hip_mechanism* m, int stages, int stage_request, int threads_request)
   {
     CoopPlan plan;
     const DevCoop& c = m->coop;
@@ -1637,24 +1757,35 @@ namespace
     return plan_coop(m, stages).lds;
   }
 
-  template<int NT, int STAGE>
+  // Workgroups of one cooperative launch: as many as fit every CU.
+  int coop_blocks(const CoopPlan& plan, int64_t& blocks)
+  {
+    int dev = 0, cus = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    blocks = static_cast<int64_t>(cus) * std::max(coop_occupancy(plan.threads, plan.stage, plan.compact, plan.lds), 1);
+    return MICM_HIP_OK;
+  }
+
+  template<int NT, int STAGE, bool COMPACT = false>
   int launch_coop_shape(micm_hip_state* s, const DevParams& p, const CoopPlan& plan, const int32_t* list,
                         const int* count, int64_t n_fixed, const CellControl* resume, bool update_rates, hipStream_t st)
   {
     const micm_hip_mechanism* m = s->mech;
-    const void* fn = reinterpret_cast<const void*>(k_coop<NT, STAGE>);
+    const void* fn = reinterpret_cast<const void*>(k_coop<NT, STAGE, COMPACT>);
     HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(plan.lds)));
-    int dev = 0, cus = 0, per_cu = 0;
-    HIP_TRY(hipGetDevice(&dev));
-    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coop<NT, STAGE>, NT, plan.lds));
-    int64_t blocks = static_cast<int64_t>(cus) * std::max(per_cu, 1);
+    int64_t blocks = 0;
+    if (int rc = coop_blocks(plan, blocks))
+      return rc;
+    if (COMPACT && s->coop_k_rows < blocks)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "cooperative scratch not prepared");
     if (!list)
       blocks = std::max<int64_t>(1, std::min(blocks, n_fixed));
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, s->coop_next);
-    hipLaunchKernelGGL((k_coop<NT, STAGE>), dim3(static_cast<unsigned>(blocks)), dim3(NT), plan.lds, st, m->dev, m->coop,
-                       p, plan.p_capacity, n_fixed, list, count, s->coop_next, resume, s->y, s->k, s->atol, s->status, s->stats,
-                       s->final_time, update_rates ? s->cond : nullptr, update_rates ? s->custom : nullptr);
+    hipLaunchKernelGGL((k_coop<NT, STAGE, COMPACT>), dim3(static_cast<unsigned>(blocks)), dim3(NT), plan.lds, st, m->dev,
+                       m->coop, p, plan.p_capacity, n_fixed, list, count, s->coop_next, resume, s->y, s->k, s->atol,
+                       s->status, s->stats, s->final_time, update_rates ? s->cond : nullptr,
+                       update_rates ? s->custom : nullptr, s->coop_k);
     HIP_TRY(hipGetLastError());
     return MICM_HIP_OK;
   }
@@ -1663,6 +1794,8 @@ namespace
   int launch_coop_nt(micm_hip_state* s, const DevParams& p, const CoopPlan& plan, const int32_t* list,
                      const int* count, int64_t n_fixed, const CellControl* resume, bool update_rates, hipStream_t st)
   {
+    if (plan.compact)
+      return launch_coop_shape<NT, 0, true>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
     switch (plan.stage)
     {
       case 1: return launch_coop_shape<NT, 1>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
@@ -1706,6 +1839,26 @@ namespace
 
   int prepare_solve(micm_hip_state* s, int32_t form, int stages)
   {
+    if (form == MICM_HIP_SOLVE_COOPERATIVE || form == MICM_HIP_SOLVE_PERSISTENT)
+    {
+      const CoopPlan plan = plan_coop(s->mech, stages, s->coop_stage, s->coop_threads);
+      if (plan.compact)
+      {
+        int64_t rows = 0;
+        if (int rc = coop_blocks(plan, rows))
+          return rc;
+        if (s->coop_k_rows < rows)
+        {
+          if (s->coop_k)
+            HIP_TRY(hipFree(s->coop_k));
+          s->coop_k = nullptr;
+          s->coop_k_rows = 0;
+          HIP_TRY(hipMalloc(&s->coop_k, sizeof(double) * static_cast<size_t>(rows) *
+                                            static_cast<size_t>(std::max(s->mech->dev.n_reactions, 1))));
+          s->coop_k_rows = rows;
+        }
+      }
+    }
     if (form != MICM_HIP_SOLVE_PERSISTENT)
       return MICM_HIP_OK;
     int64_t blocks = 0;
@@ -2096,7 +2249,7 @@ extern "C"
     if (!s)
       return;
     for (double* p : { s->y, s->cond, s->custom, s->k, s->jac, s->lower, s->upper, s->atol, s->f0, s->ynew, s->yerr, s->K,
-                       s->final_time, s->lockstep_error, s->slot_blob })
+                       s->final_time, s->lockstep_error, s->slot_blob, s->coop_k })
       if (p)
         (void)hipFree(p);
     if (s->ctl)
@@ -2218,8 +2371,8 @@ extern "C"
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null state");
     if (threads != 0 && threads != 64 && threads != 128 && threads != 256 && threads != 512 && threads != 1024)
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "threads per cell must be 0, 64, 128, 256, 512 or 1024");
-    if (stage < -1 || stage > 3)
-      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "stage must be -1 .. 3");
+    if (stage < -1 || stage > 4)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "stage must be -1 .. 4");
     s->coop_threads = threads;
     s->coop_stage = stage;
     return MICM_HIP_OK;

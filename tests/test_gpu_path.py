@@ -222,7 +222,7 @@ def test_per_cell_forms_agree_bitwise(name):
     runs = [_solve(mech, cells, order, tableau, dt, _lib.SOLVE_PERSISTENT, handoff=0),
             _solve(mech, cells, order, tableau, dt, _lib.SOLVE_PERSISTENT, handoff=3)]
     if mech.cooperative_supported(tableau):
-        for threads, stage in ((64, -1), (128, 0), (256, 3), (64, 1), (512, -1), (1024, 2)):
+        for threads, stage in ((64, -1), (128, 0), (256, 3), (64, 1), (512, -1), (1024, 2), (128, 4), (256, 4)):
             runs.append(_solve(mech, cells, order, tableau, dt, _lib.SOLVE_COOPERATIVE, coop_shape=(threads, stage)))
     for r in runs:
         assert bits_equal(r[0], ref[0])
