@@ -146,7 +146,7 @@ def measured_traffic(workload_name, n_cells, threads):
         for block in text.split("```json")[1:]:
             rec = json.loads(block.split("```")[0])
             if (rec.get("workload"), rec.get("cells"), rec.get("kernel")) == \
-                    (workload_name, n_cells, f"k_coop<{threads}, 1>"):
+                    (workload_name, n_cells, f"k_coop<{threads}>"):
                 return rec["traffic_bytes"], os.path.relpath(path, ROOT)
     return None, None
 
