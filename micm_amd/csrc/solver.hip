@@ -967,8 +967,17 @@ namespace
   // access is a plain LDS or global access.  COMPACT (with STAGE 0 and at least one
   // thread per species): the compact layout of coop_layout_compact, with k_wg holding
   // gridDim.x rate-constant rows of n_reactions doubles.
+  //
+  // With 256 threads per cell the kernel is compiled for three waves per SIMD (at most
+  // 168 VGPRs), which lets a third cell onto a CU where LDS allows (TS1: 1.26x).
+  template<int NT>
+  constexpr int coop_min_waves_per_eu()
+  {
+    return NT == 256 ? 3 : 1;
+  }
+
   template<int NT, int STAGE, bool COMPACT = false>
-  __global__ void __launch_bounds__(NT) k_coop(DevMech m, DevCoop c, DevParams p, int p_capacity, int64_t n_fixed,
+  __global__ void __launch_bounds__(NT, coop_min_waves_per_eu<NT>()) k_coop(DevMech m, DevCoop c, DevParams p, int p_capacity, int64_t n_fixed,
                                                const int32_t* __restrict__ list, const int* list_count, int* next,
                                                const CellControl* resume, double* __restrict__ y_state,
                                                double* __restrict__ k_state, const double* __restrict__ atol,
