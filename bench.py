@@ -199,7 +199,9 @@ def main():
     ap.add_argument("--compare-sample", type=int, default=8192,
                     help="cells on which the plain form and the chosen form are timed side by side (0: skip)")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel roofline of the single-phase kernels")
-    ap.add_argument("--cpu-sample", type=int, default=512, help="cells in the CPU-baseline sample (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=-1,
+                    help="cells in the CPU-baseline sample (0: skip; -1: about 15 core-seconds of CPU work, "
+                         "sized from the single-core leg)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU-baseline threads (0: all host cores)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device-to-device copy of 1 GiB (counter calibration for profiles)")
@@ -354,16 +356,20 @@ def main():
     if not args.no_kernels:
         result["kernels"] = kernel_rooflines(mech, state, stream, n_cells)
         log("single-phase kernels timed")
-    if args.cpu_sample > 0:
+    if args.cpu_sample != 0:
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
-        sample = min(args.cpu_sample, n_cells)
+        # one core first; its rate sizes the all-core sample when none is given
+        sample1 = min(n_cells, 16 if args.cpu_sample < 0 else max(8, args.cpu_sample // threads))
+        kind1, cps1, el1 = cpu_baseline(desc, cells, order, dt, sample1, 1, tableau)
+        sample = min(n_cells, args.cpu_sample if args.cpu_sample > 0 else max(256, int(15.0 * cps1)))
         kind, cps, el = cpu_baseline(desc, cells, order, dt, sample, threads, tableau)
+        log(f"CPU baseline: {sample} cells on {threads} cores in {el:.1f} s")
         result["cpu_baseline"] = {"value": round(cps, 2), "unit": "cell-solves/s", "cores": threads, "kind": kind,
                                   "cpu": cpu_model(),
                                   "sample": f"first {sample} cells of the same workload, one 1-cell State per "
-                                            f"cell, {el:.1f} s wall"}
-        kind1, cps1, el1 = cpu_baseline(desc, cells, order, dt, max(8, sample // threads), 1, tableau)
-        result["cpu_baseline"]["single_core_value"] = round(cps1, 2)
+                                            f"cell, {el:.1f} s wall on {threads} cores",
+                                  "single_core_value": round(cps1, 2),
+                                  "single_core_sample": f"first {sample1} cells, {el1:.1f} s"}
     print(json.dumps(result), flush=True)
 
 
