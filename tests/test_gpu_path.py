@@ -220,7 +220,9 @@ def test_per_cell_forms_agree_bitwise(name):
     print(f"forms agree: {name}", flush=True)
     ref = _solve(mech, cells, order, tableau, dt, _lib.SOLVE_PLAIN)
     runs = [_solve(mech, cells, order, tableau, dt, _lib.SOLVE_PERSISTENT, handoff=0),
-            _solve(mech, cells, order, tableau, dt, _lib.SOLVE_PERSISTENT, handoff=3)]
+            _solve(mech, cells, order, tableau, dt, _lib.SOLVE_PERSISTENT, handoff=3),
+            # hand-off into the compact cooperative layout
+            _solve(mech, cells, order, tableau, dt, _lib.SOLVE_PERSISTENT, handoff=3, coop_shape=(256, 4))]
     if mech.cooperative_supported(tableau):
         for threads, stage in ((64, -1), (128, 0), (256, 3), (64, 1), (512, -1), (1024, 2), (128, 4), (256, 4)):
             runs.append(_solve(mech, cells, order, tableau, dt, _lib.SOLVE_COOPERATIVE, coop_shape=(threads, stage)))
@@ -230,18 +232,22 @@ def test_per_cell_forms_agree_bitwise(name):
         assert bits_equal(r[3], ref[3])
 
 
-@pytest.mark.parametrize("form", [_lib.SOLVE_PLAIN, _lib.SOLVE_PERSISTENT, _lib.SOLVE_COOPERATIVE])
-def test_fused_rate_constants_match_separate_update(form):
+@pytest.mark.parametrize("form, coop_shape", [(_lib.SOLVE_PLAIN, (0, -1)), (_lib.SOLVE_PERSISTENT, (0, -1)),
+                                              (_lib.SOLVE_COOPERATIVE, (0, -1)), (_lib.SOLVE_PERSISTENT, (128, 4)),
+                                              (_lib.SOLVE_COOPERATIVE, (128, 4))])
+def test_fused_rate_constants_match_separate_update(form, coop_shape):
     """UpdateStateParameters fused into the Solve launch gives the same rate constants
-    and the same solve as the separate call (hand-off included)."""
+    and the same solve as the separate call (hand-off and the compact cooperative
+    layout, whose rate constants go through global memory, included)."""
     desc, inputs, tableau, dt = CASES["60_200"]
     mech = _lib.Mechanism(desc)
     cells = inputs(desc, 300, seed=17)
     order = mech.species_order()
-    print(f"fused rates: form={form}", flush=True)
+    print(f"fused rates: form={form} shape={coop_shape}", flush=True)
     ref = _solve(mech, cells, order, tableau, dt, form, handoff=2)
     state = _lib.DeviceState(mech, cells.n_cells)
     state.set_handoff(2)
+    state.set_cooperative_shape(*coop_shape)
     state.set_variables(cells.variables[:, order])
     state.set_conditions(cells.conditions)
     state.set_custom_parameters(cells.custom)
