@@ -8,7 +8,8 @@ Solve of 900 s per step.  Inputs are resident in HBM before timing starts; a
 step restores the initial concentrations (device-to-device copy) and runs
 Solver::Solve with the per-cell form the autotuner picks (micm_amd/autotune.py;
 every form gives bit-identical results).  `--workload ts1` runs the reference's
-own benchmark instead: TS1, 10,000 cells, 3-stage, 30 s steps.
+own benchmark instead: TS1, 10,000 cells, 3-stage, 30 s steps; `--workload packed`
+three mechanisms of different sizes in one graph-captured launch (main_packed).
 
 Prints one JSON line.  `roofline` relates the algorithmic bytes of the path --
 per attempt F x forcing + Jacobian + LU + S x substitution + error, with the
@@ -191,7 +192,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="210_560", choices=["210_560", "ts1"])
+    ap.add_argument("--workload", default="210_560", choices=["210_560", "ts1", "packed"])
     ap.add_argument("--cells", type=int, default=0, help="cells (0: 100000 for 210_560, 10000 for ts1)")
     ap.add_argument("--form", default="autotune", choices=["autotune", "persistent", "plain", "cooperative"])
     ap.add_argument("--coop-threads", type=int, default=0, help="threads per cell of the cooperative form (0: library)")
@@ -219,6 +220,9 @@ def main():
         print(f"bench.py measures one GPU: the per-cell forms need no exchange between GPUs and this build "
               f"runs on a single MI355X; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "packed":
+        main_packed(args)
+        return
 
     import torch
 
@@ -370,6 +374,112 @@ def main():
                                             f"cell, {el:.1f} s wall on {threads} cores",
                                   "single_core_value": round(cps1, 2),
                                   "single_core_sample": f"first {sample1} cells, {el1:.1f} s"}
+    print(json.dumps(result), flush=True)
+
+
+PACKED_PARTS = (("20/50", "mechanism_20_50", 200000), ("60/200", "mechanism_60_200", 50000),
+                ("210/560", "mechanism_210_560", 20000))
+
+
+def main_packed(args):
+    """Three seeded mechanisms of different sizes (20/50 x 200k cells, 60/200 x 50k,
+    210/560 x 20k; T 180-320 K, P log-uniform 1-101 kPa, 50 % night, 4-stage, 1 x 900 s)
+    solved by one graph-captured launch (PackedSolve: the three cooperative launches back
+    to back), each in the cooperative shape the autotuner picks for its mechanism.  `value` is
+    all cells over the wall time of the packed solve; `separate` times the three
+    solves one after another without the graph."""
+    import torch
+
+    from micm_amd import _lib, autotune, synthetic
+    torch.cuda.set_device(0)
+    dt, tableau = 900.0, 4
+    params = _lib.rosenbrock_parameters(tableau)
+    stream = torch.cuda.current_stream()
+    coop = [c for c in autotune.CANDIDATES if c[1] == _lib.SOLVE_COOPERATIVE]
+    parts = []
+    for i, (label, maker, n) in enumerate(PACKED_PARTS):
+        desc = getattr(synthetic, maker)()
+        cells = synthetic.random_inputs(desc, n, 31 + i, (180.0, 320.0), log_pressure=(1.0e3, 1.01e5),
+                                        night_fraction=0.5)
+        mech = _lib.Mechanism(desc)
+        order = mech.species_order()
+        state = _lib.DeviceState(mech, n)
+        state.set_variables(np.ascontiguousarray(cells.variables[:, order]))
+        state.set_conditions(cells.conditions)
+        state.set_custom_parameters(cells.custom)
+        state.update_state_parameters()
+        choice = autotune.autotune(mech, state, dt, params, candidates=coop)
+        autotune.apply(state, choice)
+        ptr = state.pointers()
+        n_y = ptr.n_groups * 64 * mech.n_species
+        y0 = torch.empty(n_y, dtype=torch.float64, device="cuda")
+        _copy_device(y0.data_ptr(), ptr.variables, n_y * 8, stream)
+        parts.append(dict(label=label, desc=desc, cells=cells, mech=mech, order=order, state=state, ptr=ptr, n_y=n_y,
+                          y0=y0, choice=choice["label"]))
+        log(f"{label}: {n} cells, {choice['label']}")
+    torch.cuda.synchronize()
+
+    def restore():
+        for q in parts:
+            _copy_device(q["ptr"].variables, q["y0"].data_ptr(), q["n_y"] * 8, stream)
+        torch.cuda.synchronize()
+
+    packed = _lib.PackedSolve([q["state"] for q in parts], _lib.SOLVE_COOPERATIVE)
+    for _ in range(max(args.warmup, 1)):
+        restore()
+        packed.solve(dt, params)
+    wall = 0.0
+    for _ in range(args.steps):
+        restore()
+        t0 = time.perf_counter()
+        packed.solve(dt, params)
+        wall += time.perf_counter() - t0
+    replayed = packed.replayed
+    n_total = sum(q["state"].n_cells for q in parts)
+    stats_all = [q["state"].results()[1] for q in parts]
+    algo = sum(algorithmic_bytes(q["mech"].info, q["mech"].n_species, q["mech"].n_reactions, st)
+               for q, st in zip(parts, stats_all))
+    log(f"{args.steps} packed steps: {wall:.2f} s")
+    restore()
+    t0 = time.perf_counter()
+    for q in parts:
+        q["state"].solve(dt, params, _lib.SOLVE_COOPERATIVE)
+    torch.cuda.synchronize()
+    separate = time.perf_counter() - t0
+    ms = wall * 1e3 / args.steps
+    achieved = algo / (ms * 1e-3) / 1e9
+    result = {
+        "metric": "cell-solves/s, three mechanisms packed in one graph launch (20/50 x 200k, 60/200 x 50k, "
+                  "210/560 x 20k cells, FP64)",
+        "value": round(n_total * args.steps / wall, 2), "unit": "cell-solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded mechanisms and cells)",
+        "config": {"workload": "packed: " + ", ".join(f"{q['label']} x {q['state'].n_cells} ({q['choice']})"
+                                                      for q in parts) +
+                               "; T 180-320 K, P 1-101 kPa, 50 % night, 4-stage Rosenbrock, 1 x 900 s",
+                   "cells": n_total, "graph_replayed": replayed},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "the packed Solve (three cooperative launches in one graph), wall time "
+                               f"of the graph launch; algorithmic bytes per step {algo}"},
+        "attempts_per_step": int(sum(st[:, 2].sum() for st in stats_all)),
+        "separate": {"value": round(n_total / separate, 2), "unit": "cell-solves/s",
+                     "note": "the three solves one after another, no graph"},
+    }
+    if args.cpu_sample != 0:
+        threads = args.cpu_threads or len(os.sched_getaffinity(0))
+        seconds = 0.0
+        parts_cpu = []
+        for q in parts:
+            kind1, cps1, _ = cpu_baseline(q["desc"], q["cells"], q["order"], dt, 16, 1, tableau)
+            sample = min(q["state"].n_cells, max(256, int(5.0 * cps1)))
+            kind, cps, el = cpu_baseline(q["desc"], q["cells"], q["order"], dt, sample, threads, tableau)
+            seconds += q["state"].n_cells / cps
+            parts_cpu.append(f"{q['label']}: {sample} cells {el:.1f} s ({cps:.0f} cell-solves/s)")
+        result["cpu_baseline"] = {"value": round(n_total / seconds, 2), "unit": "cell-solves/s", "cores": threads,
+                                  "kind": kind, "cpu": cpu_model(),
+                                  "sample": "each mechanism's first cells, one 1-cell State per cell, combined as "
+                                            "all cells over the summed times; " + "; ".join(parts_cpu)}
     print(json.dumps(result), flush=True)
 
 

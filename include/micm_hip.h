@@ -303,8 +303,10 @@ int micm_hip_solve(const micm_hip_mechanism* mechanism, micm_hip_state* state, d
 /* ---------------------------------------------------------------- packed solves
  * Several states -- of one mechanism or of different ones -- solved together.  The
  * per-cell Solve of every state (MICM_HIP_SOLVE_PERSISTENT or _COOPERATIVE) is
- * captured once into a hipGraph with one concurrent branch per state, and later
- * solves with the same time step, parameters and launch shapes replay it.  The
+ * captured once into a hipGraph, one state after another (each launch fills the
+ * GPU; run as concurrent branches, launches of different mechanisms fragment the
+ * CUs' LDS and finish later), and later solves with the same time step, parameters
+ * and launch shapes replay it.  The
  * states must outlive the packed object and must not be used by another call while
  * a packed solve runs.  No counterpart in the reference, whose States are solved one
  * at a time; for callers that integrate several mechanisms (or several cell sets)

@@ -334,7 +334,7 @@ class DeviceState:
 
 class PackedSolve:
     """Several DeviceStates (any mechanisms) solved together by one graph-captured
-    launch with a concurrent branch per state (micm_hip_packed_*).  Keeps the states
+    launch, one state after another (micm_hip_packed_*).  Keeps the states
     alive while it exists."""
 
     def __init__(self, states, form=SOLVE_COOPERATIVE):

@@ -1271,9 +1271,6 @@ struct micm_hip_packed
   std::vector<micm_hip_state*> states;
   int32_t form = MICM_HIP_SOLVE_COOPERATIVE;
   hipStream_t origin = nullptr;
-  std::vector<hipStream_t> branch;
-  std::vector<hipEvent_t> done;
-  hipEvent_t fork = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   // what the captured graph was built for
@@ -2491,16 +2488,6 @@ extern "C"
     p->states.assign(states, states + n_states);
     p->form = form;
     HIP_TRY(hipStreamCreate(&p->origin));
-    HIP_TRY(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
-    for (int32_t i = 0; i < n_states; ++i)
-    {
-      hipStream_t b = nullptr;
-      hipEvent_t e = nullptr;
-      HIP_TRY(hipStreamCreate(&b));
-      p->branch.push_back(b);
-      HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-      p->done.push_back(e);
-    }
     *out = p.release();
     return MICM_HIP_OK;
   }
@@ -2534,20 +2521,11 @@ extern "C"
       pk->captured = false;
       HIP_TRY(hipStreamBeginCapture(pk->origin, hipStreamCaptureModeThreadLocal));
       int rc = MICM_HIP_OK;
-      if (hipEventRecord(pk->fork, pk->origin) != hipSuccess)
-        rc = fail(MICM_HIP_ERR_DEVICE, "event record failed during capture");
       for (size_t i = 0; rc == MICM_HIP_OK && i < pk->states.size(); ++i)
       {
         micm_hip_state* s = pk->states[i];
         const DevParams p = make_params(*params, time_step, s->rtol);
-        if (hipStreamWaitEvent(pk->branch[i], pk->fork, 0) != hipSuccess)
-          rc = fail(MICM_HIP_ERR_DEVICE, "stream wait failed during capture");
-        else if ((rc = enqueue_solve(s, p, pk->form, false, pk->branch[i])) == MICM_HIP_OK)
-        {
-          if (hipEventRecord(pk->done[i], pk->branch[i]) != hipSuccess ||
-              hipStreamWaitEvent(pk->origin, pk->done[i], 0) != hipSuccess)
-            rc = fail(MICM_HIP_ERR_DEVICE, "join failed during capture");
-        }
+        rc = enqueue_solve(s, p, pk->form, false, pk->origin);
       }
       hipGraph_t graph = nullptr;
       const hipError_t end = hipStreamEndCapture(pk->origin, &graph);
@@ -2585,12 +2563,6 @@ extern "C"
       (void)hipGraphExecDestroy(pk->exec);
     if (pk->graph)
       (void)hipGraphDestroy(pk->graph);
-    for (hipEvent_t e : pk->done)
-      (void)hipEventDestroy(e);
-    for (hipStream_t b : pk->branch)
-      (void)hipStreamDestroy(b);
-    if (pk->fork)
-      (void)hipEventDestroy(pk->fork);
     if (pk->origin)
       (void)hipStreamDestroy(pk->origin);
     delete pk;
