@@ -357,6 +357,23 @@ def main():
         result["plain_form"] = {"value": round(cmp["plain"], 2), "unit": "cell-solves/s", "cells": m,
                                 "chosen_form_same_cells": round(cmp["chosen"], 2),
                                 "speedup": round(cmp["chosen"] / cmp["plain"], 3)}
+        # the reference's own multi-cell State: one step size for every cell
+        ml = min(m, 512)
+        lock = _lib.DeviceState(mech, ml)
+        subl = sub.subset(np.arange(ml))
+        lock.set_variables(subl.variables[:, order])
+        lock.set_conditions(subl.conditions)
+        lock.set_custom_parameters(subl.custom)
+        lock.update_state_parameters()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lock.solve(dt, params, _lib.SOLVE_LOCKSTEP)
+        el = time.perf_counter() - t0
+        st_l, stats_l, _ = lock.results()
+        log("lockstep form compared")
+        result["lockstep_form"] = {"value": round(ml / el, 2), "unit": "cell-solves/s", "cells": ml,
+                                   "steps": int(stats_l[0, 2]), "converged": bool(np.all(st_l == 2)),
+                                   "note": "one step size for all cells (the reference's multi-cell State)"}
     if not args.no_kernels:
         result["kernels"] = kernel_rooflines(mech, state, stream, n_cells)
         log("single-phase kernels timed")
