@@ -217,8 +217,8 @@ def main():
         print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8, "launches": 3}))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: the per-cell forms need no exchange between GPUs and this build "
-              f"runs on a single MI355X; --gpus {args.gpus} is not supported", file=sys.stderr)
+        print(f"bench.py measures one GPU (the per-cell forms need no exchange between GPUs; a multi-GPU run "
+              f"shards cells, one process per GPU); --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
     if args.workload == "packed":
         main_packed(args)
