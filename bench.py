@@ -172,6 +172,16 @@ def workload(name, n_cells):
         return desc, cells, 4, 900.0, (f"random 210-species/560-reaction mechanism (seed {desc.seed}), {n_cells} cells "
                                       f"(seed {cells.seed}), T 180-320 K, P 1-101 kPa, 50 % night, 4-stage Rosenbrock, "
                                       f"1 x 900 s")
+    if name == "60_200":
+        desc = synthetic.mechanism_60_200()
+        cells = synthetic.inputs_60_200(desc, n_cells)
+        return desc, cells, 4, 300.0, (f"random 60-species/200-reaction mechanism (seed {desc.seed}), {n_cells} cells "
+                                      f"(seed {cells.seed}), T 220-310 K, P 101325 Pa, 4-stage Rosenbrock, 1 x 300 s")
+    if name == "readme":
+        desc = synthetic.readme_mechanism()
+        cells = synthetic.readme_inputs(n_cells)
+        return desc, cells, 3, 500.0, (f"the reference README's 3-species system, {n_cells} cells, T 270-310 K, "
+                                      f"P 90-105 kPa, 3-stage Rosenbrock, 1 x 500 s")
     if name == "ts1":
         from micm_amd import mechanism_io
         desc = mechanism_io.load(os.path.join(ROOT, "tests", "golden", "ts1_mechanism.json"))
@@ -192,8 +202,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="210_560", choices=["210_560", "ts1", "packed"])
-    ap.add_argument("--cells", type=int, default=0, help="cells (0: 100000 for 210_560, 10000 for ts1)")
+    ap.add_argument("--workload", default="210_560", choices=["210_560", "60_200", "readme", "ts1", "packed"])
+    ap.add_argument("--cells", type=int, default=0,
+                    help="cells (0: 100000 for 210_560, 10000 for 60_200 and ts1, 1000 for readme)")
     ap.add_argument("--form", default="autotune", choices=["autotune", "persistent", "plain", "cooperative"])
     ap.add_argument("--coop-threads", type=int, default=0, help="threads per cell of the cooperative form (0: library)")
     ap.add_argument("--handoff", type=int, default=-1, help="persistent form: hand-off step count (-1: library)")
@@ -228,7 +239,7 @@ def main():
 
     from micm_amd import _lib, autotune
 
-    n_cells = args.cells or (100000 if args.workload == "210_560" else 10000)
+    n_cells = args.cells or {"210_560": 100000, "readme": 1000}.get(args.workload, 10000)
     torch.cuda.set_device(0)
     desc, cells, tableau, dt, label = workload(args.workload, n_cells)
     log(f"workload {args.workload}: {n_cells} cells generated")
@@ -305,8 +316,10 @@ def main():
     achieved = bytes_per_step / (kernel_ms * 1e-3) / 1e9
     traffic, traffic_source = measured_traffic(args.workload, n_cells, coop_threads) if coop_threads else (None, None)
     result = {
-        "metric": "cell-solves/s, 210-species x 100k cells, batched Rosenbrock (FP64)"
-        if args.workload == "210_560" else "cell-solves/s, TS1 x 10k cells (reference benchmark mechanism)",
+        "metric": {"210_560": "cell-solves/s, 210-species x 100k cells, batched Rosenbrock (FP64)",
+                   "60_200": f"cell-solves/s, 60-species x {n_cells} cells, Rosenbrock (FP64)",
+                   "readme": f"cell-solves/s, README 3-species system x {n_cells} cells (FP64)",
+                   "ts1": "cell-solves/s, TS1 x 10k cells (reference benchmark mechanism)"}[args.workload],
         "value": round(value, 2),
         "unit": "cell-solves/s",
         "n_gpus": 1,
@@ -317,8 +330,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded mechanism and cells)" if args.workload == "210_560"
-                else "the reference's TS1 benchmark mechanism and initial state",
+        "data": "the reference's TS1 benchmark mechanism and initial state" if args.workload == "ts1"
+                else "synthetic (seeded mechanism and cells)",
         "config": {"workload": label, "form": form_label, "cells": n_cells, "tableau_stages": int(params.stages),
                    "time_step_s": dt},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
