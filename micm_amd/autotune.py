@@ -18,17 +18,18 @@ import numpy as np
 
 from . import _lib
 
-# (label, form, handoff steps, cooperative threads, staging)
+# (label, form, handoff steps, cooperative threads, staging); the shapes that are
+# usually fast come first, so that the pilot can give up on slow ones early
 CANDIDATES = (
-    ("persistent", _lib.SOLVE_PERSISTENT, 0, 0, -1),
-    ("persistent+handoff16", _lib.SOLVE_PERSISTENT, 16, 0, -1),
-    ("persistent+handoff48", _lib.SOLVE_PERSISTENT, 48, 0, -1),
-    ("cooperative64", _lib.SOLVE_COOPERATIVE, -1, 64, -1),
     ("cooperative128", _lib.SOLVE_COOPERATIVE, -1, 128, -1),
     ("cooperative256", _lib.SOLVE_COOPERATIVE, -1, 256, -1),
+    ("cooperative64", _lib.SOLVE_COOPERATIVE, -1, 64, -1),
     ("cooperative512", _lib.SOLVE_COOPERATIVE, -1, 512, -1),
     ("cooperative64-global-tables", _lib.SOLVE_COOPERATIVE, -1, 64, 0),
     ("cooperative128-global-tables", _lib.SOLVE_COOPERATIVE, -1, 128, 0),
+    ("persistent+handoff16", _lib.SOLVE_PERSISTENT, 16, 0, -1),
+    ("persistent+handoff48", _lib.SOLVE_PERSISTENT, 48, 0, -1),
+    ("persistent", _lib.SOLVE_PERSISTENT, 0, 0, -1),
 )
 
 _cache = {}
@@ -47,7 +48,10 @@ def apply(state, choice):
     return choice["form"]
 
 
-def _time_candidates(mech, y, k, dt, params, candidates, repeats, verbose):
+def _time_candidates(mech, y, k, dt, params, candidates, repeats, verbose, give_up=None):
+    """Seconds per Solve of each candidate on these cells (best of `repeats` after a
+    warm-up run); a candidate whose warm-up run takes more than `give_up` seconds is
+    not repeated and keeps that time."""
     probe = _lib.DeviceState(mech, y.shape[0])
     probe.set_rate_constants(k)
     times = {}
@@ -55,25 +59,30 @@ def _time_candidates(mech, y, k, dt, params, candidates, repeats, verbose):
         probe.set_handoff(handoff)
         probe.set_cooperative_shape(threads, stage)
         el = []
-        for _ in range(repeats + 1):  # the first run warms up
+        for r in range(repeats + 1):  # the first run warms up
             probe.set_variables(y)
             t0 = time.perf_counter()
             probe.solve(dt, params, form)
             el.append(time.perf_counter() - t0)
-        times[label] = min(el[1:])
+            if r == 0 and give_up is not None and el[0] > give_up:
+                break
+        times[label] = min(el[1:]) if len(el) > 1 else el[0]
         if verbose:
             print(f"autotune: {label:32s} {y.shape[0]:6d} cells {times[label] * 1e3:9.2f} ms", file=sys.stderr,
                   flush=True)
     return times
 
 
-def autotune(mech, state, dt, params, sample=2048, first_sample=512, finalists=3, repeats=1, candidates=CANDIDATES,
-             verbose=False):
+def autotune(mech, state, dt, params, sample=16384, first_sample=2048, pilot=256, finalists=3, repeats=1,
+             candidates=CANDIDATES, verbose=False):
     """Pick the fastest per-cell form for `state` (whose inputs and rate constants
-    are set): every candidate is timed on a strided sample of `first_sample` cells,
-    the `finalists` fastest again on `sample` cells.  Returns a dict with keys label,
-    form, handoff, threads, stage, sample, seconds (label -> seconds on the larger
-    sample, or on the first one for candidates that did not go on)."""
+    are set), on strided samples of its cells: every candidate on `pilot` cells,
+    those within twice the best on `first_sample` cells, the `finalists` fastest of
+    those on `sample` cells.  The last samples must hold many cells per resident
+    workgroup: on a few cells per workgroup the time is that of the slowest cells,
+    which favours the shapes that run one cell fastest over those that run the most
+    cells at once.  Returns a dict with keys label, form, handoff, threads, stage,
+    sample, seconds (label -> seconds on the largest sample the candidate reached)."""
     n = state.n_cells
     m = min(sample, n)
     key = _key(mech, params.stages, dt, m)
@@ -88,12 +97,26 @@ def autotune(mech, state, dt, params, sample=2048, first_sample=512, finalists=3
         idx = np.linspace(0, n - 1, count).round().astype(np.int64)
         return y_all[idx], k_all[idx]
 
+    m0 = min(pilot, m)
+    times = {}
+    best = None
+    slow_handoff = False
+    for c in cands:  # pilot: each candidate gives up once it is four times slower than the best so far
+        if c[1] == _lib.SOLVE_PERSISTENT and c[2] == 0 and slow_handoff:
+            continue  # without hand-off the slowest cells run even longer one per lane
+        t = _time_candidates(mech, *pick(m0), dt, params, [c], repeats, verbose,
+                             give_up=None if best is None else 4.0 * best)[c[0]]
+        times[c[0]] = t
+        if c[1] == _lib.SOLVE_PERSISTENT and c[2] > 0 and best is not None and t > 4.0 * best:
+            slow_handoff = True
+        best = t if best is None else min(best, t)
+    cands = [c for c in cands if c[0] in times and times[c[0]] <= 2.0 * best]
     m1 = min(first_sample, m)
-    times = _time_candidates(mech, *pick(m1), dt, params, cands, repeats, verbose)
-    if m > m1:
-        best = sorted(cands, key=lambda c: times[c[0]])[:finalists]
-        times.update(_time_candidates(mech, *pick(m), dt, params, best, repeats, verbose))
-        cands = best
+    if m1 > m0 and len(cands) > 1:
+        times.update(_time_candidates(mech, *pick(m1), dt, params, cands, repeats, verbose))
+        cands = sorted(cands, key=lambda c: times[c[0]])[:finalists]
+    if m > m1 and len(cands) > 1:
+        times.update(_time_candidates(mech, *pick(m), dt, params, cands, repeats, verbose))
     label, form, handoff, threads, stage = min(cands, key=lambda c: times[c[0]])
     choice = {"label": label, "form": form, "handoff": handoff, "threads": threads, "stage": stage,
               "sample": m, "seconds": times}

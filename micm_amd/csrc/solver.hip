@@ -840,6 +840,58 @@ namespace
     *p = 0;
   }
 
+  // ---- processing order of the per-cell forms: the cells of the previous Solve,
+  // most step attempts first, so that the longest cells do not start last.  A
+  // counting sort over 64 bins of the attempt count (four bins per power of two);
+  // any permutation gives the same results, so ties may land in any order.
+  constexpr int kOrderBins = 64;
+
+  __device__ inline int order_bin(int64_t attempts)
+  {
+    const uint64_t a = attempts > 1 ? static_cast<uint64_t>(attempts) : 1u;
+    const int msb = 63 - __clzll(a);
+    const int frac = msb >= 2 ? static_cast<int>((a >> (msb - 2)) & 3u) : 0;
+    const int b = msb * 4 + frac;
+    return kOrderBins - 1 - (b < kOrderBins - 1 ? b : kOrderBins - 1);  // heaviest first
+  }
+
+  __global__ void __launch_bounds__(kBlock) k_order_clear(int* bins)
+  {
+    if (threadIdx.x <= kOrderBins)
+      bins[threadIdx.x] = 0;
+  }
+
+  __global__ void __launch_bounds__(kBlock) k_order_count(const int64_t* __restrict__ stats, int64_t n_cells, int* bins)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c < n_cells)
+      atomicAdd(&bins[order_bin(stats[c * 7 + MICM_HIP_STAT_STEPS])], 1);
+  }
+
+  // exclusive prefix sum of the bins; bins[kOrderBins] = number of cells
+  __global__ void k_order_offsets(int* bins)
+  {
+    if (threadIdx.x == 0)
+    {
+      int acc = 0;
+      for (int b = 0; b < kOrderBins; ++b)
+      {
+        const int v = bins[b];
+        bins[b] = acc;
+        acc += v;
+      }
+      bins[kOrderBins] = acc;
+    }
+  }
+
+  __global__ void __launch_bounds__(kBlock) k_order_place(const int64_t* __restrict__ stats, int64_t n_cells, int* bins,
+                                                          int32_t* order)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c < n_cells)
+      order[atomicAdd(&bins[order_bin(stats[c * 7 + MICM_HIP_STAT_STEPS])], 1)] = static_cast<int32_t>(c);
+  }
+
   // ---- persistent form
 
   struct Queue
@@ -1259,6 +1311,7 @@ struct micm_hip_state
   int32_t* handoff_list = nullptr;
   int* handoff_count = nullptr;
   int* coop_next = nullptr;
+  int* order_bins = nullptr;  // kOrderBins + 1: the processing order's counting sort
   double* coop_k = nullptr;  // rate-constant rows of the compact cooperative layout
   int64_t coop_k_rows = 0;
   // cooperative launch shape (-1 / 0: chosen from the mechanism)
@@ -1875,12 +1928,37 @@ namespace
 
   // The launches of one Solve of a per-cell form (persistent or cooperative), enqueued
   // on `st` without waiting; prepare_solve must have run.
+  // The processing order of the next Solve from this Solve's per-cell attempt counts.
+  int enqueue_order(micm_hip_state* s, hipStream_t st)
+  {
+    const unsigned grid = static_cast<unsigned>((s->n_cells + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_order_clear, dim3(1), dim3(kBlock), 0, st, s->order_bins);
+    hipLaunchKernelGGL(k_order_count, dim3(grid), dim3(kBlock), 0, st, s->stats, s->n_cells, s->order_bins);
+    hipLaunchKernelGGL(k_order_offsets, dim3(1), dim3(1), 0, st, s->order_bins);
+    hipLaunchKernelGGL(k_order_place, dim3(grid), dim3(kBlock), 0, st, s->stats, s->n_cells, s->order_bins, s->order);
+    HIP_TRY(hipGetLastError());
+    s->has_order = true;
+    return MICM_HIP_OK;
+  }
+
+  int enqueue_solve_cells(micm_hip_state* s, const DevParams& p, int32_t form, bool update_rates, hipStream_t st);
+
+  // A per-cell Solve (cells in the order of the previous Solve's attempt counts, most
+  // first, once there was one), then the order for the next.
   int enqueue_solve(micm_hip_state* s, const DevParams& p, int32_t form, bool update_rates, hipStream_t st)
+  {
+    if (int rc = enqueue_solve_cells(s, p, form, update_rates, st))
+      return rc;
+    return enqueue_order(s, st);
+  }
+
+  int enqueue_solve_cells(micm_hip_state* s, const DevParams& p, int32_t form, bool update_rates, hipStream_t st)
   {
     const micm_hip_mechanism* m = s->mech;
     const DevMech& d = m->dev;
     if (form == MICM_HIP_SOLVE_COOPERATIVE)
-      return launch_coop(s, p, nullptr, nullptr, s->n_cells, nullptr, update_rates, st);
+      return s->has_order ? launch_coop(s, p, s->order, s->order_bins + kOrderBins, 0, nullptr, update_rates, st)
+                          : launch_coop(s, p, nullptr, nullptr, s->n_cells, nullptr, update_rates, st);
     int64_t blocks = 0;
     if (int rc = persistent_blocks(s, blocks))
       return rc;
@@ -2244,6 +2322,7 @@ extern "C"
     HIP_TRY(hipMalloc(&s->handoff_list, sizeof(int32_t) * n_cells));
     HIP_TRY(hipMalloc(&s->handoff_count, sizeof(int)));
     HIP_TRY(hipMalloc(&s->coop_next, sizeof(int)));
+    HIP_TRY(hipMalloc(&s->order_bins, sizeof(int) * (kOrderBins + 1)));
     HIP_TRY(hipHostMalloc(&s->host_active, sizeof(unsigned int)));
     HIP_TRY(hipMemcpy(s->atol, m->t.absolute_tolerances.data(), sizeof(double) * d.n_species, hipMemcpyHostToDevice));
     *out = s.release();
@@ -2276,6 +2355,8 @@ extern "C"
       (void)hipFree(s->handoff_count);
     if (s->coop_next)
       (void)hipFree(s->coop_next);
+    if (s->order_bins)
+      (void)hipFree(s->order_bins);
     if (s->host_active)
       (void)hipHostFree(s->host_active);
     delete s;

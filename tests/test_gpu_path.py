@@ -232,6 +232,25 @@ def test_per_cell_forms_agree_bitwise(name):
         assert bits_equal(r[3], ref[3])
 
 
+@pytest.mark.parametrize("name", ["60_200", "210_560"])
+def test_repeated_solves_in_cost_order_agree_bitwise(name):
+    """From the second Solve of a state on, the per-cell forms take cells in the order
+    of the previous Solve's attempt counts (most first); results do not change."""
+    desc, inputs, tableau, dt = CASES[name]
+    mech = _lib.Mechanism(desc)
+    cells = inputs(desc, 300 if name == "60_200" else 130, seed=5)
+    order = mech.species_order()
+    print(f"repeated solves: {name}", flush=True)
+    ref = _solve(mech, cells, order, tableau, dt, _lib.SOLVE_PLAIN, n_solves=3)
+    runs = [_solve(mech, cells, order, tableau, dt, _lib.SOLVE_PERSISTENT, n_solves=3, handoff=2),
+            _solve(mech, cells, order, tableau, dt, _lib.SOLVE_COOPERATIVE, n_solves=3),
+            _solve(mech, cells, order, tableau, dt, _lib.SOLVE_COOPERATIVE, n_solves=3, coop_shape=(256, 4))]
+    for r in runs:
+        assert bits_equal(r[0], ref[0])
+        assert np.array_equal(r[1], ref[1]) and np.array_equal(r[2], ref[2])
+        assert bits_equal(r[3], ref[3])
+
+
 @pytest.mark.parametrize("form, coop_shape", [(_lib.SOLVE_PLAIN, (0, -1)), (_lib.SOLVE_PERSISTENT, (0, -1)),
                                               (_lib.SOLVE_COOPERATIVE, (0, -1)), (_lib.SOLVE_PERSISTENT, (128, 4)),
                                               (_lib.SOLVE_COOPERATIVE, (128, 4))])
