@@ -1021,15 +1021,17 @@ namespace
   // gridDim.x rate-constant rows of n_reactions doubles.
   //
   // With 256 threads per cell the kernel is compiled for three waves per SIMD (at most
-  // 168 VGPRs), which lets a third cell onto a CU where LDS allows (TS1: 1.26x).
+  // 168 VGPRs, a few of them spilled), which lets a third cell onto a CU where LDS
+  // allows (TS1: 1.26x); the compact layout also has a build without that bound
+  // (MINW 1), taken where LDS allows two cells per CU anyway (210/560).
   template<int NT>
   constexpr int coop_min_waves_per_eu()
   {
     return NT == 256 ? 3 : 1;
   }
 
-  template<int NT, int STAGE, bool COMPACT = false>
-  __global__ void __launch_bounds__(NT, coop_min_waves_per_eu<NT>()) k_coop(DevMech m, DevCoop c, DevParams p, int p_capacity, int64_t n_fixed,
+  template<int NT, int STAGE, bool COMPACT = false, int MINW = coop_min_waves_per_eu<NT>()>
+  __global__ void __launch_bounds__(NT, MINW) k_coop(DevMech m, DevCoop c, DevParams p, int p_capacity, int64_t n_fixed,
                                                const int32_t* __restrict__ list, const int* list_count, int* next,
                                                const CellControl* resume, double* __restrict__ y_state,
                                                double* __restrict__ k_state, const double* __restrict__ atol,
@@ -1629,14 +1631,23 @@ namespace
     int threads = kWave;
     int p_capacity = 0;     // doubles of the per-level product buffer
     bool compact = false;   // coop_layout_compact
+    bool unbounded = false;  // compact, 256 threads: the build without the three-waves bound
   };
 
   // Workgroups of the cooperative kernel that fit one CU (registers and LDS).
   template<int NT>
-  int coop_occupancy_nt(int stage, bool compact, size_t lds)
+  int coop_occupancy_nt(int stage, bool compact, size_t lds, bool unbounded = false)
   {
     int per_cu = 0;
     hipError_t e = hipSuccess;
+    if constexpr (NT == 256)
+    {
+      if (compact && unbounded)
+      {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coop<NT, 0, true, 1>, NT, lds);
+        return e == hipSuccess ? per_cu : 0;
+      }
+    }
     if (compact)
       e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coop<NT, 0, true>, NT, lds);
     else if (stage == 1)
@@ -1650,12 +1661,12 @@ namespace
     return e == hipSuccess ? per_cu : 0;
   }
 
-  int coop_occupancy(int threads, int stage, bool compact, size_t lds)
+  int coop_occupancy(int threads, int stage, bool compact, size_t lds, bool unbounded = false)
   {
     switch (threads)
     {
       case 128: return coop_occupancy_nt<128>(stage, compact, lds);
-      case 256: return coop_occupancy_nt<256>(stage, compact, lds);
+      case 256: return coop_occupancy_nt<256>(stage, compact, lds, unbounded);
       case 512: return coop_occupancy_nt<512>(stage, compact, lds);
       case 1024: return coop_occupancy_nt<1024>(stage, compact, lds);
       default: return coop_occupancy_nt<kWave>(stage, compact, lds);
@@ -1753,7 +1764,9 @@ namespace
                            sizeof(double);
     if (compact > lds_limit())
       return plan;
-    const int occ_compact = coop_occupancy(threads, 0, true, compact);
+    const int occ_bounded = coop_occupancy(threads, 0, true, compact);
+    const int occ_unbounded = threads == 256 ? coop_occupancy(threads, 0, true, compact, true) : 0;
+    const int occ_compact = std::max(occ_bounded, occ_unbounded);
     const int occ_full = plan.lds > 0 ? coop_occupancy(threads, plan.stage, false, plan.lds) : 0;
     if (stage_request == 4 ? occ_compact > 0 : occ_compact > occ_full)
     {
@@ -1762,6 +1775,7 @@ namespace
       plan.threads = threads;
       plan.p_capacity = c.bwd.max_level_terms;
       plan.compact = true;
+      plan.unbounded = occ_unbounded >= occ_bounded;  // no register spills when it costs no cell per CU
     }
     return plan;
   }
@@ -1822,16 +1836,17 @@ namespace
     int dev = 0, cus = 0;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    blocks = static_cast<int64_t>(cus) * std::max(coop_occupancy(plan.threads, plan.stage, plan.compact, plan.lds), 1);
+    blocks = static_cast<int64_t>(cus) *
+             std::max(coop_occupancy(plan.threads, plan.stage, plan.compact, plan.lds, plan.unbounded), 1);
     return MICM_HIP_OK;
   }
 
-  template<int NT, int STAGE, bool COMPACT = false>
+  template<int NT, int STAGE, bool COMPACT = false, int MINW = coop_min_waves_per_eu<NT>()>
   int launch_coop_shape(micm_hip_state* s, const DevParams& p, const CoopPlan& plan, const int32_t* list,
                         const int* count, int64_t n_fixed, const CellControl* resume, bool update_rates, hipStream_t st)
   {
     const micm_hip_mechanism* m = s->mech;
-    const void* fn = reinterpret_cast<const void*>(k_coop<NT, STAGE, COMPACT>);
+    const void* fn = reinterpret_cast<const void*>(k_coop<NT, STAGE, COMPACT, MINW>);
     HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(plan.lds)));
     int64_t blocks = 0;
     if (int rc = coop_blocks(plan, blocks))
@@ -1841,7 +1856,8 @@ namespace
     if (!list)
       blocks = std::max<int64_t>(1, std::min(blocks, n_fixed));
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, s->coop_next);
-    hipLaunchKernelGGL((k_coop<NT, STAGE, COMPACT>), dim3(static_cast<unsigned>(blocks)), dim3(NT), plan.lds, st, m->dev,
+    hipLaunchKernelGGL((k_coop<NT, STAGE, COMPACT, MINW>), dim3(static_cast<unsigned>(blocks)), dim3(NT), plan.lds, st,
+                       m->dev,
                        m->coop, p, plan.p_capacity, n_fixed, list, count, s->coop_next, resume, s->y, s->k, s->atol,
                        s->status, s->stats, s->final_time, update_rates ? s->cond : nullptr,
                        update_rates ? s->custom : nullptr, s->coop_k);
@@ -1853,6 +1869,11 @@ namespace
   int launch_coop_nt(micm_hip_state* s, const DevParams& p, const CoopPlan& plan, const int32_t* list,
                      const int* count, int64_t n_fixed, const CellControl* resume, bool update_rates, hipStream_t st)
   {
+    if constexpr (NT == 256)
+    {
+      if (plan.compact && plan.unbounded)
+        return launch_coop_shape<NT, 0, true, 1>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
+    }
     if (plan.compact)
       return launch_coop_shape<NT, 0, true>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
     switch (plan.stage)
