@@ -232,6 +232,23 @@ def test_per_cell_forms_agree_bitwise(name):
         assert bits_equal(r[3], ref[3])
 
 
+def test_cooperative_shape_arguments_checked():
+    """Launch-shape requests outside the supported set fail with an error; a compact
+    request the mechanism cannot take (fewer threads than species) falls back to the
+    full layout and still solves."""
+    desc, inputs, tableau, dt = CASES["210_560"]
+    mech = _lib.Mechanism(desc)
+    cells = inputs(desc, 70, seed=3)
+    order = mech.species_order()
+    state = _lib.DeviceState(mech, cells.n_cells)
+    for threads, stage in ((96, -1), (2048, -1), (256, 5), (256, -2)):
+        with pytest.raises(_lib.MicmHipError):
+            state.set_cooperative_shape(threads, stage)
+    ref = _solve(mech, cells, order, tableau, dt, _lib.SOLVE_PLAIN)
+    run = _solve(mech, cells, order, tableau, dt, _lib.SOLVE_COOPERATIVE, coop_shape=(128, 4))
+    assert bits_equal(run[0], ref[0]) and np.array_equal(run[2], ref[2])
+
+
 @pytest.mark.parametrize("name", ["60_200", "210_560"])
 def test_repeated_solves_in_cost_order_agree_bitwise(name):
     """From the second Solve of a state on, the per-cell forms take cells in the order
