@@ -1684,17 +1684,19 @@ namespace
   }
 
   // Cells per workgroup of the tiled LU / substitution kernels: the largest power of
-  // two up to 16 whose tile of `doubles_per_cell` fits one workgroup's LDS; 0 when the
-  // level schedules are unavailable or one cell does not fit.
+  // two up to 16 whose tile of `doubles_per_cell` leaves room for two workgroups per CU
+  // (two independent workgroups hide each other's level latency: 210/560 LU 1.4x,
+  // 60/200 2.1x against one workgroup of twice the cells), else one cell if it fits;
+  // 0 when the level schedules are unavailable or one cell does not fit.
   int tile_cells(const micm_hip_mechanism* m, size_t doubles_per_cell)
   {
     if (!m->t.coop_levels_ok)
       return 0;
     const size_t limit = lds_limit();
     for (int tc = 16; tc >= 1; tc /= 2)
-      if (doubles_per_cell * tc * sizeof(double) <= limit)
+      if (2 * doubles_per_cell * tc * sizeof(double) <= limit)
         return tc;
-    return 0;
+    return doubles_per_cell * sizeof(double) <= limit ? 1 : 0;
   }
 
   // Cells per workgroup of the tiled forcing / Jacobian kernels: the largest power of two
