@@ -557,10 +557,10 @@ namespace
     out = ctl[c];
   }
 
-  // Stage s of one attempt for one lane (rosenbrock.inl:140-195).
-  __device__ inline void run_stage(const DevMech& m, const DevParams& p, int s, const Lane& y, const Lane& k,
-                                   const Lane& f0, const Lane& ynew, const Stages& K, const Lane& L, const Lane& U,
-                                   CellControl& c)
+  // Stage s of one attempt for one lane up to its linear solve: K_s holds the
+  // right-hand side (rosenbrock.inl:140-191).
+  __device__ inline void stage_rhs(const DevMech& m, const DevParams& p, int s, const Lane& y, const Lane& k,
+                                   const Lane& f0, const Lane& ynew, const Stages& K, CellControl& c)
   {
     const int n = m.n_species;
     const int sc = s * (s - 1) / 2;
@@ -603,8 +603,16 @@ namespace
         Ks[i] = v;
       }
     }
-    lu_solve<true>(m, L, U, Ks);
     c.stats[MICM_HIP_STAT_SOLVES] += 1;
+  }
+
+  // Stage s of one attempt for one lane (rosenbrock.inl:140-195).
+  __device__ inline void run_stage(const DevMech& m, const DevParams& p, int s, const Lane& y, const Lane& k,
+                                   const Lane& f0, const Lane& ynew, const Stages& K, const Lane& L, const Lane& U,
+                                   CellControl& c)
+  {
+    stage_rhs(m, p, s, y, k, f0, ynew, K, c);
+    lu_solve<true>(m, L, U, K[s]);
   }
 
   // Ynew = Y + sum m_s K_s, Yerr = sum e_s K_s; returns the error sum over species.
@@ -627,8 +635,8 @@ namespace
 
   // Start of an attempt: a new step (forcing + Jacobian) if due, then alpha and LU.
   // Returns false if the cell finished at the loop-top checks.
-  __device__ inline bool factor_attempt(const DevMech& m, const DevParams& p, const Lane& y, const Lane& k,
-                                        const Lane& f0, const Lane& J, const Lane& L, const Lane& U, CellControl& c)
+  __device__ inline bool prepare_attempt(const DevMech& m, const DevParams& p, const Lane& y, const Lane& k,
+                                         const Lane& f0, const Lane& J, CellControl& c)
   {
     if (c.flags & kNewStep)
     {
@@ -648,8 +656,16 @@ namespace
     const double alpha = next_alpha(p, c);
     for (int i = 0; i < m.n_species; ++i)
       J[m.jac_diag[i]] += alpha * 1.0;
-    lu_decompose(m, J, L, U);
     c.stats[MICM_HIP_STAT_DECOMPOSITIONS] += 1;
+    return true;
+  }
+
+  __device__ inline bool factor_attempt(const DevMech& m, const DevParams& p, const Lane& y, const Lane& k,
+                                        const Lane& f0, const Lane& J, const Lane& L, const Lane& U, CellControl& c)
+  {
+    if (!prepare_attempt(m, p, y, k, f0, J, c))
+      return false;
+    lu_decompose(m, J, L, U);
     return true;
   }
 
@@ -679,6 +695,44 @@ namespace
                    lane_view(w.f0, g, m.n_species, l), lane_view(w.jac, g, m.nnz_jac, l),
                    lane_view(w.lower, g, m.nnz_lower, l), lane_view(w.upper, g, m.nnz_upper, l), cc);
     ctl[c] = cc;
+  }
+
+  // With the level schedules available the factorisation and the substitutions run as
+  // the tiled kernels over every cell instead (finished cells' values are not used);
+  // k_plain_prepare and k_plain_rhs do the per-cell rest of k_plain_factor and
+  // k_plain_stage.
+  __global__ void __launch_bounds__(kBlock) k_plain_prepare(DevMech m, DevParams p, int64_t n_cells, Work w,
+                                                            CellControl* ctl)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n_cells)
+      return;
+    CellControl cc = ctl[c];
+    if (cc.status != MICM_HIP_STATE_RUNNING)
+      return;
+    const size_t g = c / kGroup;
+    const int l = c % kGroup;
+    prepare_attempt(m, p, lane_view(w.y, g, m.n_species, l), lane_view(w.k, g, m.n_reactions, l),
+                    lane_view(w.f0, g, m.n_species, l), lane_view(w.jac, g, m.nnz_jac, l), cc);
+    ctl[c] = cc;
+  }
+
+  __global__ void __launch_bounds__(kBlock) k_plain_rhs(DevMech m, DevParams p, int64_t n_cells, Work w, CellControl* ctl,
+                                                        int s)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n_cells)
+      return;
+    CellControl cc = ctl[c];
+    if (cc.status != MICM_HIP_STATE_RUNNING)
+      return;
+    const size_t g = c / kGroup;
+    const int l = c % kGroup;
+    const int n = m.n_species;
+    stage_rhs(m, p, s, lane_view(w.y, g, n, l), lane_view(w.k, g, m.n_reactions, l), lane_view(w.f0, g, n, l),
+              lane_view(w.ynew, g, n, l), stage_view(w, g, n, l), cc);
+    ctl[c].stats[MICM_HIP_STAT_FUNCTION_CALLS] = cc.stats[MICM_HIP_STAT_FUNCTION_CALLS];
+    ctl[c].stats[MICM_HIP_STAT_SOLVES] = cc.stats[MICM_HIP_STAT_SOLVES];
   }
 
   __global__ void __launch_bounds__(kBlock) k_plain_stage(DevMech m, DevParams p, int64_t n_cells, Work w, CellControl* ctl,
@@ -2540,12 +2594,32 @@ extern "C"
       // every attempt either finishes a cell or advances its step counter, and a cell
       // stops after max_number_of_steps + 1 attempts, so this bound is never reached
       const int64_t max_iterations = (params->max_number_of_steps + 2) * 64;
+      // the factorisation and substitutions as the LDS-tiled level-schedule kernels when
+      // the mechanism has them, else within the per-lane kernels
+      const bool tiled = tile_cells(m, static_cast<size_t>(m->coop.nnz_alu)) > 0 &&
+                         tile_cells(m, static_cast<size_t>(m->coop.nnz_alu) + 2 * static_cast<size_t>(d.n_species)) > 0;
       for (int64_t it = 0; it < max_iterations; ++it)
       {
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, s->active);
-        hipLaunchKernelGGL(k_plain_factor, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl);
-        for (int stage = 0; stage < p.stages; ++stage)
-          hipLaunchKernelGGL(k_plain_stage, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl, stage);
+        if (tiled)
+        {
+          hipLaunchKernelGGL(k_plain_prepare, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl);
+          if (int rc = micm_hip_lu_decompose(m, s->n_cells, w.jac, w.lower, w.upper, st))
+            return rc;
+          for (int stage = 0; stage < p.stages; ++stage)
+          {
+            hipLaunchKernelGGL(k_plain_rhs, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl, stage);
+            if (int rc = micm_hip_lu_solve(m, s->n_cells, w.lower, w.upper, w.K + static_cast<size_t>(stage) * w.k_stride,
+                                           st))
+              return rc;
+          }
+        }
+        else
+        {
+          hipLaunchKernelGGL(k_plain_factor, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl);
+          for (int stage = 0; stage < p.stages; ++stage)
+            hipLaunchKernelGGL(k_plain_stage, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl, stage);
+        }
         hipLaunchKernelGGL(k_plain_finish, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl, s->atol,
                            s->active, lockstep ? 1 : 0);
         if (lockstep)
