@@ -558,7 +558,10 @@ namespace
   }
 
   // Stage s of one attempt for one lane up to its linear solve: K_s holds the
-  // right-hand side (rosenbrock.inl:140-191).
+  // right-hand side (rosenbrock.inl:140-191).  PART 1 stops before the forcing of a
+  // new function evaluation (ynew formed, K_s zeroed), PART 2 is the rest after it,
+  // PART 3 both with the forcing in between.
+  template<int PART = 3>
   __device__ inline void stage_rhs(const DevMech& m, const DevParams& p, int s, const Lane& y, const Lane& k,
                                    const Lane& f0, const Lane& ynew, const Stages& K, CellControl& c)
   {
@@ -567,23 +570,30 @@ namespace
     const Lane Ks = K[s];
     if (s == 0)
     {
-      for (int i = 0; i < n; ++i)
-        Ks[i] = f0[i];
+      if constexpr ((PART & 2) != 0)
+        for (int i = 0; i < n; ++i)
+          Ks[i] = f0[i];
     }
     else if (p.new_f[s])
     {
-      for (int i = 0; i < n; ++i)
+      if constexpr ((PART & 1) != 0)
       {
-        double v = y[i];
-        for (int j = 0; j < s; ++j)
-          v += p.a[sc + j] * K[j][i];
-        ynew[i] = v;
+        for (int i = 0; i < n; ++i)
+        {
+          double v = y[i];
+          for (int j = 0; j < s; ++j)
+            v += p.a[sc + j] * K[j][i];
+          ynew[i] = v;
+        }
+        for (int i = 0; i < n; ++i)
+          Ks[i] = 0.0;
+        c.stats[MICM_HIP_STAT_FUNCTION_CALLS] += 1;
       }
-      for (int i = 0; i < n; ++i)
-        Ks[i] = 0.0;
-      add_forcing(m, k, ynew, Ks);
-      c.stats[MICM_HIP_STAT_FUNCTION_CALLS] += 1;
+      if constexpr (PART == 3)
+        add_forcing(m, k, ynew, Ks);
     }
+    if constexpr ((PART & 2) == 0)
+      return;
     if (s + 1 < p.stages && !p.new_f[s + 1])
     {
       const Lane Kn = K[s + 1];
@@ -635,6 +645,8 @@ namespace
 
   // Start of an attempt: a new step (forcing + Jacobian) if due, then alpha and LU.
   // Returns false if the cell finished at the loop-top checks.
+  // FORCING false: f(y) is left to the caller (f0 zeroed here).
+  template<bool FORCING = true>
   __device__ inline bool prepare_attempt(const DevMech& m, const DevParams& p, const Lane& y, const Lane& k,
                                          const Lane& f0, const Lane& J, CellControl& c)
   {
@@ -644,7 +656,8 @@ namespace
         return false;
       for (int i = 0; i < m.n_species; ++i)
         f0[i] = 0.0;
-      add_forcing(m, k, y, f0);
+      if constexpr (FORCING)
+        add_forcing(m, k, y, f0);
       c.stats[MICM_HIP_STAT_FUNCTION_CALLS] += 1;
       for (int i = 0; i < m.nnz_jac; ++i)
         J[i] = 0.0;
@@ -697,10 +710,12 @@ namespace
     ctl[c] = cc;
   }
 
-  // With the level schedules available the factorisation and the substitutions run as
-  // the tiled kernels over every cell instead (finished cells' values are not used);
-  // k_plain_prepare and k_plain_rhs do the per-cell rest of k_plain_factor and
-  // k_plain_stage.
+  // With the level schedules available the forcing, the factorisation and the
+  // substitutions run as the tiled kernels over every cell instead (finished cells'
+  // values are not used); k_plain_prepare and k_plain_rhs do the per-cell rest of
+  // k_plain_factor and k_plain_stage.  f(y) is evaluated for every running cell at
+  // every attempt: y is unchanged since the cell's step began, so it is the same value
+  // the step started with.
   __global__ void __launch_bounds__(kBlock) k_plain_prepare(DevMech m, DevParams p, int64_t n_cells, Work w,
                                                             CellControl* ctl)
   {
@@ -712,11 +727,15 @@ namespace
       return;
     const size_t g = c / kGroup;
     const int l = c % kGroup;
-    prepare_attempt(m, p, lane_view(w.y, g, m.n_species, l), lane_view(w.k, g, m.n_reactions, l),
-                    lane_view(w.f0, g, m.n_species, l), lane_view(w.jac, g, m.nnz_jac, l), cc);
+    const Lane f0 = lane_view(w.f0, g, m.n_species, l);
+    for (int i = 0; i < m.n_species; ++i)
+      f0[i] = 0.0;
+    prepare_attempt<false>(m, p, lane_view(w.y, g, m.n_species, l), lane_view(w.k, g, m.n_reactions, l), f0,
+                           lane_view(w.jac, g, m.nnz_jac, l), cc);
     ctl[c] = cc;
   }
 
+  template<int PART>
   __global__ void __launch_bounds__(kBlock) k_plain_rhs(DevMech m, DevParams p, int64_t n_cells, Work w, CellControl* ctl,
                                                         int s)
   {
@@ -729,8 +748,8 @@ namespace
     const size_t g = c / kGroup;
     const int l = c % kGroup;
     const int n = m.n_species;
-    stage_rhs(m, p, s, lane_view(w.y, g, n, l), lane_view(w.k, g, m.n_reactions, l), lane_view(w.f0, g, n, l),
-              lane_view(w.ynew, g, n, l), stage_view(w, g, n, l), cc);
+    stage_rhs<PART>(m, p, s, lane_view(w.y, g, n, l), lane_view(w.k, g, m.n_reactions, l), lane_view(w.f0, g, n, l),
+                    lane_view(w.ynew, g, n, l), stage_view(w, g, n, l), cc);
     ctl[c].stats[MICM_HIP_STAT_FUNCTION_CALLS] = cc.stats[MICM_HIP_STAT_FUNCTION_CALLS];
     ctl[c].stats[MICM_HIP_STAT_SOLVES] = cc.stats[MICM_HIP_STAT_SOLVES];
   }
@@ -2604,13 +2623,21 @@ extern "C"
         if (tiled)
         {
           hipLaunchKernelGGL(k_plain_prepare, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl);
+          if (int rc = micm_hip_add_forcing_terms(m, s->n_cells, w.k, w.y, w.f0, st))
+            return rc;
           if (int rc = micm_hip_lu_decompose(m, s->n_cells, w.jac, w.lower, w.upper, st))
             return rc;
           for (int stage = 0; stage < p.stages; ++stage)
           {
-            hipLaunchKernelGGL(k_plain_rhs, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl, stage);
-            if (int rc = micm_hip_lu_solve(m, s->n_cells, w.lower, w.upper, w.K + static_cast<size_t>(stage) * w.k_stride,
-                                           st))
+            double* Ks = w.K + static_cast<size_t>(stage) * w.k_stride;
+            if (stage > 0 && p.new_f[stage])
+            {
+              hipLaunchKernelGGL(k_plain_rhs<1>, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl, stage);
+              if (int rc = micm_hip_add_forcing_terms(m, s->n_cells, w.k, w.ynew, Ks, st))
+                return rc;
+            }
+            hipLaunchKernelGGL(k_plain_rhs<2>, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl, stage);
+            if (int rc = micm_hip_lu_solve(m, s->n_cells, w.lower, w.upper, Ks, st))
               return rc;
           }
         }
