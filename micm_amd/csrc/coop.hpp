@@ -1,15 +1,16 @@
-// coop.hpp -- the cooperative form: one workgroup (NT = 64, 128 or 256 threads)
-// integrates one cell with the cell's whole working set (in-place LU, Jacobian,
-// stage vectors) in LDS.
+// coop.hpp -- the cooperative form: one workgroup (NT = 64 to 1024 threads)
+// integrates one cell with the cell's working set (in-place LU, Jacobian, stage
+// vectors) in LDS; the compact layout keeps only the gather sources there.
 //
 // Work inside a cell is spread over the threads without changing any rounding:
 //   forcing   per-species gather, terms in the reference's scatter order
 //             (process_set.hpp:541-579)
 //   -J        per-element gather, terms in the reference's order (process_set.hpp:613-659)
-//   LU        level-scheduled Doolittle: one task per element of the fill pattern,
-//             subtracting L[i][j] U[j][k] for j ascending and dividing L elements
-//             by U[k][k], as in lu_decomposition_doolittle.inl:347-398; a task runs
-//             once every position it reads was written at a lower level
+//   LU        level-scheduled Doolittle: each element of the fill pattern subtracts
+//             L[i][j] U[j][k] for j ascending and divides L elements by U[k][k], as
+//             in lu_decomposition_doolittle.inl:347-398, its sum split into segments
+//             that run as soon as their operands are final (the partial sum kept in
+//             the element's position)
 //   forward   level-scheduled rows, x_i -= L_ij x_j for j ascending (linear_solver.inl:250-265)
 //   backward  level-scheduled rows, x_i -= U_ij x_j for j ascending, then / U_ii
 //             (linear_solver.inl:270-287)

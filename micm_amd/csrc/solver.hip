@@ -11,6 +11,8 @@
 //   PLAIN      -- per-cell step size; one launch per kernel per stage, host loop.
 //   PERSISTENT -- per-cell step size; one persistent launch whose wavefronts take
 //                 cells from a work queue and carry each to the end of its step.
+//   COOPERATIVE -- per-cell step size; one workgroup per cell (coop.hpp), cells from
+//                 a work queue, in the previous Solve's cost order once there was one.
 #include "coop.hpp"
 #include "device.hpp"
 #include "mechanism.hpp"
