@@ -15,8 +15,9 @@ Prints one JSON line.  `roofline` relates the algorithmic bytes of the path --
 per attempt F x forcing + Jacobian + LU + S x substitution + error, with the
 per-cell byte counts of each kernel -- summed over the attempts the cells
 actually took, to the measured time of the Solve launches (HIP events on the
-solver's stream).  `plain_form` times the plain per-step form and the chosen
-form on the same strided sample of cells.  `end_to_end` includes the PCIe
+solver's stream); `fp64` does the same with the FP64 operation count.
+`plain_form` times the plain per-step form and the chosen form on the same
+strided sample of cells.  `end_to_end` includes the PCIe
 transfers of inputs and results.  `cpu_baseline` times the reference's own CPU
 solver (or the CPU restatement, where the reference build is absent) on a
 bounded sample of the same cells with one 1-cell State per cell, on every host
@@ -35,6 +36,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 
 
 def algorithmic_bytes(info, n_species, n_reactions, stats):
@@ -50,6 +52,46 @@ def algorithmic_bytes(info, n_species, n_reactions, stats):
     error = 8 * 3 * N
     s = stats.sum(axis=0)
     return int(s[0] * forcing + s[1] * jacobian + s[5] * lu + s[6] * solve + s[2] * error)
+
+
+def flops_per_invocation(desc, mech):
+    """FP64 operations of one invocation of each phase, counted from the mechanism:
+    forcing k * prod(y) plus one update per reactant and a multiply-add per product
+    (process_set.hpp:524-590); the Jacobian's per-independent-reactant product and
+    updates (:594-670); Doolittle's multiply-subtracts (one per L[i,j] * U[j,k] pair
+    with j below both indices of the filled element) and divisions plus the alpha
+    shift (lu_decomposition_doolittle.inl:325-403); both substitutions with their
+    diagonal divisions (linear_solver.inl:235-294); the error norm's ~7 operations
+    per species.  Stage combinations (a few N per stage) are left out."""
+    nr = np.diff(desc.reactant_offsets)
+    npr = np.diff(desc.product_offsets)
+    N = mech.n_species
+    pat = mech.patterns()
+    lr, lc = pat["lower"]
+    ur, uc = pat["upper"]
+    strict_l = lr[: mech.nnz_lower] > lc[: mech.nnz_lower]
+    strict_u = ur[: mech.nnz_upper] < uc[: mech.nnz_upper]
+    lb = np.zeros((N, N), dtype=np.int64)
+    ub = np.zeros((N, N), dtype=np.int64)
+    lb[lr[: mech.nnz_lower][strict_l], lc[: mech.nnz_lower][strict_l]] = 1
+    ub[ur[: mech.nnz_upper][strict_u], uc[: mech.nnz_upper][strict_u]] = 1
+    pairs = lb @ ub  # pairs[i, k] = #{j < min(i, k): L[i, j] != 0 and U[j, k] != 0}
+    filled = (lb + ub) > 0
+    fma = int(pairs[filled].sum()) + int(np.trace(pairs))
+    n_l, n_u = int(strict_l.sum()), int(strict_u.sum())
+    return {"forcing": int(np.sum(2 * nr + 2 * npr)),
+            "jacobian": int(np.sum(nr * (2 * nr - 1 + 2 * npr))),
+            "lu": 2 * fma + n_l + N, "lu_multiply_subtracts": fma, "lu_divisions": n_l,
+            "solve": 2 * (n_l + n_u) + 2 * N,
+            "error": 7 * N}
+
+
+def algorithmic_flops(per, stats):
+    """FP64 operations summed over the invocations the cells performed (stats columns
+    as in algorithmic_bytes)."""
+    s = stats.sum(axis=0)
+    return int(s[0] * per["forcing"] + s[1] * per["jacobian"] + s[5] * per["lu"] + s[6] * per["solve"] +
+               s[2] * per["error"])
 
 
 def cpu_baseline(desc, cells, order, dt, sample, threads, tableau=4):
@@ -314,6 +356,8 @@ def main():
 
     bytes_per_step = algorithmic_bytes(mech.info, mech.n_species, mech.n_reactions, stats)
     achieved = bytes_per_step / (kernel_ms * 1e-3) / 1e9
+    flops_per_step = algorithmic_flops(flops_per_invocation(desc, mech), stats)
+    tflops = flops_per_step / (kernel_ms * 1e-3) / 1e12
     traffic, traffic_source = measured_traffic(args.workload, n_cells, coop_threads) if coop_threads else (None, None)
     result = {
         "metric": {"210_560": "cell-solves/s, 210-species x 100k cells, batched Rosenbrock (FP64)",
@@ -339,6 +383,8 @@ def main():
                      "traffic_source": traffic_source,
                      "kernel": "the Solve launch (one persistent or cooperative kernel per step); algorithmic bytes "
                                f"per step {bytes_per_step} summed over the attempts the cells took"},
+        "fp64": {"achieved": round(tflops, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "frac": round(tflops / FP64_PEAK_TFLOPS, 6), "flops_per_step": flops_per_step},
         "attempts_per_step": int(stats[:, 2].sum()),
         "converged_fraction": round(float(np.mean(status == 2)), 4),
         "update_state_parameters_ms": round(update_ms, 3),

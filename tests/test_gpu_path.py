@@ -232,6 +232,39 @@ def test_per_cell_forms_agree_bitwise(name):
         assert bits_equal(r[3], ref[3])
 
 
+def test_empty_and_negative_cell_counts():
+    """Every kernel entry point accepts zero cells as a no-op that touches no buffer
+    and rejects a negative count; a State needs at least one cell."""
+    desc = synthetic.readme_mechanism()
+    mech = _lib.Mechanism(desc)
+    i = mech.info
+    sizes = {"k": mech.n_reactions, "y": mech.n_species, "f": mech.n_species, "j": i.nnz_jacobian,
+             "l": i.nnz_lower, "u": i.nnz_upper, "c": 3, "p": max(mech.n_custom, 1), "e": 1}
+    buf = {key: dev_zeros(64, n) + 7.0 for key, n in sizes.items()}
+    p = {key: t.data_ptr() for key, t in buf.items()}
+    s = stream()
+    calls = [
+        lambda n: mech.calculate_rate_constants(n, p["c"], p["p"], p["k"], s),
+        lambda n: mech.add_forcing_terms(n, p["k"], p["y"], p["f"], s),
+        lambda n: mech.subtract_jacobian_terms(n, p["k"], p["y"], p["j"], s),
+        lambda n: mech.alpha_minus_jacobian(n, p["j"], 2.0, s),
+        lambda n: mech.lu_decompose(n, p["j"], p["l"], p["u"], s),
+        lambda n: mech.lu_solve(n, p["l"], p["u"], p["f"], s),
+        lambda n: mech.normalized_error(n, p["y"], p["y"], p["f"], p["y"], 1e-6, p["e"], s),
+    ]
+    for call in calls:
+        call(0)
+        with pytest.raises(_lib.MicmHipError):
+            call(-1)
+    import torch
+    torch.cuda.synchronize()
+    for key, t in buf.items():
+        assert bool((t == 7.0).all()), key
+    for n in (0, -5):
+        with pytest.raises(_lib.MicmHipError):
+            _lib.DeviceState(mech, n)
+
+
 def test_cooperative_shape_arguments_checked():
     """Launch-shape requests outside the supported set fail with an error; a compact
     request the mechanism cannot take (fewer threads than species) falls back to the

@@ -155,3 +155,38 @@ def test_ts1_file_matches_the_reference_benchmark():
     m = _lib.Mechanism(desc)
     info = m.info
     assert (info.nnz_jacobian, info.nnz_lower, info.nnz_upper) == (1950, 1453, 1137)
+
+
+def _doolittle_operation_count(lower, upper, n):
+    """Walks the in-place Doolittle elimination of the filled pattern element by element
+    (row i: U[i, k] for k >= i, then L[k, i] for k > i) and counts the products
+    L[i, j] * U[j, k] it forms and the divisions by U[i, i]."""
+    nz = set(zip(*lower)) | set(zip(*upper))
+    fma = div = 0
+    for i in range(n):
+        for k in range(i, n):
+            if (i, k) in nz:
+                fma += sum(1 for j in range(i) if (i, j) in nz and (j, k) in nz)
+        for k in range(i + 1, n):
+            if (k, i) in nz:
+                fma += sum(1 for j in range(i) if (k, j) in nz and (j, i) in nz)
+                div += 1
+    return fma, div
+
+
+def test_fp64_operation_count():
+    """bench.py's FP64 operation count: the Doolittle multiply-subtracts and divisions
+    agree with an element-by-element walk of the elimination, and for TS1 with the
+    counts measured on the reference's own factorisation (6,986 and 1,244)."""
+    import bench
+    from oracle import Oracle
+    o = Oracle(synthetic.mechanism_60_200(), 3)
+    pat = o.patterns()
+    per = bench.flops_per_invocation(o.desc, o)
+    assert (per["lu_multiply_subtracts"], per["lu_divisions"]) == \
+        _doolittle_operation_count(pat["lower"], pat["upper"], o.n_species)
+    t = Oracle(ts1_mechanism(), 3)
+    per = bench.flops_per_invocation(t.desc, t)
+    assert (per["lu_multiply_subtracts"], per["lu_divisions"]) == (6986, 1244)
+    assert per["lu"] == 2 * 6986 + 1244 + 209
+    assert per["solve"] == 2 * ((1453 - 209) + (1137 - 209)) + 2 * 209
