@@ -126,6 +126,20 @@ def cpu_baseline(desc, cells, order, dt, sample, threads, tableau=4):
     return kind, sample / el, el
 
 
+def cpu_lockstep(desc, sub, order, dt, tableau):
+    """Cell-solves/s of the reference CPU solver with all of `sub`'s cells in one
+    State (one step size for every cell; a State runs on one core)."""
+    from oracle import Oracle, Reference, reference_available
+    kind = "reference" if reference_available() else "port"
+    solver = (Reference if kind == "reference" else Oracle)(desc, tableau)
+    if kind == "port":
+        solver.lib.oracle_set_threads(1)
+    t0 = time.perf_counter()
+    solver.solve(sub.conditions, sub.custom, sub.variables[:, order], dt, 1, lockstep=True)
+    el = time.perf_counter() - t0
+    return kind, sub.n_cells / el, el
+
+
 def kernel_rooflines(mech, state, stream, n_cells, repeats=5):
     """Per-kernel time of the single-phase entry points (one cell per lane) over the
     whole State, and their algorithmic bytes as a fraction of the HBM peak."""
@@ -395,6 +409,7 @@ def main():
         result["autotune"] = {k: (round(v * 1e3, 2) if isinstance(v, float) else v)
                               for k, v in choice["seconds"].items()}
         result["autotune_sample_cells"] = choice["sample"]
+    lock_cells = None
     if args.compare_sample > 0:
         m = min(args.compare_sample, n_cells)
         sub = cells.subset(np.linspace(0, n_cells - 1, m).round().astype(np.int64))
@@ -419,7 +434,7 @@ def main():
         # the reference's own multi-cell State: one step size for every cell
         ml = min(m, 512)
         lock = _lib.DeviceState(mech, ml)
-        subl = sub.subset(np.arange(ml))
+        subl = lock_cells = sub.subset(np.arange(ml))
         lock.set_variables(subl.variables[:, order])
         lock.set_conditions(subl.conditions)
         lock.set_custom_parameters(subl.custom)
@@ -450,6 +465,11 @@ def main():
                                             f"cell, {el:.1f} s wall on {threads} cores",
                                   "single_core_value": round(cps1, 2),
                                   "single_core_sample": f"first {sample1} cells, {el1:.1f} s"}
+        if args.workload == "readme" and lock_cells is not None:
+            kind_l, cps_l, el_l = cpu_lockstep(desc, lock_cells, order, dt, tableau)
+            result["cpu_baseline"]["lockstep"] = {
+                "value": round(cps_l, 2), "unit": "cell-solves/s", "cores": 1, "kind": kind_l,
+                "sample": f"the lockstep form's {lock_cells.n_cells} cells in one State, {el_l:.3f} s"}
     print(json.dumps(result), flush=True)
 
 

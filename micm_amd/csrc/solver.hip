@@ -886,6 +886,109 @@ namespace
   // Lockstep loop-top: all cells share one control state, so a cell that stops at the
   // loop-top checks stops them all; the factor kernel handles it per cell identically.
 
+  // Lockstep form of a small State in one workgroup: per attempt, every cell's
+  // factorisation, stages and error terms (k_plain_factor, k_plain_stage,
+  // k_plain_finish), the State's error sum in the reference's order
+  // (k_lockstep_error) and the shared step control (k_lockstep_control),
+  // separated by workgroup barriers instead of kernel launches and a host check per
+  // attempt.  Same operations in the same order, so the results are those of the
+  // launch-per-phase loop.  Every exit condition and every barrier is uniform over
+  // the workgroup.
+  constexpr int kFusedLockstepThreads = 1024;
+  constexpr int64_t kFusedLockstepMaxCells = kFusedLockstepThreads;
+  constexpr int64_t kFusedLockstepMaxFactor = 256;  // nnz of the in-place factor
+
+  __global__ void __launch_bounds__(kFusedLockstepThreads) k_lockstep_fused(DevMech m, DevParams p, int64_t n_cells,
+                                                                            Work w, CellControl* ctl,
+                                                                            const double* __restrict__ atol,
+                                                                            int64_t max_iterations)
+  {
+    __shared__ double s_error;
+    __shared__ double s_terms[kFusedLockstepThreads];
+    const int n = m.n_species;
+    for (int64_t it = 0; it < max_iterations; ++it)
+    {
+      for (int64_t c = threadIdx.x; c < n_cells; c += kFusedLockstepThreads)
+      {
+        CellControl cc = ctl[c];
+        if (cc.status != MICM_HIP_STATE_RUNNING)
+          continue;
+        const size_t g = c / kGroup;
+        const int l = c % kGroup;
+        const Lane y = lane_view(w.y, g, n, l), k = lane_view(w.k, g, m.n_reactions, l);
+        const Lane f0 = lane_view(w.f0, g, n, l), ynew = lane_view(w.ynew, g, n, l);
+        const Lane L = lane_view(w.lower, g, m.nnz_lower, l), U = lane_view(w.upper, g, m.nnz_upper, l);
+        const Stages K = stage_view(w, g, n, l);
+        if (factor_attempt(m, p, y, k, f0, lane_view(w.jac, g, m.nnz_jac, l), L, U, cc))
+        {
+          for (int s = 0; s < p.stages; ++s)
+            run_stage(m, p, s, y, k, f0, ynew, K, L, U, cc);
+          combine_and_error(m, p, y, ynew, lane_view(w.yerr, g, n, l), K, atol);
+        }
+        ctl[c] = cc;
+      }
+      __syncthreads();
+      // the cells share one control state: all stopped at the loop-top checks or none
+      if (ctl[0].status != MICM_HIP_STATE_RUNNING)
+        break;
+      // terms of (cell, species) pairs in the reference's order (cells outer, species
+      // inner), formed by all threads a block at a time, summed one after another by
+      // thread 0
+      const int64_t n_terms = n_cells * n;
+      double acc = 0.0;
+      for (int64_t base = 0; base < n_terms; base += kFusedLockstepThreads)
+      {
+        const int64_t f = base + threadIdx.x;
+        double q = 0.0;
+        if (f < n_terms)
+        {
+          const int64_t c = f / n;
+          const int i = static_cast<int>(f - c * n);
+          const size_t g = c / kGroup;
+          const int cl = c % kGroup;
+          const double a = fabs(lane_view(w.y, g, n, cl)[i]), b = fabs(lane_view(w.ynew, g, n, cl)[i]);
+          const double ymax = a > b ? a : b;
+          const double v = lane_view(w.yerr, g, n, cl)[i] / (atol[i] + p.rtol * ymax);
+          q = v * v;
+        }
+        s_terms[threadIdx.x] = q;
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+          const int cnt = static_cast<int>(n_terms - base < kFusedLockstepThreads ? n_terms - base
+                                                                                  : kFusedLockstepThreads);
+          for (int j = 0; j < cnt; ++j)
+            acc += s_terms[j];
+        }
+        __syncthreads();
+      }
+      if (threadIdx.x == 0)
+      {
+        const double N = static_cast<double>(n_terms > 0 ? n_terms : 1);
+        s_error = smax(sqrt(acc / N), 1.0e-10);
+      }
+      __syncthreads();
+      const double error = s_error;
+      int running = 0;
+      for (int64_t c = threadIdx.x; c < n_cells; c += kFusedLockstepThreads)
+      {
+        CellControl cc = ctl[c];
+        if (cc.status != MICM_HIP_STATE_RUNNING)
+          continue;
+        const size_t g = c / kGroup;
+        const int l = c % kGroup;
+        Lane y = lane_view(w.y, g, n, l), ynew = lane_view(w.ynew, g, n, l);
+        if (control_update(p, cc, error))
+          for (int i = 0; i < n; ++i)
+            y[i] = ynew[i];
+        ctl[c] = cc;
+        running |= cc.status == MICM_HIP_STATE_RUNNING ? 1 : 0;
+      }
+      if (!__syncthreads_or(running))
+        break;
+    }
+  }
+
   // Post-solve clamp (solver.hpp:275-297) and per-cell results.
   __global__ void __launch_bounds__(kBlock) k_finalize(int n_species, int64_t n_cells, double* y, CellControl* ctl,
                                                        int32_t* status, int64_t* stats, double* final_time)
@@ -2619,7 +2722,16 @@ extern "C"
       // the mechanism has them, else within the per-lane kernels
       const bool tiled = tile_cells(m, static_cast<size_t>(m->coop.nnz_alu)) > 0 &&
                          tile_cells(m, static_cast<size_t>(m->coop.nnz_alu) + 2 * static_cast<size_t>(d.n_species)) > 0;
-      for (int64_t it = 0; it < max_iterations; ++it)
+      // a small State of a small mechanism: the whole lockstep loop in one workgroup
+      const bool fused = lockstep && s->n_cells <= kFusedLockstepMaxCells &&
+                         static_cast<int64_t>(d.nnz_lower) + d.nnz_upper - d.n_species <= kFusedLockstepMaxFactor;
+      if (fused)
+      {
+        hipLaunchKernelGGL(k_lockstep_fused, dim3(1), dim3(kFusedLockstepThreads), 0, st, d, p, s->n_cells, w, s->ctl,
+                           s->atol, max_iterations);
+        HIP_TRY(hipGetLastError());
+      }
+      for (int64_t it = 0; !fused && it < max_iterations; ++it)
       {
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, s->active);
         if (tiled)

@@ -331,15 +331,17 @@ def test_fused_rate_constants_match_separate_update(form, coop_shape):
     assert bits_equal(state.get_rate_constants(), other.get_rate_constants())
 
 
-@pytest.mark.parametrize("name", ["all_types", "60_200"])
-def test_lockstep_solve_bitwise(name):
+@pytest.mark.parametrize("name,n_cells", [("all_types", 130), ("60_200", 130), ("readme", 1024), ("readme", 1500)])
+def test_lockstep_solve_bitwise(name, n_cells):
     """One step size for all cells (one multi-cell State): bitwise with the portable-math
-    restatement, including the global error norm's summation order."""
+    restatement, including the global error norm's summation order.  States of up to
+    1,024 cells of a small mechanism (all_types, readme at 1,024) run the whole loop in
+    one workgroup; the others launch a kernel per phase."""
     desc, inputs, tableau, dt = CASES[name]
     mech = _lib.Mechanism(desc)
-    cells = inputs(desc, 130)
+    cells = inputs(desc, n_cells)
     order = mech.species_order()
-    print(f"lockstep bitwise: {name}", flush=True)
+    print(f"lockstep bitwise: {name} n_cells={n_cells}", flush=True)
     y_gpu, st_gpu, stats_gpu, ft_gpu = _solve(mech, cells, order, tableau, dt, _lib.SOLVE_LOCKSTEP, n_solves=2)
     y_ref, st_ref, stats_ref, ft_ref = OraclePortableMath(desc, tableau).solve(
         cells.conditions, cells.custom, cells.variables[:, order], dt, 2, lockstep=True)
