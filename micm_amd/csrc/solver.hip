@@ -814,48 +814,61 @@ namespace
     }
   }
 
+  constexpr int kLockstepErrorThreads = 1024;
+
   // Lockstep error: the reference's single running sum over cells (outer) and
-  // species (inner) of one State (rosenbrock.inl:372-401), by one wavefront.
-  __global__ void __launch_bounds__(kGroup) k_lockstep_error(DevMech m, DevParams p, int64_t n_cells, Work w,
-                                                             const CellControl* ctl, const double* __restrict__ atol,
-                                                             double* error_out)
+  // species (inner) of one State (rosenbrock.inl:372-401).  The terms of the
+  // flattened (cell, species) pairs are formed by the whole workgroup a block at a
+  // time and added one after another by thread 0, in that order.  Every thread of
+  // the workgroup calls it; the sum is thread 0's result.
+  __device__ inline double lockstep_error(const DevMech& m, const DevParams& p, int64_t n_cells, const Work& w,
+                                          const double* __restrict__ atol, double* terms)
   {
-    __shared__ double q[kGroup];
-    const int l = threadIdx.x;
     const int n = m.n_species;
+    const int64_t n_terms = n_cells * n;
     double acc = 0.0;
+    for (int64_t base = 0; base < n_terms; base += kLockstepErrorThreads)
+    {
+      const int64_t f = base + threadIdx.x;
+      double q = 0.0;
+      if (f < n_terms)
+      {
+        const int64_t c = f / n;
+        const int i = static_cast<int>(f - c * n);
+        const size_t g = c / kGroup;
+        const int cl = c % kGroup;
+        const double a = fabs(lane_view(w.y, g, n, cl)[i]), b = fabs(lane_view(w.ynew, g, n, cl)[i]);
+        const double ymax = a > b ? a : b;
+        const double v = lane_view(w.yerr, g, n, cl)[i] / (atol[i] + p.rtol * ymax);
+        q = v * v;
+      }
+      terms[threadIdx.x] = q;
+      __syncthreads();
+      if (threadIdx.x == 0)
+      {
+        const int cnt = static_cast<int>(n_terms - base < kLockstepErrorThreads ? n_terms - base
+                                                                                : kLockstepErrorThreads);
+        for (int j = 0; j < cnt; ++j)
+          acc += terms[j];
+      }
+      __syncthreads();
+    }
+    const double N = static_cast<double>(n_terms > 0 ? n_terms : 1);
+    return smax(sqrt(acc / N), 1.0e-10);
+  }
+
+  __global__ void __launch_bounds__(kLockstepErrorThreads) k_lockstep_error(DevMech m, DevParams p, int64_t n_cells,
+                                                                            Work w, const CellControl* ctl,
+                                                                            const double* __restrict__ atol,
+                                                                            double* error_out)
+  {
+    __shared__ double terms[kLockstepErrorThreads];
+    // the cells share one control state: all stopped at the loop-top checks or none
     if (ctl[0].status != MICM_HIP_STATE_RUNNING)
       return;
-    for (int64_t c = 0; c < n_cells; ++c)
-    {
-      const size_t g = c / kGroup;
-      const int cl = c % kGroup;
-      Lane y = lane_view(w.y, g, n, cl), ynew = lane_view(w.ynew, g, n, cl), yerr = lane_view(w.yerr, g, n, cl);
-      for (int i0 = 0; i0 < n; i0 += kGroup)
-      {
-        const int i = i0 + l;
-        if (i < n)
-        {
-          const double a = fabs(y[i]), b = fabs(ynew[i]);
-          const double ymax = a > b ? a : b;
-          const double v = yerr[i] / (atol[i] + p.rtol * ymax);
-          q[l] = v * v;
-        }
-        __syncthreads();
-        if (l == 0)
-        {
-          const int cnt = min(kGroup, n - i0);
-          for (int j = 0; j < cnt; ++j)
-            acc += q[j];
-        }
-        __syncthreads();
-      }
-    }
-    if (l == 0)
-    {
-      const double N = static_cast<double>(n_cells * n > 0 ? n_cells * n : 1);
-      *error_out = smax(sqrt(acc / N), 1.0e-10);
-    }
+    const double e = lockstep_error(m, p, n_cells, w, atol, terms);
+    if (threadIdx.x == 0)
+      *error_out = e;
   }
 
   __global__ void __launch_bounds__(kBlock) k_lockstep_control(DevMech m, DevParams p, int64_t n_cells, Work w,
@@ -894,7 +907,7 @@ namespace
   // attempt.  Same operations in the same order, so the results are those of the
   // launch-per-phase loop.  Every exit condition and every barrier is uniform over
   // the workgroup.
-  constexpr int kFusedLockstepThreads = 1024;
+  constexpr int kFusedLockstepThreads = kLockstepErrorThreads;
   constexpr int64_t kFusedLockstepMaxCells = kFusedLockstepThreads;
   constexpr int64_t kFusedLockstepMaxFactor = 256;  // nnz of the in-place factor
 
@@ -931,41 +944,10 @@ namespace
       // the cells share one control state: all stopped at the loop-top checks or none
       if (ctl[0].status != MICM_HIP_STATE_RUNNING)
         break;
-      // terms of (cell, species) pairs in the reference's order (cells outer, species
-      // inner), formed by all threads a block at a time, summed one after another by
-      // thread 0
-      const int64_t n_terms = n_cells * n;
-      double acc = 0.0;
-      for (int64_t base = 0; base < n_terms; base += kFusedLockstepThreads)
       {
-        const int64_t f = base + threadIdx.x;
-        double q = 0.0;
-        if (f < n_terms)
-        {
-          const int64_t c = f / n;
-          const int i = static_cast<int>(f - c * n);
-          const size_t g = c / kGroup;
-          const int cl = c % kGroup;
-          const double a = fabs(lane_view(w.y, g, n, cl)[i]), b = fabs(lane_view(w.ynew, g, n, cl)[i]);
-          const double ymax = a > b ? a : b;
-          const double v = lane_view(w.yerr, g, n, cl)[i] / (atol[i] + p.rtol * ymax);
-          q = v * v;
-        }
-        s_terms[threadIdx.x] = q;
-        __syncthreads();
+        const double e = lockstep_error(m, p, n_cells, w, atol, s_terms);
         if (threadIdx.x == 0)
-        {
-          const int cnt = static_cast<int>(n_terms - base < kFusedLockstepThreads ? n_terms - base
-                                                                                  : kFusedLockstepThreads);
-          for (int j = 0; j < cnt; ++j)
-            acc += s_terms[j];
-        }
-        __syncthreads();
-      }
-      if (threadIdx.x == 0)
-      {
-        const double N = static_cast<double>(n_terms > 0 ? n_terms : 1);
-        s_error = smax(sqrt(acc / N), 1.0e-10);
+          s_error = e;
       }
       __syncthreads();
       const double error = s_error;
@@ -2765,7 +2747,7 @@ extern "C"
                            s->active, lockstep ? 1 : 0);
         if (lockstep)
         {
-          hipLaunchKernelGGL(k_lockstep_error, dim3(1), dim3(kGroup), 0, st, d, p, s->n_cells, w, s->ctl, s->atol,
+          hipLaunchKernelGGL(k_lockstep_error, dim3(1), dim3(kLockstepErrorThreads), 0, st, d, p, s->n_cells, w, s->ctl, s->atol,
                              s->lockstep_error);
           hipLaunchKernelGGL(k_lockstep_control, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl,
                              s->lockstep_error, s->active);
