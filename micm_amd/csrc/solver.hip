@@ -776,11 +776,9 @@ namespace
     ctl[c].stats[MICM_HIP_STAT_SOLVES] = cc.stats[MICM_HIP_STAT_SOLVES];
   }
 
-  // Combination, error and accept/reject.  lockstep: only the error terms are
-  // formed here; k_lockstep_error and k_lockstep_control finish the attempt.
+  // Combination, error and accept/reject of the per-cell step control.
   __global__ void __launch_bounds__(kBlock) k_plain_finish(DevMech m, DevParams p, int64_t n_cells, Work w, CellControl* ctl,
-                                                           const double* __restrict__ atol, unsigned int* active,
-                                                           int lockstep)
+                                                           const double* __restrict__ atol, unsigned int* active)
   {
     const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     bool running = false;
@@ -795,23 +793,40 @@ namespace
         const Stages K = stage_view(w, g, n, l);
         Lane y = lane_view(w.y, g, n, l), ynew = lane_view(w.ynew, g, n, l);
         const double s = combine_and_error(m, p, y, ynew, lane_view(w.yerr, g, n, l), K, atol);
-        if (!lockstep)
-        {
-          const double error = smax(sqrt(s / static_cast<double>(n > 0 ? n : 1)), 1.0e-10);
-          if (control_update(p, cc, error))
-            for (int i = 0; i < n; ++i)
-              y[i] = ynew[i];
-          ctl[c] = cc;
-          running = cc.status == MICM_HIP_STATE_RUNNING;
-        }
+        const double error = smax(sqrt(s / static_cast<double>(n > 0 ? n : 1)), 1.0e-10);
+        if (control_update(p, cc, error))
+          for (int i = 0; i < n; ++i)
+            y[i] = ynew[i];
+        ctl[c] = cc;
+        running = cc.status == MICM_HIP_STATE_RUNNING;
       }
     }
-    if (!lockstep)
-    {
-      const unsigned long long mask = __ballot(running);
-      if ((threadIdx.x % kGroup) == 0 && mask)
-        atomicAdd(active, static_cast<unsigned int>(__popcll(mask)));
-    }
+    const unsigned long long mask = __ballot(running);
+    if ((threadIdx.x % kGroup) == 0 && mask)
+      atomicAdd(active, static_cast<unsigned int>(__popcll(mask)));
+  }
+
+  // Lockstep: Ynew = Y + sum m_s K_s and Yerr = sum e_s K_s (combine_and_error's
+  // expressions) with one thread per (cell, species) of the running cells, in the
+  // grouped layout's order; k_lockstep_error forms the State's error terms and
+  // k_lockstep_control finishes the attempt.
+  __global__ void __launch_bounds__(kBlock) k_lockstep_combine(DevMech m, DevParams p, int64_t n_cells, Work w,
+                                                               const CellControl* __restrict__ ctl)
+  {
+    const int64_t per_group = static_cast<int64_t>(m.n_species) * kGroup;
+    const int64_t f = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (f >= n_groups_of(n_cells) * per_group)
+      return;
+    const int64_t c = (f / per_group) * kGroup + f % kGroup;
+    if (c >= n_cells || ctl[c].status != MICM_HIP_STATE_RUNNING)
+      return;
+    double v = w.y[f], e = 0.0;
+    for (int s = 0; s < p.stages; ++s)
+      v += p.m[s] * w.K[static_cast<size_t>(s) * w.k_stride + f];
+    for (int s = 0; s < p.stages; ++s)
+      e += p.e[s] * w.K[static_cast<size_t>(s) * w.k_stride + f];
+    w.ynew[f] = v;
+    w.yerr[f] = e;
   }
 
   constexpr int kLockstepErrorThreads = 1024;
@@ -848,7 +863,19 @@ namespace
       {
         const int cnt = static_cast<int>(n_terms - base < kLockstepErrorThreads ? n_terms - base
                                                                                 : kLockstepErrorThreads);
-        for (int j = 0; j < cnt; ++j)
+        // sixteen reads in flight, then their additions in order
+        int j = 0;
+        for (; j + 16 <= cnt; j += 16)
+        {
+          double t[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u)
+            t[u] = terms[j + u];
+#pragma unroll
+          for (int u = 0; u < 16; ++u)
+            acc += t[u];
+        }
+        for (; j < cnt; ++j)
           acc += terms[j];
       }
       __syncthreads();
@@ -900,8 +927,8 @@ namespace
   // loop-top checks stops them all; the factor kernel handles it per cell identically.
 
   // Lockstep form of a small State in one workgroup: per attempt, every cell's
-  // factorisation, stages and error terms (k_plain_factor, k_plain_stage,
-  // k_plain_finish), the State's error sum in the reference's order
+  // factorisation, stages and combination (k_plain_factor, k_plain_stage,
+  // k_lockstep_combine), the State's error sum in the reference's order
   // (k_lockstep_error) and the shared step control (k_lockstep_control),
   // separated by workgroup barriers instead of kernel launches and a host check per
   // attempt.  Same operations in the same order, so the results are those of the
@@ -2743,12 +2770,16 @@ extern "C"
           for (int stage = 0; stage < p.stages; ++stage)
             hipLaunchKernelGGL(k_plain_stage, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl, stage);
         }
-        hipLaunchKernelGGL(k_plain_finish, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl, s->atol,
-                           s->active, lockstep ? 1 : 0);
-        if (lockstep)
+        if (!lockstep)
+          hipLaunchKernelGGL(k_plain_finish, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl, s->atol,
+                             s->active);
+        else
         {
-          hipLaunchKernelGGL(k_lockstep_error, dim3(1), dim3(kLockstepErrorThreads), 0, st, d, p, s->n_cells, w, s->ctl, s->atol,
-                             s->lockstep_error);
+          const int64_t pairs = n_groups_of(s->n_cells) * static_cast<int64_t>(d.n_species) * kGroup;
+          hipLaunchKernelGGL(k_lockstep_combine, dim3(static_cast<unsigned>((pairs + kBlock - 1) / kBlock)),
+                             dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl);
+          hipLaunchKernelGGL(k_lockstep_error, dim3(1), dim3(kLockstepErrorThreads), 0, st, d, p, s->n_cells, w,
+                             s->ctl, s->atol, s->lockstep_error);
           hipLaunchKernelGGL(k_lockstep_control, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl,
                              s->lockstep_error, s->active);
         }
