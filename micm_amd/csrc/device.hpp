@@ -123,12 +123,13 @@ namespace micm_hip
   struct CellControl
   {
     double H, t, last_alpha;
-    int32_t flags;   // bit 0 reject_last, bit 1 reject_more, bit 2 start of a new step
+    int32_t flags;   // bit 0 reject_last, bit 1 reject_more, bit 2 start of a new step,
+                     // bit 3 Jacobian due (set and cleared within one attempt)
     int32_t status;  // MICM_HIP_STATE_*
     int64_t stats[7];
   };
 
-  constexpr int32_t kRejectLast = 1, kRejectMore = 2, kNewStep = 4;
+  constexpr int32_t kRejectLast = 1, kRejectMore = 2, kNewStep = 4, kJacobianDue = 8;
 
   // Status of a cell handed from the per-lane form to the cooperative form at a step
   // boundary; never visible to callers.

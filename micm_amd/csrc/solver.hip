@@ -254,37 +254,46 @@ namespace
   }
 
   // jac -= J(y): per Jacobian element, its terms' products k * y... in the reference's
-  // order, four entries at a time with packed operands
-  template<int TC, int NT>
-  __global__ void __launch_bounds__(NT) k_subtract_jacobian_tiled(DevMech m, DevCoop c, int64_t n_cells,
-                                                                  const double* __restrict__ k,
-                                                                  const double* __restrict__ y,
-                                                                  double* __restrict__ jac)
+  // order, four entries at a time with packed operands.  DUE: only the running cells
+  // whose control flags ask for a new Jacobian, each formed from zero (the per-cell
+  // forms' step start); the other cells' Jacobians are left as they are.
+  template<int TC, int NT, bool DUE>
+  __device__ inline void subtract_jacobian_tile(const DevMech& m, const DevCoop& c, int64_t n_cells,
+                                                const double* __restrict__ k, const double* __restrict__ y,
+                                                double* __restrict__ jac, const CellControl* __restrict__ ctl)
   {
     extern __shared__ double tile[];
     size_t g;
     int q0;
     if (!tile_of_block<TC>(n_cells, g, q0))
       return;
+    const int cc = threadIdx.x % TC;
+    const int64_t cell = static_cast<int64_t>(g) * kGroup + q0 + cc;
+    bool due = cell < n_cells;
+    if constexpr (DUE)
+    {
+      due = due && ctl[cell].status == MICM_HIP_STATE_RUNNING && (ctl[cell].flags & kJacobianDue) != 0;
+      if (!__syncthreads_or(due ? 1 : 0))
+        return;
+    }
     const int N = m.n_species, R = m.n_reactions;
     double* yt = tile;
     double* kt = tile + static_cast<size_t>(N) * TC;
     load_tile<TC, NT>(yt, y, g, N, q0);
     load_tile<TC, NT>(kt, k, g, R, q0);
     __syncthreads();
-    const int cc = threadIdx.x % TC;
-    if (static_cast<int64_t>(g) * kGroup + q0 + cc >= n_cells)
+    if (!due)
       return;
     const int nnz = m.nnz_jac;
     const int ne = nnz * TC;
     double* out = jac + g * static_cast<size_t>(nnz) * kGroup + q0 + cc;
     int i = threadIdx.x;
-    double v_next = i < ne ? out[(i / TC) * kGroup] : 0.0;
+    double v_next = !DUE && i < ne ? out[(i / TC) * kGroup] : 0.0;
     for (; i < ne; i += NT)
     {
       const int e = i / TC;
       double v = v_next;
-      if (i + NT < ne)
+      if (!DUE && i + NT < ne)
         v_next = out[((i + NT) / TC) * kGroup];
       const int qe = c.jel_offsets[e + 1];
       for (int q = c.jel_offsets[e]; q < qe; q += 4)
@@ -327,6 +336,24 @@ namespace
       }
       out[e * kGroup] = v;
     }
+  }
+
+  template<int TC, int NT>
+  __global__ void __launch_bounds__(NT) k_subtract_jacobian_tiled(DevMech m, DevCoop c, int64_t n_cells,
+                                                                  const double* __restrict__ k,
+                                                                  const double* __restrict__ y,
+                                                                  double* __restrict__ jac)
+  {
+    subtract_jacobian_tile<TC, NT, false>(m, c, n_cells, k, y, jac, nullptr);
+  }
+
+  template<int TC, int NT>
+  __global__ void __launch_bounds__(NT) k_jacobian_due_tiled(DevMech m, DevCoop c, int64_t n_cells,
+                                                             const double* __restrict__ k,
+                                                             const double* __restrict__ y, double* __restrict__ jac,
+                                                             const CellControl* __restrict__ ctl)
+  {
+    subtract_jacobian_tile<TC, NT, true>(m, c, n_cells, k, y, jac, ctl);
   }
 
   // dst[map[v] * TC + cc] = src[(g * n_values + v) * 64 + q0 + cc] for map[v] >= 0,
@@ -734,6 +761,52 @@ namespace
       f0[i] = 0.0;
     prepare_attempt<false>(m, p, lane_view(w.y, g, m.n_species, l), lane_view(w.k, g, m.n_reactions, l), f0,
                            lane_view(w.jac, g, m.nnz_jac, l), cc);
+    ctl[c] = cc;
+  }
+
+  // The same start of an attempt with the Jacobian formed by k_jacobian_due_tiled
+  // between the two halves: k_plain_begin opens a new step where one is due (and
+  // marks the cell kJacobianDue), k_plain_alpha adds alpha to the diagonal.
+  __global__ void __launch_bounds__(kBlock) k_plain_begin(DevMech m, DevParams p, int64_t n_cells, Work w,
+                                                          CellControl* ctl)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n_cells)
+      return;
+    CellControl cc = ctl[c];
+    if (cc.status != MICM_HIP_STATE_RUNNING)
+      return;
+    const Lane f0 = lane_view(w.f0, c / kGroup, m.n_species, c % kGroup);
+    for (int i = 0; i < m.n_species; ++i)
+      f0[i] = 0.0;
+    if (cc.flags & kNewStep)
+    {
+      if (begin_step(p, cc))
+      {
+        cc.stats[MICM_HIP_STAT_FUNCTION_CALLS] += 1;
+        cc.stats[MICM_HIP_STAT_JACOBIAN_UPDATES] += 1;
+        cc.last_alpha = 0.0;
+        cc.flags = (cc.flags & ~kNewStep) | kJacobianDue;
+      }
+    }
+    ctl[c] = cc;
+  }
+
+  __global__ void __launch_bounds__(kBlock) k_plain_alpha(DevMech m, DevParams p, int64_t n_cells, Work w,
+                                                          CellControl* ctl)
+  {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n_cells)
+      return;
+    CellControl cc = ctl[c];
+    if (cc.status != MICM_HIP_STATE_RUNNING)
+      return;
+    const Lane J = lane_view(w.jac, c / kGroup, m.nnz_jac, c % kGroup);
+    const double alpha = next_alpha(p, cc);
+    for (int i = 0; i < m.n_species; ++i)
+      J[m.jac_diag[i]] += alpha * 1.0;
+    cc.stats[MICM_HIP_STAT_DECOMPOSITIONS] += 1;
+    cc.flags &= ~kJacobianDue;
     ctl[c] = cc;
   }
 
@@ -2139,6 +2212,23 @@ namespace
   // The launches of one Solve of a per-cell form (persistent or cooperative), enqueued
   // on `st` without waiting; prepare_solve must have run.
   // The processing order of the next Solve from this Solve's per-cell attempt counts.
+  // -J of the cells marked kJacobianDue, formed from zero by the LDS-tiled gather
+  int launch_jacobian_due(const micm_hip_mechanism* m, int64_t n_cells, const double* k, const double* y, double* jac,
+                          const CellControl* ctl, hipStream_t st)
+  {
+    const int tc = gather_tile_cells(m);
+    const size_t lds = gather_tile_bytes(m, tc);
+    switch (tc)
+    {
+      case 16: LAUNCH_GATHER(k_jacobian_due_tiled, 16, lds, st, m->dev, m->coop, n_cells, k, y, jac, ctl); break;
+      case 8: LAUNCH_GATHER(k_jacobian_due_tiled, 8, lds, st, m->dev, m->coop, n_cells, k, y, jac, ctl); break;
+      case 4: LAUNCH_GATHER(k_jacobian_due_tiled, 4, lds, st, m->dev, m->coop, n_cells, k, y, jac, ctl); break;
+      default: LAUNCH_GATHER(k_jacobian_due_tiled, 2, lds, st, m->dev, m->coop, n_cells, k, y, jac, ctl); break;
+    }
+    HIP_TRY(hipGetLastError());
+    return MICM_HIP_OK;
+  }
+
   int enqueue_order(micm_hip_state* s, hipStream_t st)
   {
     const unsigned grid = static_cast<unsigned>((s->n_cells + kBlock - 1) / kBlock);
@@ -2731,6 +2821,8 @@ extern "C"
       // the mechanism has them, else within the per-lane kernels
       const bool tiled = tile_cells(m, static_cast<size_t>(m->coop.nnz_alu)) > 0 &&
                          tile_cells(m, static_cast<size_t>(m->coop.nnz_alu) + 2 * static_cast<size_t>(d.n_species)) > 0;
+      // the Jacobian as the tiled gather for the cells that start a step
+      const bool gather = gather_tile_cells(m) > 0;
       // a small State of a small mechanism: the whole lockstep loop in one workgroup
       const bool fused = lockstep && s->n_cells <= kFusedLockstepMaxCells &&
                          static_cast<int64_t>(d.nnz_lower) + d.nnz_upper - d.n_species <= kFusedLockstepMaxFactor;
@@ -2745,7 +2837,17 @@ extern "C"
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, s->active);
         if (tiled)
         {
-          hipLaunchKernelGGL(k_plain_prepare, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl);
+          if (gather)
+          {
+            hipLaunchKernelGGL(k_plain_begin, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl);
+            if (int rc = launch_jacobian_due(m, s->n_cells, w.k, w.y, w.jac, s->ctl, st))
+              return rc;
+            hipLaunchKernelGGL(k_plain_alpha, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl);
+          }
+          else
+          {
+            hipLaunchKernelGGL(k_plain_prepare, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl);
+          }
           if (int rc = micm_hip_add_forcing_terms(m, s->n_cells, w.k, w.y, w.f0, st))
             return rc;
           if (int rc = micm_hip_lu_decompose(m, s->n_cells, w.jac, w.lower, w.upper, st))
