@@ -440,7 +440,8 @@ namespace
   template<int TC>
   __global__ void __launch_bounds__(kTileThreads) k_lu_solve_tiled(DevMech m, DevCoop c, int64_t n_cells,
                                                                    const double* __restrict__ lower,
-                                                                   const double* __restrict__ upper, double* __restrict__ x)
+                                                                   const double* __restrict__ upper, double* __restrict__ x,
+                                                                   int p_capacity)
   {
     extern __shared__ double lds_solve[];
     const int N = m.n_species;
@@ -469,7 +470,10 @@ namespace
         xt[i] = xt[i] / ld[i];
     }
     __syncthreads();
-    for (int pass = 0; pass < 2; ++pass)
+    // one cell per workgroup: the back substitution's long ordered sums run two-phase
+    // (products into LDS after ld, then one thread per row), as in the cooperative form
+    const int passes = TC == 1 ? 1 : 2;
+    for (int pass = 0; pass < passes; ++pass)
     {
       const DevLevels& lv = pass == 0 ? c.fwd : c.bwd;
       for (int l = 0; l < lv.n_levels; ++l)
@@ -489,6 +493,8 @@ namespace
         __syncthreads();
       }
     }
+    if constexpr (TC == 1)
+      coop_levels<kTileThreads>(c.bwd, c.bwd.tasks, c.bwd.terms, alu, xt, xt, ld + N, p_capacity);
     for (int i = tid; i < N * TC; i += kTileThreads)
     {
       const int r = i / TC, cc = i % TC;
@@ -2547,16 +2553,35 @@ extern "C"
     const int tc = tile_cells(m, static_cast<size_t>(m->coop.nnz_alu) + 2 * static_cast<size_t>(m->dev.n_species));
     if (tc > 0)
     {
-      const size_t lds = (static_cast<size_t>(m->coop.nnz_alu) + 2 * static_cast<size_t>(m->dev.n_species)) * tc *
-                         sizeof(double);
+      size_t lds = (static_cast<size_t>(m->coop.nnz_alu) + 2 * static_cast<size_t>(m->dev.n_species)) * tc *
+                   sizeof(double);
+      // one cell per workgroup: a product buffer for the two-phase back substitution
+      // from what two workgroups per CU leave (levels with more terms run task by task)
+      int p_capacity = 0;
+      if (tc == 1)
+      {
+        const size_t spare = lds_limit() / 2 > lds ? (lds_limit() / 2 - lds) / sizeof(double) : 0;
+        p_capacity = static_cast<int>(std::min<size_t>(static_cast<size_t>(m->coop.bwd.max_level_terms), spare));
+        lds += static_cast<size_t>(p_capacity) * sizeof(double);
+      }
       const unsigned blocks = tiled_blocks(tc, n_cells);
       switch (tc)
       {
-        case 16: LAUNCH_TILED(k_lu_solve_tiled<16>, blocks, lds, st, m->dev, m->coop, n_cells, lower, upper, x); break;
-        case 8: LAUNCH_TILED(k_lu_solve_tiled<8>, blocks, lds, st, m->dev, m->coop, n_cells, lower, upper, x); break;
-        case 4: LAUNCH_TILED(k_lu_solve_tiled<4>, blocks, lds, st, m->dev, m->coop, n_cells, lower, upper, x); break;
-        case 2: LAUNCH_TILED(k_lu_solve_tiled<2>, blocks, lds, st, m->dev, m->coop, n_cells, lower, upper, x); break;
-        default: LAUNCH_TILED(k_lu_solve_tiled<1>, blocks, lds, st, m->dev, m->coop, n_cells, lower, upper, x); break;
+        case 16:
+          LAUNCH_TILED(k_lu_solve_tiled<16>, blocks, lds, st, m->dev, m->coop, n_cells, lower, upper, x, p_capacity);
+          break;
+        case 8:
+          LAUNCH_TILED(k_lu_solve_tiled<8>, blocks, lds, st, m->dev, m->coop, n_cells, lower, upper, x, p_capacity);
+          break;
+        case 4:
+          LAUNCH_TILED(k_lu_solve_tiled<4>, blocks, lds, st, m->dev, m->coop, n_cells, lower, upper, x, p_capacity);
+          break;
+        case 2:
+          LAUNCH_TILED(k_lu_solve_tiled<2>, blocks, lds, st, m->dev, m->coop, n_cells, lower, upper, x, p_capacity);
+          break;
+        default:
+          LAUNCH_TILED(k_lu_solve_tiled<1>, blocks, lds, st, m->dev, m->coop, n_cells, lower, upper, x, p_capacity);
+          break;
       }
     }
     else
