@@ -747,7 +747,8 @@ namespace
 
   // With the level schedules available the forcing, the factorisation and the
   // substitutions run as the tiled kernels over every cell instead (finished cells'
-  // values are not used); k_plain_prepare and k_plain_rhs do the per-cell rest of
+  // values are not used); k_plain_prepare (or k_plain_begin, the masked tiled
+  // Jacobian and k_plain_alpha) and k_stage_rhs_elem do the per-cell rest of
   // k_plain_factor and k_plain_stage.  f(y) is evaluated for every running cell at
   // every attempt: y is unchanged since the cell's step began, so it is the same value
   // the step started with.
@@ -816,23 +817,58 @@ namespace
     ctl[c] = cc;
   }
 
+  // stage_rhs<PART> with one thread per (cell, species) of the running cells, in the
+  // grouped layout's order: the same expressions element by element; the thread of
+  // species 0 counts the cell's function call or solve.
   template<int PART>
-  __global__ void __launch_bounds__(kBlock) k_plain_rhs(DevMech m, DevParams p, int64_t n_cells, Work w, CellControl* ctl,
-                                                        int s)
+  __global__ void __launch_bounds__(kBlock) k_stage_rhs_elem(DevMech m, DevParams p, int64_t n_cells, Work w,
+                                                             CellControl* ctl, int s)
   {
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (c >= n_cells)
-      return;
-    CellControl cc = ctl[c];
-    if (cc.status != MICM_HIP_STATE_RUNNING)
-      return;
-    const size_t g = c / kGroup;
-    const int l = c % kGroup;
     const int n = m.n_species;
-    stage_rhs<PART>(m, p, s, lane_view(w.y, g, n, l), lane_view(w.k, g, m.n_reactions, l), lane_view(w.f0, g, n, l),
-                    lane_view(w.ynew, g, n, l), stage_view(w, g, n, l), cc);
-    ctl[c].stats[MICM_HIP_STAT_FUNCTION_CALLS] = cc.stats[MICM_HIP_STAT_FUNCTION_CALLS];
-    ctl[c].stats[MICM_HIP_STAT_SOLVES] = cc.stats[MICM_HIP_STAT_SOLVES];
+    const int64_t per_group = static_cast<int64_t>(n) * kGroup;
+    const int64_t f = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (f >= n_groups_of(n_cells) * per_group)
+      return;
+    const int64_t rem = f % per_group;
+    const int64_t c = (f / per_group) * kGroup + rem % kGroup;
+    if (c >= n_cells || ctl[c].status != MICM_HIP_STATE_RUNNING)
+      return;
+    const bool first = rem < kGroup;  // species 0 of the cell
+    const int sc = s * (s - 1) / 2;
+    double* K = w.K + f;
+    double* Ks = K + static_cast<size_t>(s) * w.k_stride;
+    if (s == 0)
+    {
+      if constexpr ((PART & 2) != 0)
+        *Ks = w.f0[f];
+    }
+    else if (p.new_f[s])
+    {
+      if constexpr ((PART & 1) != 0)
+      {
+        double v = w.y[f];
+        for (int j = 0; j < s; ++j)
+          v += p.a[sc + j] * K[static_cast<size_t>(j) * w.k_stride];
+        w.ynew[f] = v;
+        *Ks = 0.0;
+        if (first)
+          ctl[c].stats[MICM_HIP_STAT_FUNCTION_CALLS] += 1;
+      }
+    }
+    if constexpr ((PART & 2) == 0)
+      return;
+    if (s + 1 < p.stages && !p.new_f[s + 1])
+      Ks[w.k_stride] = *Ks;
+    if (s > 0)
+    {
+      const double H = ctl[c].H;
+      double v = *Ks;
+      for (int j = 0; j < s; ++j)
+        v += (p.c[sc + j] / H) * K[static_cast<size_t>(j) * w.k_stride];
+      *Ks = v;
+    }
+    if (first)
+      ctl[c].stats[MICM_HIP_STAT_SOLVES] += 1;
   }
 
   __global__ void __launch_bounds__(kBlock) k_plain_stage(DevMech m, DevParams p, int64_t n_cells, Work w, CellControl* ctl,
@@ -2848,6 +2884,9 @@ extern "C"
                          tile_cells(m, static_cast<size_t>(m->coop.nnz_alu) + 2 * static_cast<size_t>(d.n_species)) > 0;
       // the Jacobian as the tiled gather for the cells that start a step
       const bool gather = gather_tile_cells(m) > 0;
+      // kernels with one thread per (cell, species)
+      const unsigned pair_grid = static_cast<unsigned>(
+          (n_groups_of(s->n_cells) * static_cast<int64_t>(d.n_species) * kGroup + kBlock - 1) / kBlock);
       // a small State of a small mechanism: the whole lockstep loop in one workgroup
       const bool fused = lockstep && s->n_cells <= kFusedLockstepMaxCells &&
                          static_cast<int64_t>(d.nnz_lower) + d.nnz_upper - d.n_species <= kFusedLockstepMaxFactor;
@@ -2882,11 +2921,13 @@ extern "C"
             double* Ks = w.K + static_cast<size_t>(stage) * w.k_stride;
             if (stage > 0 && p.new_f[stage])
             {
-              hipLaunchKernelGGL(k_plain_rhs<1>, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl, stage);
+              hipLaunchKernelGGL(k_stage_rhs_elem<1>, dim3(pair_grid), dim3(kBlock), 0, st, d, p, s->n_cells, w,
+                                 s->ctl, stage);
               if (int rc = micm_hip_add_forcing_terms(m, s->n_cells, w.k, w.ynew, Ks, st))
                 return rc;
             }
-            hipLaunchKernelGGL(k_plain_rhs<2>, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl, stage);
+            hipLaunchKernelGGL(k_stage_rhs_elem<2>, dim3(pair_grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl,
+                               stage);
             if (int rc = micm_hip_lu_solve(m, s->n_cells, w.lower, w.upper, Ks, st))
               return rc;
           }
@@ -2902,9 +2943,7 @@ extern "C"
                              s->active);
         else
         {
-          const int64_t pairs = n_groups_of(s->n_cells) * static_cast<int64_t>(d.n_species) * kGroup;
-          hipLaunchKernelGGL(k_lockstep_combine, dim3(static_cast<unsigned>((pairs + kBlock - 1) / kBlock)),
-                             dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl);
+          hipLaunchKernelGGL(k_lockstep_combine, dim3(pair_grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl);
           hipLaunchKernelGGL(k_lockstep_error, dim3(1), dim3(kLockstepErrorThreads), 0, st, d, p, s->n_cells, w,
                              s->ctl, s->atol, s->lockstep_error);
           hipLaunchKernelGGL(k_lockstep_control, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl,
