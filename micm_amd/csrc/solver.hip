@@ -891,7 +891,7 @@ namespace
     ctl[c].stats[MICM_HIP_STAT_SOLVES] = cc.stats[MICM_HIP_STAT_SOLVES];
   }
 
-  // Combination, error and accept/reject of the per-cell step control.
+  // Error and accept/reject of the per-cell step control, after k_combine_elem.
   __global__ void __launch_bounds__(kBlock) k_plain_finish(DevMech m, DevParams p, int64_t n_cells, Work w, CellControl* ctl,
                                                            const double* __restrict__ atol, unsigned int* active)
   {
@@ -905,9 +905,8 @@ namespace
         const size_t g = c / kGroup;
         const int l = c % kGroup;
         const int n = m.n_species;
-        const Stages K = stage_view(w, g, n, l);
         Lane y = lane_view(w.y, g, n, l), ynew = lane_view(w.ynew, g, n, l);
-        const double s = combine_and_error(m, p, y, ynew, lane_view(w.yerr, g, n, l), K, atol);
+        const double s = error_sum(n, y, ynew, lane_view(w.yerr, g, n, l), atol, p.rtol);
         const double error = smax(sqrt(s / static_cast<double>(n > 0 ? n : 1)), 1.0e-10);
         if (control_update(p, cc, error))
           for (int i = 0; i < n; ++i)
@@ -921,11 +920,11 @@ namespace
       atomicAdd(active, static_cast<unsigned int>(__popcll(mask)));
   }
 
-  // Lockstep: Ynew = Y + sum m_s K_s and Yerr = sum e_s K_s (combine_and_error's
-  // expressions) with one thread per (cell, species) of the running cells, in the
-  // grouped layout's order; k_lockstep_error forms the State's error terms and
-  // k_lockstep_control finishes the attempt.
-  __global__ void __launch_bounds__(kBlock) k_lockstep_combine(DevMech m, DevParams p, int64_t n_cells, Work w,
+  // Ynew = Y + sum m_s K_s and Yerr = sum e_s K_s (combine_and_error's expressions)
+  // with one thread per (cell, species) of the running cells, in the grouped layout's
+  // order; k_plain_finish or (lockstep) k_lockstep_error and k_lockstep_control
+  // finish the attempt.
+  __global__ void __launch_bounds__(kBlock) k_combine_elem(DevMech m, DevParams p, int64_t n_cells, Work w,
                                                                const CellControl* __restrict__ ctl)
   {
     const int64_t per_group = static_cast<int64_t>(m.n_species) * kGroup;
@@ -1043,7 +1042,7 @@ namespace
 
   // Lockstep form of a small State in one workgroup: per attempt, every cell's
   // factorisation, stages and combination (k_plain_factor, k_plain_stage,
-  // k_lockstep_combine), the State's error sum in the reference's order
+  // k_combine_elem), the State's error sum in the reference's order
   // (k_lockstep_error) and the shared step control (k_lockstep_control),
   // separated by workgroup barriers instead of kernel launches and a host check per
   // attempt.  Same operations in the same order, so the results are those of the
@@ -2938,12 +2937,12 @@ extern "C"
           for (int stage = 0; stage < p.stages; ++stage)
             hipLaunchKernelGGL(k_plain_stage, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl, stage);
         }
+        hipLaunchKernelGGL(k_combine_elem, dim3(pair_grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl);
         if (!lockstep)
           hipLaunchKernelGGL(k_plain_finish, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl, s->atol,
                              s->active);
         else
         {
-          hipLaunchKernelGGL(k_lockstep_combine, dim3(pair_grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl);
           hipLaunchKernelGGL(k_lockstep_error, dim3(1), dim3(kLockstepErrorThreads), 0, st, d, p, s->n_cells, w,
                              s->ctl, s->atol, s->lockstep_error);
           hipLaunchKernelGGL(k_lockstep_control, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl,
