@@ -2,9 +2,10 @@
 
 The fastest way to run a mechanism depends on its size and on how unevenly its
 cells behave: the per-lane persistent form (one cell per lane, with or without
-handing slow cells to the cooperative form), or the cooperative form (one
+handing slow cells to the cooperative form), the cooperative form (one
 workgroup per cell) with 64, 128, 256 or 512 threads and with its constant tables
-staged in LDS or read from global memory.  `autotune` times every candidate on a
+staged in LDS or read from global memory, or the plain form (a kernel per phase
+over every cell, fastest when the cells take the same steps).  `autotune` times every candidate on a
 strided sample of the state's own cells and returns the fastest.  Every
 candidate produces bit-identical results, so the choice only affects speed.
 
@@ -30,6 +31,8 @@ CANDIDATES = (
     ("persistent+handoff16", _lib.SOLVE_PERSISTENT, 16, 0, -1),
     ("persistent+handoff48", _lib.SOLVE_PERSISTENT, 48, 0, -1),
     ("persistent", _lib.SOLVE_PERSISTENT, 0, 0, -1),
+    # a kernel per phase over every cell: fastest when all cells take the same steps
+    ("plain", _lib.SOLVE_PLAIN, -1, 0, -1),
 )
 
 _cache = {}
@@ -77,7 +80,8 @@ def autotune(mech, state, dt, params, sample=16384, first_sample=2048, pilot=256
              candidates=CANDIDATES, verbose=False):
     """Pick the fastest per-cell form for `state` (whose inputs and rate constants
     are set), on strided samples of its cells: every candidate on `pilot` cells,
-    those within twice the best on `first_sample` cells, the `finalists` fastest of
+    those within twice the best (the plain form: four times) on `first_sample` cells,
+    the `finalists` fastest of
     those on `sample` cells.  The last samples must hold many cells per resident
     workgroup: on a few cells per workgroup the time is that of the slowest cells,
     which favours the shapes that run one cell fastest over those that run the most
@@ -110,7 +114,10 @@ def autotune(mech, state, dt, params, sample=16384, first_sample=2048, pilot=256
         if c[1] == _lib.SOLVE_PERSISTENT and c[2] > 0 and best is not None and t > 4.0 * best:
             slow_handoff = True
         best = t if best is None else min(best, t)
-    cands = [c for c in cands if c[0] in times and times[c[0]] <= 2.0 * best]
+    # the plain form's time grows slowly with the cell count (its launches serve every
+    # cell at once), so a pilot understates it: it stays in at four times the best
+    cands = [c for c in cands
+             if c[0] in times and times[c[0]] <= (4.0 if c[1] == _lib.SOLVE_PLAIN else 2.0) * best]
     m1 = min(first_sample, m)
     if m1 > m0 and len(cands) > 1:
         times.update(_time_candidates(mech, *pick(m1), dt, params, cands, repeats, verbose))
