@@ -7,8 +7,11 @@
 // Integration forms (micm_hip_solve):
 //   LOCKSTEP   -- every cell of the state shares one step size, as the reference's
 //                 multi-cell State (rosenbrock.inl:12-304); the normalized error is
-//                 the reference's single running sum over cells and species.
-//   PLAIN      -- per-cell step size; one launch per kernel per stage, host loop.
+//                 the reference's single running sum over cells and species.  A small
+//                 State runs the whole loop in one workgroup (k_lockstep_fused).
+//   PLAIN      -- per-cell step size; one launch per phase of an attempt over all
+//                 cells (tiled gathers, factorisation and substitutions; per-cell or
+//                 per-(cell, species) kernels for the rest), host loop.
 //   PERSISTENT -- per-cell step size; one persistent launch whose wavefronts take
 //                 cells from a work queue and carry each to the end of its step.
 //   COOPERATIVE -- per-cell step size; one workgroup per cell (coop.hpp), cells from
