@@ -197,8 +197,26 @@ namespace micm_hip
   __device__ inline double error_sum(int n, const Lane& y, const Lane& ynew, const Lane& err, const double* atol,
                                      double rtol)
   {
+    // terms of eight species formed together (their loads in flight at once), then
+    // added one after another in species order
     double acc = 0.0;
-    for (int i = 0; i < n; ++i)
+    int i = 0;
+    for (; i + 8 <= n; i += 8)
+    {
+      double t[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+      {
+        const double a = fabs(y[i + u]), b = fabs(ynew[i + u]);
+        const double ymax = a > b ? a : b;
+        const double q = err[i + u] / (atol[i + u] + rtol * ymax);
+        t[u] = q * q;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        acc += t[u];
+    }
+    for (; i < n; ++i)
     {
       const double a = fabs(y[i]), b = fabs(ynew[i]);
       const double ymax = a > b ? a : b;
