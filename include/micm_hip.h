@@ -101,7 +101,7 @@ extern "C" {
 
 /* Integration forms of micm_hip_solve. */
 #define MICM_HIP_SOLVE_LOCKSTEP 0   /* one step size per state, as rosenbrock.inl:12-304 */
-#define MICM_HIP_SOLVE_PLAIN 1      /* per-cell step size, one launch per kernel per stage */
+#define MICM_HIP_SOLVE_PLAIN 1      /* per-cell step size, one launch per phase of an attempt */
 #define MICM_HIP_SOLVE_PERSISTENT 2 /* per-cell step size, persistent work-queue kernel; cells that
                                        reach the hand-off step count continue in the cooperative form */
 #define MICM_HIP_SOLVE_COOPERATIVE 3 /* per-cell step size, one wavefront per cell, working set in LDS */
