@@ -1,0 +1,90 @@
+"""The autotuner's selection logic on the host, with the GPU timings replaced by a
+model: the pilot prunes candidates at twice the best (the plain form at four times),
+the per-lane persistent form without hand-off is skipped once a hand-off variant
+was four times slower than the best, and the fastest of the finalists on the
+largest sample wins."""
+import types
+
+import numpy as np
+import pytest
+
+from micm_amd import _lib, autotune
+
+
+class _Mech:
+    def __init__(self, tag, supported=True):
+        self.info = types.SimpleNamespace(n_species=tag, n_reactions=1, nnz_jacobian=1, nnz_lower=1, nnz_upper=1,
+                                          lu_multiply_adds=1)
+        self._supported = supported
+
+    def cooperative_supported(self, stages):
+        return self._supported
+
+
+class _State:
+    def __init__(self, n):
+        self.n_cells = n
+
+    def get_variables(self):
+        return np.zeros((self.n_cells, 2))
+
+    def get_rate_constants(self):
+        return np.zeros((self.n_cells, 1))
+
+
+def _run(monkeypatch, model, n_cells, tag, supported=True):
+    """Runs autotune with seconds = model[label](cells); returns (choice, calls)."""
+    calls = []
+
+    def fake(mech, y, k, dt, params, candidates, repeats, verbose, give_up=None):
+        out = {}
+        for c in candidates:
+            calls.append((c[0], y.shape[0]))
+            out[c[0]] = model[c[0]](y.shape[0])
+        return out
+
+    monkeypatch.setattr(autotune, "_time_candidates", fake)
+    params = types.SimpleNamespace(stages=3)
+    return autotune.autotune(_Mech(tag, supported), _State(n_cells), 30.0, params), calls
+
+
+def _linear(t0, per_cell):
+    return lambda n: t0 + per_cell * n
+
+
+def test_plain_form_survives_the_pilot_and_wins_at_scale(monkeypatch):
+    # a launch-bound plain form: slow on 256 cells, fastest on 10,000
+    model = {label: _linear(0.008, 1.8e-5) for label, *_ in autotune.CANDIDATES}
+    model["cooperative256"] = _linear(0.006, 1.2e-5)
+    model["cooperative128"] = _linear(0.006, 1.55e-5)
+    model["plain"] = _linear(0.026, 0.66e-5)
+    model["persistent+handoff16"] = model["persistent+handoff48"] = _linear(0.2, 1e-4)
+    choice, calls = _run(monkeypatch, model, 10000, tag=101)
+    assert choice["label"] == "plain" and choice["form"] == _lib.SOLVE_PLAIN
+    assert ("plain", 10000) in calls
+    # the persistent variants four times slower than the best: the one without hand-off is skipped
+    assert ("persistent", 256) not in calls
+
+
+def test_plain_form_pruned_beyond_four_times(monkeypatch):
+    model = {label: _linear(0.5, 1e-4) for label, *_ in autotune.CANDIDATES}
+    model["cooperative256"] = _linear(0.4, 0.5e-4)
+    model["plain"] = _linear(2.5, 1e-4)  # over four times the best pilot
+    choice, calls = _run(monkeypatch, model, 100000, tag=102)
+    assert choice["label"] == "cooperative256"
+    assert [n for label, n in calls if label == "plain"] == [256]
+    assert max(n for _, n in calls) == 16384
+
+
+def test_without_cooperative_support_only_per_lane_forms_compete(monkeypatch):
+    model = {label: _linear(0.1, 1e-5) for label, *_ in autotune.CANDIDATES}
+    choice, calls = _run(monkeypatch, model, 5000, tag=103, supported=False)
+    assert all(not label.startswith("cooperative") for label, _ in calls)
+    assert choice["form"] in (_lib.SOLVE_PERSISTENT, _lib.SOLVE_PLAIN)
+
+
+@pytest.fixture(autouse=True)
+def _fresh_cache():
+    autotune._cache.clear()
+    yield
+    autotune._cache.clear()
