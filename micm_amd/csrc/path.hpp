@@ -40,14 +40,18 @@ namespace micm_hip
         const double k0 = p[0] * pmath::exp(p[2] / T) * pmath::pow(T / 300.0, p[1]);
         const double kinf = p[3] * pmath::exp(p[5] / T) * pmath::pow(T / 300.0, p[4]);
         const double ratio = k0 * rho / kinf;
-        k = k0 * scale / (1.0 + ratio) * pmath::pow(p[6], p[7] / (p[7] + pmath::pow(pmath::log10(ratio), 2.0)));
+        // std::pow(x, 2.0) with the constant exponent 2 compiles to x * x (an exact folding
+        // compilers make at any optimisation level), so the reference squares with a multiply
+        const double lg = pmath::log10(ratio);
+        k = k0 * scale / (1.0 + ratio) * pmath::pow(p[6], p[7] / (p[7] + lg * lg));
         break;
       }
       case MICM_HIP_RATE_BRANCHED:
       {
         const double a = p[5] * rho;
         const double b = 0.43 * pmath::pow(T / 298.0, -8.0);
-        const double A_val = a / (1.0 + a / b) * pmath::pow(0.41, 1.0 / (1.0 + pmath::pow(pmath::log10(a / b), 2.0)));
+        const double lg = pmath::log10(a / b);  // std::pow(lg, 2.0) compiles to lg * lg
+        const double A_val = a / (1.0 + a / b) * pmath::pow(0.41, 1.0 / (1.0 + lg * lg));
         const double pre = p[1] * pmath::exp(-p[2] / T);
         k = p[0] == 0.0 ? pre * (p[6] / (p[6] + A_val)) : pre * (A_val / (A_val + p[6]));
         break;

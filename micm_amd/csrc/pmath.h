@@ -1,30 +1,21 @@
-// pmath.h -- portable double-precision exp, log, log10 and pow.
+// pmath.h -- double-precision exp, log, log10 and pow that return, bit for bit,
+// what the reference's CPU solver gets from its platform C library.
 //
-// Built only from IEEE-754 +, -, *, / and exact bit manipulation, in a fixed
-// operation order, so the same source compiled for the host (g++) and for gfx950
-// (hipcc), both without FMA contraction, returns bit-identical results.  The rate
-// constants and the step-size controller of the device path use these functions;
-// the CPU oracle can be built with them as well, which makes whole solves
-// comparable bit for bit.
-//
-// Each function is accurate to about 0.51 ulp (tests/test_pmath.py measures it
-// against 60-digit values), so its result is the correctly rounded one -- and the
-// same as a good C library's -- except in rare near-halfway cases.
-//
-// Methods (published):
-//   exp: table-driven reduction after P. T. P. Tang, "Table-driven implementation of
-//        the exponential function in IEEE floating-point arithmetic", ACM TOMS 15
-//        (1989): x = (128 k + j) ln2/128 + r, |r| <= ln2/256, exp(x) = 2^k 2^(j/128)
-//        (1 + p(r)) with 2^(j/128) held as a sum of two doubles and p the degree-5
-//        Taylor polynomial of exp(r) - 1.
-//   log: x = 2^e m with m in [sqrt(1/2), sqrt(2)), s = (m - 1)/(m + 1) carried to
-//        double-double precision with Dekker's exact product (T. J. Dekker, "A
-//        floating-point technique for extending the available precision", Numer.
-//        Math. 18 (1971)), log m = 2 atanh(s) = 2s + 2s^3/3 + ... ; the result is
-//        a double-double e ln2 + log m.
-//   log10: fdlibm's decomposition (see log10 below) over this log.
-//   pow: exp(y log|x|) with y log|x| formed in double-double and its low part
-//        folded into exp's reduced argument; IEEE special cases first.
+// The reference calls std::exp, std::pow and std::log10 (rate constants,
+// include/micm/process/rate_constant/rate_constant_functions.hpp:51-126, and the
+// step-size controller, include/micm/solver/rosenbrock.inl:212-216).  On x86-64
+// Linux those are the GNU C library's (2.28 and later), whose exp, log and pow are
+// ARM Optimized Routines' algorithms (math/exp.c, math/log.c, math/pow.c; MIT OR
+// Apache-2.0 WITH LLVM-exception), selected in the variant built with fused
+// multiply-add on CPUs that have it; log10 is fdlibm's decomposition (e_log10.c,
+// Sun Microsystems 1993) over that log.  This header restates those algorithms --
+// same reductions, same tables (pmath_table.h, generated from their defining
+// formulas by gen_pmath_table.py), same polynomial coefficients and the same
+// rounding of every operation, with each multiply-add that the C library's build
+// fuses written as an explicit fma -- so the host build (g++) and the gfx950 build
+// (hipcc) both return exactly the library's results; everything else is compiled
+// without FMA contraction.  tests/test_pmath.py checks each function bit for bit
+// against the C library on millions of arguments and against 60-digit values.
 #pragma once
 
 #include <cstdint>
@@ -40,12 +31,14 @@
 namespace pmath
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-  __constant__ static const double kExp2Hi[128] = { PMATH_EXP2_TABLE_HI };
-  __constant__ static const double kExp2Lo[128] = { PMATH_EXP2_TABLE_LO };
+#define PM_TABLE __constant__ static const
 #else
-  static const double kExp2Hi[128] = { PMATH_EXP2_TABLE_HI };
-  static const double kExp2Lo[128] = { PMATH_EXP2_TABLE_LO };
+#define PM_TABLE static const
 #endif
+  PM_TABLE uint64_t kExpTable[256] = { PMATH_EXP_TABLE };
+  PM_TABLE double kPowLogTable[128][3] = { PMATH_POW_LOG_TABLE };
+  PM_TABLE double kLogTable[128][2] = { PMATH_LOG_TABLE };
+#undef PM_TABLE
 
   PM_FN double from_bits(uint64_t b)
   {
@@ -57,236 +50,318 @@ namespace pmath
     return __builtin_bit_cast(uint64_t, d);
   }
 
-  PM_FN bool is_nan(double x)
+  PM_FN uint32_t top12(double x)
   {
-    return x != x;
+    return static_cast<uint32_t>(to_bits(x) >> 52);
+  }
+
+  PM_FN double fma(double a, double b, double c)
+  {
+    return __builtin_fma(a, b, c);
   }
 
   constexpr double kInf = __builtin_huge_val();
-  constexpr double kLn2Hi = 6.93147180369123816490e-01;  // ln 2 to 31 significant bits: e * kLn2Hi is exact
-  constexpr double kLn2Lo = 1.9082149292705877e-10;      // ln 2 - kLn2Hi
-  constexpr double k128InvLn2 = 184.6649652337873;       // 128 / ln 2
-  constexpr double kLn2Over128Hi = 0.00541521234663378;  // kLn2Hi / 128 (exact)
-  constexpr double kLn2Over128Lo = 1.4907929134926466e-12;
-  constexpr double kInvLn10Hi = 0.4342944819032518;  // 1 / ln 10
-  constexpr double kLog10Of2Hi = 3.01029995663611771306e-01;  // log10(2) to 33 significant bits
-  constexpr double kLog10Of2Lo = 3.69423907715893078616e-13;  // log10(2) - kLog10Of2Hi
-  constexpr double kExpMax = 709.78271289338397;   // largest x with exp(x) finite
-  constexpr double kExpMin = -745.13321910194122;  // below: exp(x) rounds to zero
 
-  // 2^k for -1022 <= k <= 1023
-  PM_FN double pow2i(int k)
+  // exp: x = k ln2/128 + r, exp(x) = 2^(k/128) exp(r), 2^(k/128) = scale (1 + tail)
+  constexpr double kInvLn2N = 0x1.71547652b82fep0 * 128;
+  constexpr double kNegLn2HiN = -0x1.62e42fefa0000p-8;
+  constexpr double kNegLn2LoN = -0x1.cf79abc9e3b3ap-47;
+  constexpr double kShift = 0x1.8p52;
+  constexpr double kExpC2 = 0x1.ffffffffffdbdp-2;
+  constexpr double kExpC3 = 0x1.555555555543cp-3;
+  constexpr double kExpC4 = 0x1.55555cf172b91p-5;
+  constexpr double kExpC5 = 0x1.1111167a4d017p-7;
+  constexpr uint64_t kSignBias = 0x800ull << 7;
+
+  // |x| >= 512: scale's exponent may leave the normal range, so form the result
+  // with a shifted scale (and round once into the subnormal range).
+  PM_FN double exp_special(double tmp, uint64_t sbits, uint64_t ki)
   {
-    return from_bits(static_cast<uint64_t>(k + 1023) << 52);
+    if ((ki & 0x80000000u) == 0)
+    {
+      const double scale = from_bits(sbits - (1009ull << 52));
+      return 0x1p1009 * fma(scale, tmp, scale);
+    }
+    const double scale = from_bits(sbits + (1022ull << 52));
+    const double st = scale * tmp;  // not fused: it has a second use below
+    double y = scale + st;
+    if ((y < 0.0 ? -y : y) < 1.0)
+    {
+      const double one = y < 0.0 ? -1.0 : 1.0;
+      double lo = (scale - y) + st;
+      const double hi = one + y;
+      lo = ((one - hi) + y) + lo;
+      y = (hi + lo) - one;
+      if (y == 0.0)
+        y = from_bits((sbits + (1022ull << 52)) & 0x8000000000000000ull);
+    }
+    return 0x1p-1022 * y;
   }
 
-  // a + b = s + err exactly (Knuth)
-  PM_FN double two_sum(double a, double b, double& err)
+  // exp(x + xtail) with the sign of sign_bias (pow's odd powers of negative x)
+  PM_FN double exp_core(double x, double xtail, uint64_t sign_bias)
   {
-    const double s = a + b;
-    const double bb = s - a;
-    err = (a - (s - bb)) + (b - bb);
-    return s;
-  }
-
-  // a * b = p + err exactly (Dekker, with Veltkamp's split; |a|, |b| < 2^996)
-  PM_FN double two_prod(double a, double b, double& err)
-  {
-    const double p = a * b;
-    const double ca = 134217729.0 * a;  // 2^27 + 1
-    const double ah = ca - (ca - a);
-    const double al = a - ah;
-    const double cb = 134217729.0 * b;
-    const double bh = cb - (cb - b);
-    const double bl = b - bh;
-    err = ((ah * bh - p) + ah * bl + al * bh) + al * bl;
-    return p;
-  }
-
-  // exp(x + xl) for kExpMin <= x <= kExpMax and |xl| <= 1e-12 |x|
-  PM_FN double exp_core(double x, double xl)
-  {
-    const double t = x * k128InvLn2;
-    const int n = static_cast<int>(t < 0.0 ? t - 0.5 : t + 0.5);
-    const double nd = static_cast<double>(n);
-    const double r = ((x - nd * kLn2Over128Hi) - nd * kLn2Over128Lo) + xl;
-    const int j = n & 127;
-    const int k = (n - j) / 128;
-    // exp(r) - 1, |r| <= ln2/256: the degree-6 term is below 1e-18 relative
-    double q = 1.0 / 120.0;
-    q = 1.0 / 24.0 + r * q;
-    q = 1.0 / 6.0 + r * q;
-    q = 0.5 + r * q;
-    const double p = r + (r * r) * q;
-    const double th = kExp2Hi[j];
-    const double m = th + (kExp2Lo[j] + th * p);
-    if (k < -1000)
-      return (m * pow2i(k + 1000)) * pow2i(-1000);
-    if (k > 1000)
-      return (m * pow2i(k - 1000)) * pow2i(1000);
-    return m * pow2i(k);
+    uint32_t abstop = top12(x) & 0x7ff;
+    if (abstop - top12(0x1p-54) >= top12(512.0) - top12(0x1p-54))
+    {
+      if (abstop - top12(0x1p-54) >= 0x80000000u)  // |x| < 2^-54
+      {
+        const double one = 1.0 + x;
+        return sign_bias ? -one : one;
+      }
+      if (abstop >= top12(1024.0))
+      {
+        const double s = sign_bias ? -1.0 : 1.0;
+        return (to_bits(x) >> 63) ? s * 0.0 : s * kInf;
+      }
+      abstop = 0;  // 512 <= |x| < 1024
+    }
+    double kd = fma(kInvLn2N, x, kShift);
+    const uint64_t ki = to_bits(kd);
+    kd -= kShift;
+    double r = fma(kd, kNegLn2LoN, fma(kd, kNegLn2HiN, x));
+    r += xtail;
+    const uint64_t idx = 2 * (ki % 128);
+    const uint64_t top = (ki + sign_bias) << 45;
+    const double tail = from_bits(kExpTable[idx]);
+    const uint64_t sbits = kExpTable[idx + 1] + top;
+    const double r2 = r * r;
+    const double tmp = fma(r2 * r2, fma(r, kExpC5, kExpC4), fma(r2, fma(r, kExpC3, kExpC2), tail + r));
+    if (abstop == 0)
+      return exp_special(tmp, sbits, ki);
+    const double scale = from_bits(sbits);
+    return fma(scale, tmp, scale);
   }
 
   PM_FN double exp(double x)
   {
-    if (is_nan(x))
-      return x;
-    if (x > kExpMax)
-      return kInf;
-    if (x < kExpMin)
-      return 0.0;
-    return exp_core(x, 0.0);
+    if ((top12(x) & 0x7ff) >= top12(1024.0))
+    {
+      if (to_bits(x) == to_bits(-kInf))
+        return 0.0;
+      if ((top12(x) & 0x7ff) == 0x7ff)  // +inf or nan
+        return 1.0 + x;
+    }
+    return exp_core(x, 0.0, 0);
   }
 
-  // log(x) = hi + lo for finite x > 0, |lo| <= ulp(hi)
-  PM_FN double log_dd(double x, double& lo)
-  {
-    int e = 0;
-    if (x < 2.2250738585072014e-308)  // subnormal: scale into the normal range
-    {
-      x *= 18014398509481984.0;  // 2^54
-      e = -54;
-    }
-    const uint64_t b = to_bits(x);
-    e += static_cast<int>((b >> 52) & 0x7ff) - 1023;
-    double m = from_bits((b & 0x000fffffffffffffull) | 0x3ff0000000000000ull);  // [1, 2)
-    if (m > 1.41421356237309504880)
-    {
-      m = m * 0.5;
-      e += 1;
-    }
-    const double f = m - 1.0;  // exact (Sterbenz)
-    double u_lo;
-    const double u = two_sum(m, 1.0, u_lo);
-    // s = f / (u + u_lo) as s + s_lo
-    const double s = f / u;
-    double pe;
-    const double p = two_prod(s, u, pe);
-    const double s_lo = (((f - p) - pe) - s * u_lo) / u;
-    // 2 atanh(s) = 2s + 2s (s^2/3 + s^4/5 + ... + s^22/23); |s| <= 0.1716
-    const double s2 = s * s;
-    double q = 1.0 / 23.0;
-    q = 1.0 / 21.0 + s2 * q;
-    q = 1.0 / 19.0 + s2 * q;
-    q = 1.0 / 17.0 + s2 * q;
-    q = 1.0 / 15.0 + s2 * q;
-    q = 1.0 / 13.0 + s2 * q;
-    q = 1.0 / 11.0 + s2 * q;
-    q = 1.0 / 9.0 + s2 * q;
-    q = 1.0 / 7.0 + s2 * q;
-    q = 1.0 / 5.0 + s2 * q;
-    q = 1.0 / 3.0 + s2 * q;
-    const double tail = (2.0 * s) * (s2 * q);
-    const double ed = static_cast<double>(e);
-    double l0;
-    const double h = two_sum(ed * kLn2Hi, 2.0 * s, l0);
-    const double l = l0 + ((ed * kLn2Lo + 2.0 * s_lo) + tail);
-    const double hi = h + l;
-    lo = l - (hi - h);
-    return hi;
-  }
+  // log: x = 2^k z, z in [0x1.6p-1, 0x1.6p0), log x = k ln2 + log c + log1p(z/c - 1)
+  constexpr double kLn2Hi = 0x1.62e42fefa3800p-1;
+  constexpr double kLn2Lo = 0x1.ef35793c76730p-45;
+  constexpr double kLogA[5] = { -0x1.0000000000001p-1, 0x1.555555551305bp-2, -0x1.fffffffeb459p-3,
+                                0x1.999b324f10111p-3, -0x1.55575e506c89fp-3 };
+  // log1p(r) near x = 1, |x - 1| < 0x1p-4
+  constexpr double kLogB[11] = { -0x1p-1,
+                                 0x1.5555555555577p-2,
+                                 -0x1.ffffffffffdcbp-3,
+                                 0x1.999999995dd0cp-3,
+                                 -0x1.55555556745a7p-3,
+                                 0x1.24924a344de3p-3,
+                                 -0x1.fffffa4423d65p-4,
+                                 0x1.c7184282ad6cap-4,
+                                 -0x1.999eb43b068ffp-4,
+                                 0x1.78182f7afd085p-4,
+                                 -0x1.5521375d145cdp-4 };
 
   PM_FN double log(double x)
   {
-    if (is_nan(x))
-      return x;
-    if (x < 0.0)
-      return __builtin_nan("");
-    if (x == 0.0)
-      return -kInf;
-    if (x == kInf)
-      return kInf;
-    double lo;
-    return log_dd(x, lo);
+    uint64_t ix = to_bits(x);
+    const uint32_t top = static_cast<uint32_t>(ix >> 48);
+    constexpr uint64_t kLo = 0x3fee000000000000ull;  // 1 - 2^-4
+    constexpr uint64_t kHi = 0x3ff1090000000000ull;  // 1 + 0x1.09p-4
+    if (ix - kLo < kHi - kLo)
+    {
+      if (ix == 0x3ff0000000000000ull)
+        return 0.0;
+      const double r = x - 1.0;
+      const double r2 = r * r;
+      const double r3 = r * r2;
+      const double q3 = fma(r3, kLogB[10], fma(r2, kLogB[9], fma(r, kLogB[8], kLogB[7])));
+      const double q2 = fma(r3, q3, fma(r2, kLogB[6], fma(r, kLogB[5], kLogB[4])));
+      const double q1 = fma(r3, q2, fma(r2, kLogB[3], fma(r, kLogB[2], kLogB[1])));
+      // r r / 2 in double-double: r = rhi + rlo with rhi of 26 bits
+      const double w0 = r * 0x1p27;
+      const double rhi = (r + w0) - w0;
+      const double rlo = r - rhi;
+      const double w = (rhi * rhi) * kLogB[0];
+      const double hi = r + w;
+      double lo = (r - hi) + w;
+      lo = fma(kLogB[0] * rlo, rhi + r, lo);
+      return fma(r3, q1, lo) + hi;
+    }
+    if (top - 0x0010 >= 0x7ff0 - 0x0010)
+    {
+      if (ix * 2 == 0)
+        return -kInf;
+      if (ix == to_bits(kInf))
+        return x;
+      if ((top & 0x8000) || (top & 0x7ff0) == 0x7ff0)
+        return (x - x) / (x - x);
+      ix = to_bits(x * 0x1p52) - (52ull << 52);  // subnormal
+    }
+    const uint64_t tmp = ix - 0x3fe6000000000000ull;
+    const int i = static_cast<int>((tmp >> 45) % 128);
+    const int k = static_cast<int>(static_cast<int64_t>(tmp) >> 52);
+    const double z = from_bits(ix - (tmp & (0xfffull << 52)));
+    const double invc = kLogTable[i][0];
+    const double logc = kLogTable[i][1];
+    const double r = fma(z, invc, -1.0);
+    const double kd = static_cast<double>(k);
+    const double w = fma(kd, kLn2Hi, logc);
+    const double hi = w + r;
+    const double lo = fma(kd, kLn2Lo, (w - hi) + r);
+    const double r2 = r * r;
+    const double p = fma(r2, fma(r, kLogA[4], kLogA[3]), fma(r, kLogA[2], kLogA[1]));
+    return fma(r * r2, p, fma(r2, kLogA[0], lo)) + hi;
   }
 
-  // log10 as in fdlibm's e_log10.c (Sun Microsystems, 1993): x = 2^n x' with
-  // x' in [1, 2) for n >= 0 and in [1/2, 1) for n < 0, then
-  // log10(x) = (n log10(2)_lo + log(x') / ln 10) + n log10(2)_hi.  With a log that
-  // is correctly rounded in all but rare cases this reproduces C libraries built
-  // on that method bit for bit, and log10(10^k) == k for k = 0 .. 22.
+  // log10 as fdlibm's e_log10.c: x = 2^n x' with x' in [1, 2) for n >= 0 and in
+  // [1/2, 1) for n < 0, log10(x) = (n log10(2)_lo + log(x') / ln 10) + n log10(2)_hi.
+  constexpr double kInvLn10 = 4.34294481903251816668e-01;
+  constexpr double kLog10Of2Hi = 3.01029995663611771306e-01;
+  constexpr double kLog10Of2Lo = 3.69423907715893078616e-13;
+
   PM_FN double log10(double x)
   {
-    if (is_nan(x) || x <= 0.0 || x == kInf)
-      return log(x);
+    uint64_t b = to_bits(x);
+    int32_t hx = static_cast<int32_t>(b >> 32);
     int n = 0;
-    if (x < 2.2250738585072014e-308)
+    if (hx < 0x00100000)
     {
-      x *= 18014398509481984.0;  // 2^54
-      n = -54;
+      if (((hx & 0x7fffffff) | static_cast<uint32_t>(b)) == 0)
+        return -kInf;
+      if (hx < 0)
+        return (x - x) / (x - x);
+      n -= 54;
+      x *= 0x1p54;  // subnormal
+      b = to_bits(x);
+      hx = static_cast<int32_t>(b >> 32);
     }
-    const uint64_t b = to_bits(x);
-    n += static_cast<int>((b >> 52) & 0x7ff) - 1023;
+    if (hx >= 0x7ff00000)
+      return x + x;
+    n += (hx >> 20) - 1023;
     const int neg = n < 0 ? 1 : 0;
-    const double xr = from_bits((b & 0x000fffffffffffffull) | (static_cast<uint64_t>(1023 - neg) << 52));
-    const double nd = static_cast<double>(n + neg);
-    double lo;
-    const double z = nd * kLog10Of2Lo + kInvLn10Hi * log_dd(xr, lo);
-    return z + nd * kLog10Of2Hi;
+    hx = (hx & 0x000fffff) | ((0x3ff - neg) << 20);
+    const double y = static_cast<double>(n + neg);
+    const double xr = from_bits((static_cast<uint64_t>(static_cast<uint32_t>(hx)) << 32) | (b & 0xffffffffull));
+    const double z = y * kLog10Of2Lo + kInvLn10 * log(xr);
+    return z + y * kLog10Of2Hi;
   }
 
-  PM_FN bool is_integer(double y)
+  // pow's logarithm: log|x| = hi + tail to about 2^-68 relative
+  constexpr double kPowA[7] = { -0x1p-1,
+                                0x1.555555555556p-2 * -2,
+                                -0x1.0000000000006p-2 * -2,
+                                0x1.999999959554ep-3 * 4,
+                                -0x1.555555529a47ap-3 * 4,
+                                0x1.2495b9b4845e9p-3 * -8,
+                                -0x1.0002b8b263fc3p-3 * -8 };
+
+  PM_FN double pow_log(uint64_t ix, double& tail)
   {
-    if (!(y > -4503599627370496.0 && y < 4503599627370496.0))  // |y| >= 2^52: an integer (or inf)
-      return y == y && y != kInf && y != -kInf;
-    return static_cast<double>(static_cast<int64_t>(y)) == y;
+    const uint64_t tmp = ix - 0x3fe6955500000000ull;
+    const int i = static_cast<int>((tmp >> 45) % 128);
+    const int k = static_cast<int>(static_cast<int64_t>(tmp) >> 52);
+    const double z = from_bits(ix - (tmp & (0xfffull << 52)));
+    const double kd = static_cast<double>(k);
+    const double invc = kPowLogTable[i][0];
+    const double logc = kPowLogTable[i][1];
+    const double logctail = kPowLogTable[i][2];
+    const double r = fma(z, invc, -1.0);  // exact: 1/c has few bits
+    const double t1 = fma(kd, kLn2Hi, logc);
+    const double t2 = t1 + r;
+    const double lo1 = fma(kd, kLn2Lo, logctail);
+    const double lo2 = (t1 - t2) + r;
+    const double ar = kPowA[0] * r;
+    const double ar2 = r * ar;
+    const double ar3 = r * ar2;
+    const double hi = t2 + ar2;
+    const double lo3 = fma(ar, r, -ar2);
+    const double lo4 = (t2 - hi) + ar2;
+    const double q = fma(ar2, fma(ar2, fma(r, kPowA[6], kPowA[5]), fma(r, kPowA[4], kPowA[3])),
+                         fma(r, kPowA[2], kPowA[1]));
+    const double lo = fma(ar3, q, ((lo1 + lo2) + lo3) + lo4);
+    const double y = hi + lo;
+    tail = (hi - y) + lo;
+    return y;
   }
 
-  PM_FN bool is_odd_integer(double y)
+  // 0: not an integer, 1: odd integer, 2: even integer
+  PM_FN int check_int(uint64_t iy)
   {
-    if (!(y > -9007199254740992.0 && y < 9007199254740992.0))
-      return false;
-    const int64_t i = static_cast<int64_t>(y);
-    return static_cast<double>(i) == y && (i & 1) != 0;
+    const int e = static_cast<int>((iy >> 52) & 0x7ff);
+    if (e < 0x3ff)
+      return 0;
+    if (e > 0x3ff + 52)
+      return 2;
+    if (iy & ((1ull << (0x3ff + 52 - e)) - 1))
+      return 0;
+    if (iy & (1ull << (0x3ff + 52 - e)))
+      return 1;
+    return 2;
+  }
+
+  PM_FN bool zero_inf_nan(uint64_t i)
+  {
+    return 2 * i - 1 >= 2 * to_bits(kInf) - 1;
   }
 
   PM_FN double pow(double x, double y)
   {
-    if (y == 0.0 || x == 1.0)
-      return 1.0;
-    if (is_nan(x) || is_nan(y))
-      return x + y;
-    if (y == 1.0)
-      return x;
-    if (y == 2.0)
-      return x * x;
-    if (x == 0.0)
+    uint64_t sign_bias = 0;
+    uint64_t ix = to_bits(x);
+    const uint64_t iy = to_bits(y);
+    uint32_t topx = top12(x);
+    const uint32_t topy = top12(y);
+    if (topx - 0x001 >= 0x7ff - 0x001 || (topy & 0x7ff) - 0x3be >= 0x43e - 0x3be)
     {
-      const double s0 = ((to_bits(x) >> 63) != 0 && is_odd_integer(y)) ? -1.0 : 1.0;
-      return y > 0.0 ? s0 * 0.0 : s0 * kInf;
+      // x subnormal, zero, negative, inf or nan; or |y| < 2^-65, |y| >= 2^63 or nan
+      if (zero_inf_nan(iy))
+      {
+        if (2 * iy == 0)
+          return 1.0;
+        if (ix == 0x3ff0000000000000ull)
+          return 1.0;
+        if (2 * ix > 2 * to_bits(kInf) || 2 * iy > 2 * to_bits(kInf))
+          return x + y;
+        if (2 * ix == 2 * 0x3ff0000000000000ull)
+          return 1.0;
+        if ((2 * ix < 2 * 0x3ff0000000000000ull) == !(iy >> 63))
+          return 0.0;
+        return y * y;
+      }
+      if (zero_inf_nan(ix))
+      {
+        double x2 = x * x;
+        if ((ix >> 63) && check_int(iy) == 1)
+          x2 = -x2;
+        return (iy >> 63) ? 1.0 / x2 : x2;
+      }
+      if (ix >> 63)
+      {
+        const int yint = check_int(iy);
+        if (yint == 0)
+          return (x - x) / (x - x);
+        if (yint == 1)
+          sign_bias = kSignBias;
+        ix &= 0x7fffffffffffffffull;
+        topx &= 0x7ff;
+      }
+      if ((topy & 0x7ff) - 0x3be >= 0x43e - 0x3be)
+      {
+        if (ix == 0x3ff0000000000000ull)
+          return 1.0;
+        if ((topy & 0x7ff) < 0x3be)
+          return ix > 0x3ff0000000000000ull ? 1.0 + y : 1.0 - y;
+        return ((ix > 0x3ff0000000000000ull) == (topy < 0x800)) ? kInf : 0.0;
+      }
+      if (topx == 0)
+        ix = (to_bits(x * 0x1p52) & 0x7fffffffffffffffull) - (52ull << 52);  // subnormal
     }
-    double sign = 1.0;
-    if (x < 0.0)
-    {
-      if (y == kInf || y == -kInf)
-        return (x == -1.0) ? 1.0 : ((x > -1.0) == (y > 0.0) ? 0.0 : kInf);
-      if (!is_integer(y))
-        return __builtin_nan("");
-      if (is_odd_integer(y))
-        sign = -1.0;
-      x = -x;
-    }
-    if (x == kInf)
-      return y > 0.0 ? sign * kInf : sign * 0.0;
-    if (y == kInf)
-      return x > 1.0 ? kInf : 0.0;
-    if (y == -kInf)
-      return x > 1.0 ? 0.0 : kInf;
-    double l;
-    const double h = log_dd(x, l);
-    const double t = y * h;
-    if (!(t < 1000.0))
-      return sign * kInf;
-    if (!(t > -1000.0))
-      return sign * 0.0;
-    double pe;
-    const double p = two_prod(y, h, pe);
-    double wl;
-    const double w = two_sum(p, pe + y * l, wl);
-    if (w > kExpMax)
-      return sign * kInf;
-    if (w < kExpMin)
-      return sign * 0.0;
-    return sign * exp_core(w, wl);
+    double lo;
+    const double hi = pow_log(ix, lo);
+    const double ehi = y * hi;
+    const double elo = fma(y, lo, fma(y, hi, -ehi));
+    return exp_core(ehi, elo, sign_bias);
   }
 }  // namespace pmath

@@ -549,14 +549,17 @@ namespace
         double k0 = p[0] * RC_EXP(p[2] / T) * RC_POW(T / 300.0, p[1]);
         double kinf = p[3] * RC_EXP(p[5] / T) * RC_POW(T / 300.0, p[4]);
         double ratio = k0 * rho / kinf;
-        k = k0 * scale / (1.0 + ratio) * RC_POW(p[6], p[7] / (p[7] + RC_POW(RC_LOG10(ratio), 2.0)));
+        // std::pow(x, 2.0) with a constant 2 compiles to x * x (rate_constant_functions.hpp:56)
+        const double lg = RC_LOG10(ratio);
+        k = k0 * scale / (1.0 + ratio) * RC_POW(p[6], p[7] / (p[7] + lg * lg));
         break;
       }
       case MICM_HIP_RATE_BRANCHED:
       {
         double a = p[5] * rho;
         double b = 0.43 * RC_POW(T / 298.0, -8.0);
-        double A_val = a / (1.0 + a / b) * RC_POW(0.41, 1.0 / (1.0 + RC_POW(RC_LOG10(a / b), 2.0)));
+        const double lg = RC_LOG10(a / b);  // std::pow(lg, 2.0) compiles to lg * lg
+        double A_val = a / (1.0 + a / b) * RC_POW(0.41, 1.0 / (1.0 + lg * lg));
         double pre = p[1] * RC_EXP(-p[2] / T);
         k = p[0] == 0.0 ? pre * (p[6] / (p[6] + A_val)) : pre * (A_val / (A_val + p[6]));
         break;

@@ -1,12 +1,14 @@
 """The portable exp / log / log10 / pow of micm_amd/csrc/pmath.h (host build, via
 the portable-math oracle library).
 
-Accuracy is measured against values computed to 60 significant digits with
-Python's `decimal` module (correctly rounded exp and ln), independently of any C
-library.  A dropped series term, a wrong reduction constant or a wrong scaling
-exponent each push the error far beyond these bounds.  The device build of the
-same header is pinned to this host build bit for bit by the GPU rate-constant
-tests."""
+They restate the C library's exp / log / pow / log10 operation for operation, so
+they are checked bit for bit against it (Python's math module calls it) on the
+whole domain and on the ranges the path uses.  Independently of any C library,
+accuracy is measured against values computed to 60 significant digits with
+Python's `decimal` module: a dropped series term, a wrong reduction constant or a
+wrong scaling exponent each push the error far beyond these bounds.  The device
+build of the same header is pinned to this host build bit for bit by the GPU
+rate-constant and whole-solve tests."""
 import ctypes
 import decimal
 import math
@@ -78,18 +80,65 @@ def test_pow_accuracy_in_rate_constant_ranges():
     assert worst <= 0.6, worst
 
 
-def test_agreement_with_the_c_library():
-    """The reference computes rate constants with the C library; where that library is
-    accurate to about half an ulp (glibc's exp / log / pow, and log10 by the fdlibm
-    method), these functions return its results in nearly every case."""
+def _libm_sample(rng, n):
+    """Arguments over the whole domain and over the ranges the path uses: exp(C/T),
+    (T/D)^B, Fc^(N/(N + log10^2)), error^(1/ELO), log10 of falloff ratios; random
+    bit patterns (subnormals, infinities and NaNs included) and special values."""
+    bits = rng.integers(0, 2 ** 63, n, dtype=np.int64).view(np.float64) * rng.choice([-1.0, 1.0], n)
+    special = np.array([0.0, -0.0, 1.0, -1.0, 2.0, 0.5, math.inf, -math.inf, math.nan, 5e-324, -5e-324,
+                        2.2250738585072014e-308, 1e308, 709.78, -745.13, -708.4, 3.0, -3.0, 1e-300, 10.0])
+    return bits, special
+
+
+def _same(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return (a.view(np.int64) == b.view(np.int64)) | (np.isnan(a) & np.isnan(b))
+
+
+def test_bitwise_equal_to_the_c_library():
+    """exp, log, log10 and pow return exactly what the C library returns -- the
+    reference's own arithmetic (std::exp / std::pow / std::log10 on this platform)."""
     rng = np.random.default_rng(4)
-    x = rng.uniform(-20.0, 5.0, 20000)
-    assert np.mean(pm("exp", x) == np.array([math.exp(v) for v in x])) >= 0.99
-    x = np.exp(rng.uniform(-30.0, 30.0, 20000))
-    assert np.mean(pm("log", x) == np.array([math.log(v) for v in x])) >= 0.99
-    assert np.mean(pm("log10", x) == np.array([math.log10(v) for v in x])) >= 0.99
-    a, b = rng.uniform(0.6, 1.1, 20000), rng.uniform(-5.0, 1.0, 20000)
-    assert np.mean(pm("pow", a, b) == np.array([math.pow(p, q) for p, q in zip(a, b)])) >= 0.99
+    n = 60000
+    bits, special = _libm_sample(rng, n)
+    for name, fn, args in [
+        ("exp", math.exp, [bits, rng.uniform(-750, 750, n), rng.uniform(-2000, 500, n) / rng.uniform(180, 320, n),
+                           special]),
+        ("log", math.log, [np.abs(bits), np.exp(rng.uniform(-700, 700, n)), rng.uniform(0.9, 1.1, n), special]),
+        ("log10", math.log10, [np.abs(bits), np.exp(rng.uniform(-60, 60, n)), special]),
+    ]:
+        x = np.concatenate(args)
+        want = np.empty_like(x)
+        for i, v in enumerate(x):
+            try:
+                want[i] = fn(v)
+            except (ValueError, OverflowError):  # CPython raises where the C library returns NaN / inf
+                want[i] = np.nan
+        got = pm(name, x)
+        ok = _same(got, want) | np.isnan(want)
+        assert ok.all(), (name, x[~ok][:5], got[~ok][:5], want[~ok][:5])
+    a = np.concatenate([np.abs(bits[:n // 2]), rng.uniform(0.5, 1.2, n), np.exp(rng.uniform(-23, 14, n)),
+                        rng.uniform(0.1, 1.0, n), -np.exp(rng.uniform(-5, 5, n))])
+    b = np.concatenate([bits[n // 2:], rng.uniform(-10, 2, n), rng.choice([0.25, 1 / 3, 0.5, 1 / 6], n),
+                        1.0 / (1.0 + rng.uniform(0, 30, n)), np.round(rng.uniform(-30, 30, n))])
+    want = np.empty_like(a)
+    for i, (p, q) in enumerate(zip(a, b)):
+        try:
+            want[i] = math.pow(p, q)
+        except (ValueError, OverflowError):
+            want[i] = np.nan
+    got = pm("pow", a, b)
+    ok = _same(got, want) | np.isnan(want)
+    assert ok.all(), (a[~ok][:5], b[~ok][:5], got[~ok][:5], want[~ok][:5])
+    sa, sb = np.repeat(special, len(special)), np.tile(special, len(special))
+    want = np.empty_like(sa)
+    for i, (p, q) in enumerate(zip(sa, sb)):
+        try:
+            want[i] = math.pow(p, q)
+        except (ValueError, OverflowError):
+            want[i] = np.nan
+    got = pm("pow", sa, sb)
+    assert (_same(got, want) | np.isnan(want)).all()
 
 
 def test_special_values():
@@ -113,18 +162,25 @@ def test_special_values():
     assert pm("pow", -2.0, 3.0)[0] == -8.0
 
 
-def test_portable_oracle_differs_only_in_rate_constant_bits():
-    """The two restatements share every step but exp / log10 / pow: their rate
-    constants agree to a few ulp, and given the same rate constants the forcing,
-    Jacobian and LU are bit for bit the same."""
+def test_portable_oracle_equals_c_library_oracle():
+    """The two restatements differ only in where exp / log10 / pow come from, so rate
+    constants and whole solves (step sequences, outcomes, final states) are bit for
+    bit the same."""
     desc = synthetic.mechanism_60_200()
     cells = synthetic.inputs_60_200(desc, 257)
     a, b = Oracle(desc, 4), OraclePortableMath(desc, 4)
     ka, kb = a.rate_constants(cells.conditions, cells.custom), b.rate_constants(cells.conditions, cells.custom)
-    nz = ka != 0
-    assert np.array_equal(nz, kb != 0)
-    assert np.max(np.abs(ka - kb)[nz] / np.abs(ka)[nz]) <= 4e-15
-    assert np.mean(ka == kb) >= 0.99
+    assert np.array_equal(ka.view(np.int64), kb.view(np.int64))
     y = cells.variables[:, a.species_order()]
     assert np.array_equal(a.forcing(ka, y), b.forcing(ka, y))
     assert np.array_equal(a.jacobian(ka, y), b.jacobian(ka, y))
+    for desc, inputs, tableau, dt, n, ns in [(desc, synthetic.inputs_60_200, 4, 300.0, 300, 10),
+                                             (synthetic.mechanism_210_560(), synthetic.inputs_210_560, 4, 900.0, 64, 1)]:
+        a, b = Oracle(desc, tableau), OraclePortableMath(desc, tableau)
+        cells = inputs(desc, n, seed=29)
+        y0 = cells.variables[:, a.species_order()]
+        ra = a.solve(cells.conditions, cells.custom, y0, dt, ns)
+        rb = b.solve(cells.conditions, cells.custom, y0, dt, ns)
+        assert np.array_equal(ra[1], rb[1]) and np.array_equal(ra[2], rb[2])
+        assert np.array_equal(ra[0].view(np.int64), rb[0].view(np.int64))
+        assert np.array_equal(ra[3].view(np.int64), rb[3].view(np.int64))
