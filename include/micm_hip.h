@@ -300,6 +300,54 @@ int micm_hip_update_state_parameters(const micm_hip_mechanism* mechanism, micm_h
 int micm_hip_solve(const micm_hip_mechanism* mechanism, micm_hip_state* state, double time_step,
                    const micm_hip_rosenbrock_parameters* parameters, int32_t form);
 
+/* ---------------------------------------------------------------- device-to-device transfers
+ * Grouped device arrays of n_groups * 64 rows, on the state's stream; the call waits for the copy.
+ * CudaDenseMatrix::CopyToDevice / CopyToHost (cuda_dense_matrix.hpp:46-50) for callers whose
+ * data already lives on the device (the autograd binding). */
+int micm_hip_state_set_variables_device(micm_hip_state* state, const double* grouped);
+int micm_hip_state_get_variables_device(const micm_hip_state* state, double* grouped);
+int micm_hip_state_set_rate_constants_device(micm_hip_state* state, const double* grouped);
+int micm_hip_state_get_rate_constants_device(const micm_hip_state* state, double* grouped);
+
+/* ---------------------------------------------------------------- discrete adjoint
+ * No counterpart in the reference, which has no derivatives of Solve.  What is
+ * differentiated is the map a recorded Solve applied to each cell: its accepted steps
+ * (rosenbrock.inl:140-207, one Jacobian and factorisation per step), each with its step
+ * size H frozen, and the clamp of solver.hpp:275-297 after every Solve.  Rejected attempts
+ * only chose H and drop out.
+ *
+ * A tape holds per cell up to max_records checkpoints of n_species + 1 doubles:
+ * (H, y at the start of an accepted step) or (-1, y before the post-solve clamp).  The
+ * adjoint replays each step from its checkpoint (stage vectors recomputed, not stored),
+ * then runs its stages backwards with the transposed factors. */
+typedef struct micm_hip_tape micm_hip_tape;
+int micm_hip_tape_create(const micm_hip_mechanism* mechanism, int64_t n_cells, int64_t max_records,
+                         micm_hip_tape** out);
+void micm_hip_tape_destroy(micm_hip_tape* tape);
+/* Forget every record: the next recorded Solve starts a new sequence. */
+int micm_hip_tape_reset(micm_hip_tape* tape);
+/* Host copies of cells [first_cell, first_cell + n_cells): records [n_cells][max_records][n_species + 1],
+ * counts [n_cells]; either may be NULL.  A count above max_records marks a cell whose tape
+ * overflowed (its later records were dropped). */
+int micm_hip_tape_get(const micm_hip_tape* tape, int64_t first_cell, int64_t n_cells, double* records,
+                      int64_t* counts);
+int micm_hip_tape_set(micm_hip_tape* tape, const double* records, const int64_t* counts);
+int micm_hip_tape_overflowed(const micm_hip_tape* tape, int64_t* n_cells_overflowed);
+/* Solver::Solve (solver.hpp:133-142) in the cooperative form, appending each cell's
+ * checkpoints to the tape (results identical to micm_hip_solve).  flags: 0 or
+ * MICM_HIP_SOLVE_UPDATE_RATES.  Successive recorded Solves extend the same tape. */
+int micm_hip_solve_recorded(const micm_hip_mechanism* mechanism, micm_hip_state* state, double time_step,
+                            const micm_hip_rosenbrock_parameters* parameters, int32_t flags, micm_hip_tape* tape);
+/* Reverse sweep over the tape with the state's rate constants (those of the recorded
+ * Solves).  Device arrays, grouped: seed = dL/dy_final [n_species], grad_variables =
+ * dL/dy_initial [n_species], grad_log_rate_constants = k dL/dk [n_reactions] per cell (for
+ * every rate type the derivative with respect to the log of a common factor on the
+ * reaction's A-prefactors; sum over cells for a shared parameter).  Cells whose tape
+ * overflowed get NaN.  The parameters must be those of the recorded Solves. */
+int micm_hip_adjoint(const micm_hip_mechanism* mechanism, micm_hip_state* state,
+                     const micm_hip_rosenbrock_parameters* parameters, const micm_hip_tape* tape,
+                     const double* seed, double* grad_variables, double* grad_log_rate_constants);
+
 /* ---------------------------------------------------------------- packed solves
  * Several states -- of one mechanism or of different ones -- solved together.  The
  * per-cell Solve of every state (MICM_HIP_SOLVE_PERSISTENT or _COOPERATIVE) is

@@ -95,8 +95,20 @@ _SIGNATURES = {
     "micm_hip_mechanism_cooperative_supported": [_vp, ctypes.c_int32, _i32p],
     "micm_hip_update_state_parameters": [_vp, _vp],
     "micm_hip_solve": [_vp, _vp, ctypes.c_double, ctypes.POINTER(RosenbrockParameters), ctypes.c_int32],
+    "micm_hip_state_set_variables_device": [_vp, _vp],
+    "micm_hip_state_get_variables_device": [_vp, _vp],
+    "micm_hip_state_set_rate_constants_device": [_vp, _vp],
+    "micm_hip_state_get_rate_constants_device": [_vp, _vp],
+    "micm_hip_tape_create": [_vp, _i64, _i64, ctypes.POINTER(_vp)],
+    "micm_hip_tape_destroy": [_vp],
+    "micm_hip_tape_reset": [_vp],
+    "micm_hip_tape_get": [_vp, _i64, _i64, _dp, _ip],
+    "micm_hip_tape_set": [_vp, _dp, _ip],
+    "micm_hip_tape_overflowed": [_vp, _ip],
+    "micm_hip_solve_recorded": [_vp, _vp, ctypes.c_double, ctypes.POINTER(RosenbrockParameters), ctypes.c_int32, _vp],
+    "micm_hip_adjoint": [_vp, _vp, ctypes.POINTER(RosenbrockParameters), _vp, _vp, _vp, _vp],
 }
-_VOID = {"micm_hip_mechanism_destroy", "micm_hip_state_destroy", "micm_hip_packed_destroy"}
+_VOID = {"micm_hip_mechanism_destroy", "micm_hip_state_destroy", "micm_hip_packed_destroy", "micm_hip_tape_destroy"}
 
 _lib = None
 
@@ -330,6 +342,79 @@ class DeviceState:
 
     def solve(self, time_step, parameters, form=SOLVE_PLAIN):
         check(lib().micm_hip_solve(self.mechanism.h, self.h, float(time_step), ctypes.byref(parameters), int(form)))
+
+    # ---- device-resident transfers (grouped device arrays given by address)
+    def set_variables_device(self, ptr):
+        check(lib().micm_hip_state_set_variables_device(self.h, ptr))
+
+    def get_variables_device(self, ptr):
+        check(lib().micm_hip_state_get_variables_device(self.h, ptr))
+
+    def set_rate_constants_device(self, ptr):
+        check(lib().micm_hip_state_set_rate_constants_device(self.h, ptr))
+
+    def get_rate_constants_device(self, ptr):
+        check(lib().micm_hip_state_get_rate_constants_device(self.h, ptr))
+
+    # ---- discrete adjoint
+    def solve_recorded(self, time_step, parameters, tape, update_rates=False):
+        """A cooperative-form Solve that appends each cell's checkpoints to `tape`."""
+        check(lib().micm_hip_solve_recorded(self.mechanism.h, self.h, float(time_step), ctypes.byref(parameters),
+                                            SOLVE_UPDATE_RATES if update_rates else 0, tape.h))
+
+    def adjoint(self, parameters, tape, seed, grad_variables, grad_log_rate_constants):
+        """Reverse sweep over `tape`; device addresses of grouped arrays: seed = dL/dy_final,
+        outputs dL/dy_initial and k dL/dk per cell."""
+        check(lib().micm_hip_adjoint(self.mechanism.h, self.h, ctypes.byref(parameters), tape.h, seed,
+                                     grad_variables, grad_log_rate_constants))
+
+
+class Tape:
+    """Per-accepted-step checkpoints of recorded Solves (micm_hip_tape): per cell up to
+    max_records records (H, y at the step's start) or (-1, y before the clamp)."""
+
+    CLAMP = -1.0
+
+    def __init__(self, mechanism, n_cells, max_records):
+        self.mechanism = mechanism
+        self.n_cells = int(n_cells)
+        self.max_records = int(max_records)
+        h = _vp()
+        check(lib().micm_hip_tape_create(mechanism.h, self.n_cells, self.max_records, ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.micm_hip_tape_destroy(self.h)
+            self.h = None
+
+    def reset(self):
+        check(lib().micm_hip_tape_reset(self.h))
+
+    def counts(self):
+        c = np.zeros(self.n_cells, dtype=np.int64)
+        check(lib().micm_hip_tape_get(self.h, 0, self.n_cells, _dp(), _i(c)))
+        return c
+
+    def records(self, cells=None):
+        """Records of the given cells (all by default), [cells, max_records, n_species + 1]."""
+        idx = np.arange(self.n_cells) if cells is None else np.asarray(cells, dtype=np.int64)
+        r = np.zeros((len(idx), self.max_records, self.mechanism.n_species + 1))
+        for j, c in enumerate(idx):
+            check(lib().micm_hip_tape_get(self.h, int(c), 1, _d(r[j]), _ip()))
+        return r
+
+    def set(self, records, counts):
+        r = np.ascontiguousarray(records, dtype=np.float64)
+        c = np.ascontiguousarray(counts, dtype=np.int64)
+        if r.shape != (self.n_cells, self.max_records, self.mechanism.n_species + 1) or c.shape != (self.n_cells,):
+            raise ValueError("records / counts of the wrong shape")
+        check(lib().micm_hip_tape_set(self.h, _d(r), _i(c)))
+
+    def overflowed(self):
+        n = ctypes.c_int64(0)
+        check(lib().micm_hip_tape_overflowed(self.h, ctypes.byref(n)))
+        return n.value
 
 
 class PackedSolve:

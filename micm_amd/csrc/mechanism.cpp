@@ -419,6 +419,95 @@ namespace micm_hip
         t.jel_pack.push_back(t.term_pack[t.jel_term[q]]);
       t.jel_pack.insert(t.jel_pack.end(), 8, 0);
     }
+
+    // Tables of the discrete adjoint.  The transposed substitutions have no counterpart in
+    // the reference; their sums run in ascending column order.
+    void build_adjoint_tables(MechanismTables& t)
+    {
+      const int32_t n = static_cast<int32_t>(t.n_species);
+      const int32_t R = static_cast<int32_t>(t.n_reactions);
+      const int32_t nt = static_cast<int32_t>(t.term_process.size());
+      t.lv_ut = LevelSchedule{};
+      t.lv_lt = LevelSchedule{};
+      if (t.coop_levels_ok)
+      {
+        // columns of U and of the strict lower L, ascending rows within a column
+        std::vector<std::vector<std::pair<int32_t, int32_t>>> ucol(n), lcol(n);  // (row j, ALU position)
+        for (int32_t j = 0; j < n; ++j)
+        {
+          for (int32_t p = t.upper_row_ptr[j]; p < t.upper_row_ptr[j + 1]; ++p)
+            if (t.upper_col[p] > j)
+              ucol[t.upper_col[p]].emplace_back(j, t.alu_upper[p]);
+          for (int32_t p = t.lower_row_ptr[j]; p < t.lower_row_ptr[j + 1]; ++p)
+            if (t.lower_col[p] < j)
+              lcol[t.lower_col[p]].emplace_back(j, t.alu_lower[p]);
+        }
+        std::vector<int32_t> level(n, 0), ready;
+        std::vector<uint32_t> terms;
+        std::vector<LevelTask> tasks;
+        for (int32_t i = 0; i < n; ++i)
+        {
+          terms.clear();
+          ready.clear();
+          for (auto [j, pos] : ucol[i])
+          {
+            terms.push_back(static_cast<uint32_t>(pos) | (static_cast<uint32_t>(j) << 16));
+            ready.push_back(level[j]);
+          }
+          level[i] = add_segments(tasks, static_cast<uint32_t>(i), static_cast<uint32_t>(t.alu_diag[i]), terms, ready, 0,
+                                  kSolveSegmentMin) + 1;
+        }
+        t.lv_ut = pack_levels(tasks);
+        std::fill(level.begin(), level.end(), 0);
+        tasks.clear();
+        for (int32_t i = n - 1; i >= 0; --i)
+        {
+          if (lcol[i].empty())
+            continue;
+          terms.clear();
+          ready.clear();
+          for (auto [j, pos] : lcol[i])
+          {
+            terms.push_back(static_cast<uint32_t>(pos) | (static_cast<uint32_t>(j) << 16));
+            ready.push_back(level[j]);
+          }
+          level[i] = add_segments(tasks, static_cast<uint32_t>(i), kNoDivisor, terms, ready, 0, kSolveSegmentMin) + 1;
+        }
+        t.lv_lt = pack_levels(tasks);
+      }
+      std::vector<std::vector<int32_t>> by_rxn(R), by_indep(n);
+      std::vector<std::vector<std::pair<int32_t, int32_t>>> by_dep(n);
+      for (int32_t q = 0; q < nt; ++q)
+      {
+        by_rxn[t.term_process[q]].push_back(q);
+        by_indep[t.term_indep[q]].push_back(q);
+        for (int32_t a = t.term_dep_offsets[q]; a < t.term_dep_offsets[q + 1]; ++a)
+          by_dep[t.term_dep_ids[a]].emplace_back(q, a - t.term_dep_offsets[q]);
+      }
+      auto flatten = [](const std::vector<std::vector<int32_t>>& v, std::vector<int32_t>& off, std::vector<int32_t>& out) {
+        off.assign(1, 0);
+        out.clear();
+        for (const auto& l : v)
+        {
+          out.insert(out.end(), l.begin(), l.end());
+          off.push_back(static_cast<int32_t>(out.size()));
+        }
+      };
+      flatten(by_rxn, t.rxn_term_offsets, t.rxn_terms);
+      flatten(by_indep, t.indep_term_offsets, t.indep_terms);
+      t.hes_offsets.assign(1, 0);
+      t.hes_term.clear();
+      t.hes_slot.clear();
+      for (const auto& l : by_dep)
+      {
+        for (auto [q, a] : l)
+        {
+          t.hes_term.push_back(q);
+          t.hes_slot.push_back(a);
+        }
+        t.hes_offsets.push_back(static_cast<int32_t>(t.hes_term.size()));
+      }
+    }
   }  // namespace
 
   MechanismTables compile_mechanism(const micm_hip_mechanism_desc& d)
@@ -593,6 +682,7 @@ namespace micm_hip
     for (auto [ind, k] : columns)
     {
       t.term_process.push_back(k);
+      t.term_indep.push_back(ind);
       bool skipped = false;
       for (int32_t i = t.reactant_offsets[k]; i < t.reactant_offsets[k + 1]; ++i)
       {
@@ -734,6 +824,7 @@ namespace micm_hip
       t.bwd_offsets.push_back(static_cast<int32_t>(t.bwd_mat.size()));
     }
     build_cooperative_schedule(t);
+    build_adjoint_tables(t);
     for (size_t i = 0; i < t.lu_term_l.size(); ++i)
     {
       t.lu_term_pairs.push_back(t.lu_term_l[i]);

@@ -105,6 +105,18 @@ namespace micm_hip
     std::vector<double> jel_coef;
     std::vector<uint64_t> jel_pack;                        // term_pack of each gather entry
 
+    // ---- discrete adjoint (one workgroup per cell, over the in-place factor)
+    std::vector<int32_t> term_indep;  // independent species of each Jacobian term
+    // transposed substitutions: A^T x = b with A = L U is U^T z = b (rows ascending), then
+    // L^T x = z (rows descending); row i of U^T is column i of U
+    LevelSchedule lv_ut;  // out = row, diag = U[i][i] position, terms (ALU position of U[j][i] | j << 16), j ascending
+    LevelSchedule lv_lt;  // out = row, no divisor, terms (ALU position of L[j][i] | j << 16), j ascending
+    std::vector<int32_t> rxn_term_offsets, rxn_terms;      // Jacobian terms of each reaction
+    std::vector<int32_t> indep_term_offsets, indep_terms;  // Jacobian terms whose independent species is q
+    // second derivative: entry (term t, dependent slot a) differentiates d_t by the species
+    // in slot a; grouped by that species
+    std::vector<int32_t> hes_offsets, hes_term, hes_slot;
+
     int64_t nnz_alu() const { return (int64_t)alu_col.size(); }
     int64_t nnz_jacobian() const { return (int64_t)jac_col.size(); }
     int64_t nnz_lower() const { return (int64_t)lower_col.size(); }

@@ -16,6 +16,7 @@
 //                 cells from a work queue and carry each to the end of its step.
 //   COOPERATIVE -- per-cell step size; one workgroup per cell (coop.hpp), cells from
 //                 a work queue, in the previous Solve's cost order once there was one.
+#include "adjoint.hpp"
 #include "coop.hpp"
 #include "device.hpp"
 #include "mechanism.hpp"
@@ -1341,7 +1342,8 @@ namespace
                                                double* __restrict__ k_state, const double* __restrict__ atol,
                                                int32_t* status_out, int64_t* stats_out, double* final_time_out,
                                                const double* __restrict__ cond_state,
-                                               const double* __restrict__ custom_state, double* __restrict__ k_wg)
+                                               const double* __restrict__ custom_state, double* __restrict__ k_wg,
+                                               Tape tape)
   {
     static_assert(!COMPACT || STAGE == 0, "the compact layout reads the tables from global memory");
     extern __shared__ double lds[];
@@ -1425,6 +1427,22 @@ namespace
         cc = resume[cell];
       else
         control_init(p, cc);
+      // recorded Solve: the cell's checkpoints are appended to its tape
+      int64_t n_rec = tape.records ? tape.counts[cell] : 0;
+      double* const rec_base =
+          tape.records ? tape.records + static_cast<size_t>(cell) * tape.max_records * (static_cast<size_t>(n) + 1)
+                       : nullptr;
+      auto record = [&](double tag) {
+        if (n_rec < tape.max_records)
+        {
+          double* rec = rec_base + static_cast<size_t>(n_rec) * (static_cast<size_t>(n) + 1);
+          if (lane == 0)
+            rec[0] = tag;
+          for (int i = lane; i < n; i += NT)
+            rec[1 + i] = L.y[i];
+        }
+        ++n_rec;
+      };
       coop_sync<NT>();
       while (true)
       {
@@ -1554,12 +1572,23 @@ namespace
         }
         coop_sync<NT>();
         const double error = s_error;
+        const double h_used = cc.H;
         if (control_update(p, cc, error))
+        {
+          if (rec_base)
+            record(h_used);
           for (int i = lane; i < n; i += NT)
             L.y[i] = COMPACT ? ynew_r : L.ynew[i];
+        }
         coop_sync<NT>();
         if (cc.status != MICM_HIP_STATE_RUNNING)
           break;
+      }
+      if (rec_base)
+      {
+        record(kClampTag);
+        if (lane == 0)
+          tape.counts[cell] = n_rec;
       }
       {
         Lane ys = lane_view(y_state, g, n, cl);
@@ -1585,6 +1614,7 @@ struct micm_hip_mechanism
   MechanismTables t;
   DevMech dev{};
   DevCoop coop{};
+  DevAdjoint adj{};
   void* blob = nullptr;  // one device allocation for every table
   int device = 0;
 };
@@ -1623,6 +1653,16 @@ struct micm_hip_state
   // cooperative launch shape (-1 / 0: chosen from the mechanism)
   int coop_stage = -1;
   int coop_threads = 0;
+  // the tape a recorded Solve appends to (records == nullptr otherwise)
+  Tape recording{ nullptr, nullptr, 0 };
+};
+
+struct micm_hip_tape
+{
+  const micm_hip_mechanism* mech = nullptr;
+  int64_t n_cells = 0, max_records = 0;
+  double* records = nullptr;
+  int64_t* counts = nullptr;
 };
 
 struct micm_hip_packed
@@ -1769,6 +1809,11 @@ namespace
         host[c * n_vals + v] = in[(static_cast<size_t>(c / kGroup) * n_vals + v) * kGroup + c % kGroup];
   }
 
+  size_t grouped_doubles(const micm_hip_state* s, int64_t n_vals)
+  {
+    return static_cast<size_t>(s->n_groups) * static_cast<size_t>(n_vals) * kGroup;
+  }
+
   int set_grouped(micm_hip_state* s, double* dst, const double* host, int64_t n_vals)
   {
     if (!s || (!host && n_vals))
@@ -1861,7 +1906,10 @@ namespace
              padded_bytes(t.rxn_pack) + padded_bytes(t.term_pack) +
              padded_bytes(t.spec_offsets) + padded_bytes(t.spec_rxn) +
              padded_bytes(t.spec_coef) + padded_bytes(t.jel_offsets) + padded_bytes(t.jel_term) +
-             padded_bytes(t.jel_coef) + padded_bytes(t.jel_pack) + 256;
+             padded_bytes(t.jel_coef) + padded_bytes(t.jel_pack) + level_bytes(t.lv_ut) + level_bytes(t.lv_lt) +
+             padded_bytes(t.term_indep) + padded_bytes(t.rxn_term_offsets) + padded_bytes(t.rxn_terms) +
+             padded_bytes(t.indep_term_offsets) + padded_bytes(t.indep_terms) + padded_bytes(t.hes_offsets) +
+             padded_bytes(t.hes_term) + padded_bytes(t.hes_slot) + 256;
     HIP_TRY(hipMalloc(&m->blob, bytes));
     std::vector<char> staging;
     size_t off = 0;
@@ -1920,6 +1968,17 @@ namespace
     upload(m->blob, off, t.jel_term, c.jel_term, staging);
     upload(m->blob, off, t.jel_coef, c.jel_coef, staging);
     upload(m->blob, off, t.jel_pack, c.jel_pack, staging);
+    DevAdjoint& a = m->adj;
+    upload_levels(m->blob, off, t.lv_ut, a.ut, staging);
+    upload_levels(m->blob, off, t.lv_lt, a.lt, staging);
+    upload(m->blob, off, t.term_indep, a.term_indep, staging);
+    upload(m->blob, off, t.rxn_term_offsets, a.rxn_term_offsets, staging);
+    upload(m->blob, off, t.rxn_terms, a.rxn_terms, staging);
+    upload(m->blob, off, t.indep_term_offsets, a.indep_term_offsets, staging);
+    upload(m->blob, off, t.indep_terms, a.indep_terms, staging);
+    upload(m->blob, off, t.hes_offsets, a.hes_offsets, staging);
+    upload(m->blob, off, t.hes_term, a.hes_term, staging);
+    upload(m->blob, off, t.hes_slot, a.hes_slot, staging);
     if (off > bytes)
       return fail(MICM_HIP_ERR_DEVICE, "internal table sizing error");
     HIP_TRY(hipMemcpy(m->blob, staging.data(), off, hipMemcpyHostToDevice));
@@ -2166,7 +2225,7 @@ namespace
                        m->dev,
                        m->coop, p, plan.p_capacity, n_fixed, list, count, s->coop_next, resume, s->y, s->k, s->atol,
                        s->status, s->stats, s->final_time, update_rates ? s->cond : nullptr,
-                       update_rates ? s->custom : nullptr, s->coop_k);
+                       update_rates ? s->custom : nullptr, s->coop_k, s->recording);
     HIP_TRY(hipGetLastError());
     return MICM_HIP_OK;
   }
@@ -2973,6 +3032,199 @@ extern "C"
       return MICM_HIP_OK;
     }
     return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "unknown solve form");
+  }
+
+  // ---------------------------------------------------------------- discrete adjoint
+
+  int micm_hip_tape_create(const micm_hip_mechanism* m, int64_t n_cells, int64_t max_records, micm_hip_tape** out)
+  {
+    if (int rc = check_device_mech(m))
+      return rc;
+    if (!out || n_cells < 1 || max_records < 1)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "a tape needs at least one cell and one record per cell");
+    auto tp = std::make_unique<micm_hip_tape>();
+    tp->mech = m;
+    tp->n_cells = n_cells;
+    tp->max_records = max_records;
+    const size_t doubles = static_cast<size_t>(n_cells) * static_cast<size_t>(max_records) *
+                           (static_cast<size_t>(m->dev.n_species) + 1);
+    if (hipMalloc(&tp->records, doubles * sizeof(double)) != hipSuccess)
+    {
+      (void)hipGetLastError();
+      return fail(MICM_HIP_ERR_OUT_OF_MEMORY, "tape of " + std::to_string(doubles * sizeof(double)) + " bytes");
+    }
+    if (hipMalloc(&tp->counts, sizeof(int64_t) * static_cast<size_t>(n_cells)) != hipSuccess)
+    {
+      (void)hipGetLastError();
+      (void)hipFree(tp->records);
+      return fail(MICM_HIP_ERR_OUT_OF_MEMORY, "tape counters");
+    }
+    HIP_TRY(hipMemset(tp->counts, 0, sizeof(int64_t) * static_cast<size_t>(n_cells)));
+    *out = tp.release();
+    return MICM_HIP_OK;
+  }
+
+  void micm_hip_tape_destroy(micm_hip_tape* tp)
+  {
+    if (!tp)
+      return;
+    (void)hipDeviceSynchronize();
+    (void)hipFree(tp->records);
+    (void)hipFree(tp->counts);
+    delete tp;
+  }
+
+  int micm_hip_tape_reset(micm_hip_tape* tp)
+  {
+    if (!tp)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null tape");
+    HIP_TRY(hipMemset(tp->counts, 0, sizeof(int64_t) * static_cast<size_t>(tp->n_cells)));
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_tape_get(const micm_hip_tape* tp, int64_t first_cell, int64_t n_cells, double* records,
+                        int64_t* counts)
+  {
+    if (!tp || first_cell < 0 || n_cells < 0 || first_cell + n_cells > tp->n_cells)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null tape or cell range outside it");
+    HIP_TRY(hipDeviceSynchronize());
+    const size_t per_cell = static_cast<size_t>(tp->max_records) * (static_cast<size_t>(tp->mech->dev.n_species) + 1);
+    if (records && n_cells)
+      HIP_TRY(hipMemcpy(records, tp->records + static_cast<size_t>(first_cell) * per_cell,
+                        sizeof(double) * static_cast<size_t>(n_cells) * per_cell, hipMemcpyDeviceToHost));
+    if (counts && n_cells)
+      HIP_TRY(hipMemcpy(counts, tp->counts + first_cell, sizeof(int64_t) * static_cast<size_t>(n_cells),
+                        hipMemcpyDeviceToHost));
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_tape_set(micm_hip_tape* tp, const double* records, const int64_t* counts)
+  {
+    if (!tp || !records || !counts)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(tp->records, records,
+                      sizeof(double) * static_cast<size_t>(tp->n_cells) * static_cast<size_t>(tp->max_records) *
+                          (static_cast<size_t>(tp->mech->dev.n_species) + 1),
+                      hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(tp->counts, counts, sizeof(int64_t) * static_cast<size_t>(tp->n_cells), hipMemcpyHostToDevice));
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_tape_overflowed(const micm_hip_tape* tp, int64_t* n_cells_overflowed)
+  {
+    if (!tp || !n_cells_overflowed)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null argument");
+    std::vector<int64_t> counts(static_cast<size_t>(tp->n_cells));
+    if (int rc = micm_hip_tape_get(tp, 0, tp->n_cells, nullptr, counts.data()))
+      return rc;
+    int64_t n = 0;
+    for (int64_t c : counts)
+      n += c > tp->max_records;
+    *n_cells_overflowed = n;
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_solve_recorded(const micm_hip_mechanism* m, micm_hip_state* s, double time_step,
+                              const micm_hip_rosenbrock_parameters* params, int32_t flags, micm_hip_tape* tape)
+  {
+    if (int rc = check_mech(m))
+      return rc;
+    if (!s || s->mech != m || !params || !tape || tape->mech != m || tape->n_cells != s->n_cells)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "bad state, parameters or tape");
+    if ((flags & ~MICM_HIP_SOLVE_UPDATE_RATES) != 0)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "flags: only MICM_HIP_SOLVE_UPDATE_RATES");
+    if (params->stages < 1 || params->stages > 6)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "stages must be 1..6");
+    if (plan_coop(m, static_cast<int>(params->stages), s->coop_stage, s->coop_threads).lds == 0)
+      return fail(MICM_HIP_ERR_UNSUPPORTED, "recorded solves run the cooperative form, which does not fit this mechanism");
+    const DevParams p = make_params(*params, time_step, s->rtol);
+    if (int rc = prepare_solve(s, MICM_HIP_SOLVE_COOPERATIVE, p.stages))
+      return rc;
+    s->recording = Tape{ tape->records, tape->counts, tape->max_records };
+    const int rc = enqueue_solve(s, p, MICM_HIP_SOLVE_COOPERATIVE, (flags & MICM_HIP_SOLVE_UPDATE_RATES) != 0, s->stream);
+    s->recording = Tape{ nullptr, nullptr, 0 };
+    if (rc)
+      return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_adjoint(const micm_hip_mechanism* m, micm_hip_state* s, const micm_hip_rosenbrock_parameters* params,
+                       const micm_hip_tape* tape, const double* seed, double* grad_variables,
+                       double* grad_log_rate_constants)
+  {
+    if (int rc = check_mech(m))
+      return rc;
+    if (!s || s->mech != m || !params || !tape || tape->mech != m || tape->n_cells != s->n_cells || !seed ||
+        !grad_variables || !grad_log_rate_constants)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "bad state, parameters, tape or buffers");
+    if (params->stages < 1 || params->stages > 6)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "stages must be 1..6");
+    if (!m->t.coop_levels_ok)
+      return fail(MICM_HIP_ERR_UNSUPPORTED, "the adjoint needs the level schedules (positions above 16 bits)");
+    const DevMech& d = m->dev;
+    const DevCoop& c = m->coop;
+    const size_t lds = adjoint_lds_doubles(c.nnz_alu, d.n_species, static_cast<int>(params->stages), d.n_reactions,
+                                           c.n_terms) *
+                       sizeof(double);
+    if (lds > lds_limit())
+      return fail(MICM_HIP_ERR_UNSUPPORTED, "the adjoint's per-cell working set (" + std::to_string(lds) +
+                                                " bytes) does not fit one workgroup's LDS");
+    const DevParams p = make_params(*params, 1.0, s->rtol);
+    constexpr int kAdjThreads = 256;
+    const void* fn = reinterpret_cast<const void*>(k_adjoint<kAdjThreads>);
+    HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    int dev = 0, cus = 0, per_cu = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adjoint<kAdjThreads>, kAdjThreads, lds));
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(s->n_cells, static_cast<int64_t>(cus) *
+                                                                                   std::max(per_cu, 1) * 4));
+    const Tape tp{ tape->records, tape->counts, tape->max_records };
+    hipLaunchKernelGGL(k_adjoint<kAdjThreads>, dim3(static_cast<unsigned>(blocks)), dim3(kAdjThreads), lds, s->stream,
+                       d, c, m->adj, p, s->n_cells, s->k, tp, seed, grad_variables, grad_log_rate_constants);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MICM_HIP_OK;
+  }
+
+  // device-to-device transfers of grouped arrays (the state's stream, then it waits)
+  int micm_hip_state_set_variables_device(micm_hip_state* s, const double* src)
+  {
+    if (!s || !src)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipMemcpyAsync(s->y, src, sizeof(double) * grouped_doubles(s, s->mech->dev.n_species),
+                           hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MICM_HIP_OK;
+  }
+  int micm_hip_state_get_variables_device(const micm_hip_state* s, double* dst)
+  {
+    if (!s || !dst)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipMemcpyAsync(dst, s->y, sizeof(double) * grouped_doubles(s, s->mech->dev.n_species),
+                           hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MICM_HIP_OK;
+  }
+  int micm_hip_state_set_rate_constants_device(micm_hip_state* s, const double* src)
+  {
+    if (!s || !src)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipMemcpyAsync(s->k, src, sizeof(double) * grouped_doubles(s, s->mech->dev.n_reactions),
+                           hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MICM_HIP_OK;
+  }
+  int micm_hip_state_get_rate_constants_device(const micm_hip_state* s, double* dst)
+  {
+    if (!s || !dst)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipMemcpyAsync(dst, s->k, sizeof(double) * grouped_doubles(s, s->mech->dev.n_reactions),
+                           hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MICM_HIP_OK;
   }
 
   int micm_hip_packed_create(int32_t n_states, micm_hip_state* const* states, int32_t form, micm_hip_packed** out)

@@ -78,6 +78,15 @@ class _Base:
                 "solve": [_vp, ctypes.c_int64, ctypes.c_int32, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int64,
                           ctypes.c_double, _dp, _i32p, _ip, _dp],
             }
+            if p == "oracle_":
+                sig.update({
+                    "solve_record": [_vp, ctypes.c_int64, _dp, _dp, ctypes.c_double, ctypes.c_int64, ctypes.c_double,
+                                     _dp, _i32p, _ip, ctypes.c_int64, _dp, _ip],
+                    "replay": [_vp, ctypes.c_int64, _dp, _dp, ctypes.c_int64, _dp, _ip],
+                    "adjoint": [_vp, ctypes.c_int64, _dp, ctypes.c_int64, _dp, _ip, _dp, _dp, _dp],
+                    "tangent": [_vp, ctypes.c_int64, _dp, _dp, ctypes.c_int64, _dp, _ip, ctypes.c_int64, _dp, _dp,
+                                _dp],
+                })
             for name, args in sig.items():
                 f = getattr(lib, p + name)
                 f.argtypes = args
@@ -181,19 +190,89 @@ class _Base:
         return y, states, stats, ft
 
 
+class Tape:
+    """Per-cell records of a recorded Solve sequence: records[c, r] = (H, y at the step's
+    start) of an accepted step, or (-1, y before the clamp) at the end of a Solve."""
+
+    CLAMP = -1.0
+
+    def __init__(self, records, counts):
+        self.records = records   # [cells, max_records, N + 1]
+        self.counts = counts     # [cells]
+
+    @property
+    def max_records(self):
+        return self.records.shape[1]
+
+
+class _AdjointMixin:
+    """The discrete adjoint of the recorded map (each accepted step with its H frozen),
+    its tangent-linear model and the map itself, for finite differences."""
+
+    def solve_record(self, k, y, dt, n_solves=1, max_records=4096, rtol=1e-6, atol=None):
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        y = np.array(y, dtype=np.float64, copy=True, order="C")
+        n = y.shape[0]
+        states = np.zeros(n, dtype=np.int32)
+        stats = np.zeros((n, 7), dtype=np.int64)
+        tape = np.zeros((n, max_records, self.n_species + 1))
+        counts = np.zeros(n, dtype=np.int64)
+        atol = None if atol is None else np.ascontiguousarray(atol, dtype=np.float64)
+        self._call("solve_record", n, _d(k), _d(y), float(dt), int(n_solves), float(rtol), _d(atol), _i32(states),
+                   _i(stats), int(max_records), _d(tape), _i(counts))
+        return y, states, stats, Tape(tape, counts)
+
+    def replay(self, k, y0, tape):
+        y = np.array(y0, dtype=np.float64, copy=True, order="C")
+        self._call("replay", y.shape[0], _d(np.ascontiguousarray(k, dtype=np.float64)), _d(y), tape.max_records,
+                   _d(tape.records), _i(tape.counts))
+        return y
+
+    def adjoint(self, k, tape, lam_final):
+        lam_final = np.ascontiguousarray(lam_final, dtype=np.float64)
+        n = lam_final.shape[0]
+        gy = np.zeros_like(lam_final)
+        gk = np.zeros((n, self.n_reactions))
+        self._call("adjoint", n, _d(np.ascontiguousarray(k, dtype=np.float64)), tape.max_records, _d(tape.records),
+                   _i(tape.counts), _d(lam_final), _d(gy), _d(gk))
+        return gy, gk
+
+    def tangent(self, k, y0, tape, dy0, dlogk):
+        """dy0 [cells, n_dir, N], dlogk [cells, n_dir, R] -> dy_final [cells, n_dir, N]."""
+        dy0 = np.ascontiguousarray(dy0, dtype=np.float64)
+        dlogk = np.ascontiguousarray(dlogk, dtype=np.float64)
+        n, nd = dy0.shape[:2]
+        out = np.zeros_like(dy0)
+        self._call("tangent", n, _d(np.ascontiguousarray(k, dtype=np.float64)),
+                   _d(np.ascontiguousarray(y0, dtype=np.float64)), tape.max_records, _d(tape.records),
+                   _i(tape.counts), nd, _d(dy0), _d(dlogk), _d(out))
+        return out
+
+
+def set_adjoint_fault(which):
+    """Plants a mistake in the oracle's adjoint (0 none, 1 no second-derivative term,
+    2 A instead of A^T, 3 a sign flip in the forcing's k-derivative) -- only for the tests
+    that show the adjoint's checks catch such mistakes."""
+    for cls in (Oracle, OraclePortableMath):
+        lib = cls._lib()
+        lib.oracle_set_adjoint_fault.argtypes = [ctypes.c_int32]
+        if lib.oracle_set_adjoint_fault(int(which)) != 0:
+            raise ValueError(which)
+
+
 class Reference(_Base):
     """The reference's compiled CPU solver (needs oracle/_ref)."""
     prefix = "ref_"
     path = REF_LIB
 
 
-class Oracle(_Base):
+class Oracle(_Base, _AdjointMixin):
     """The CPU restatement."""
     prefix = "oracle_"
     path = ORACLE_LIB
 
 
-class OraclePortableMath(_Base):
+class OraclePortableMath(_Base, _AdjointMixin):
     """The CPU restatement with the HIP path's exp / log10 / pow (micm_amd/csrc/pmath.h)."""
     prefix = "oracle_"
     path = ORACLE_PM_LIB

@@ -677,8 +677,14 @@ namespace
 
   // One Rosenbrock Solve over `nc` cells sharing one step size (rosenbrock.inl:12-304,
   // separate-LU path), followed by the post-solve clamp.  nc == 1 gives the per-cell form.
+  //
+  // With `tape` (nc == 1) the Solve appends its records for the discrete adjoint: one per
+  // accepted step (H, then y at the step's start) and one for the clamp (kClampTag, then y
+  // before the clamp); each record is N + 1 doubles.
+  constexpr double kClampTag = -1.0;
+
   void solve_state(const Mech& M, int64_t nc, const double* k, double* Y, double dt, double rtol,
-                   const double* atol, Stats& st)
+                   const double* atol, Stats& st, std::vector<double>* tape = nullptr)
   {
     const Tableau& p = M.tab;
     const int64_t N = M.N, nJ = M.nnzA;
@@ -800,6 +806,11 @@ namespace
         if ((error < 1) || (H < h_min))
         {
           s[MICM_HIP_STAT_ACCEPTED] += 1;
+          if (tape)
+          {
+            tape->push_back(H);
+            tape->insert(tape->end(), Y, Y + N);
+          }
           t = t + H;
           std::memcpy(Y, Ynew.data(), sizeof(double) * nc * N);
           Hnew = std::max(h_min, std::min(Hnew, h_max));
@@ -826,8 +837,330 @@ namespace
       state = MICM_HIP_STATE_CONVERGED;
     st.final_time = t;
     st.state = state;
+    if (tape)
+    {
+      tape->push_back(kClampTag);
+      tape->insert(tape->end(), Y, Y + N);
+    }
     for (int64_t c = 0; c < nc * N; ++c)
       Y[c] = std::max(Y[c], 0.0);  // PostSolveClamp (solver.hpp:295)
+  }
+
+  // ------------------------------------------------------------------ discrete adjoint
+  //
+  // The adjoint differentiates the map the recorded Solve applied: each accepted step with
+  // its step size H frozen (the reference has no adjoint; the step is rosenbrock.inl:140-207
+  // with A = alpha I - J(y), alpha = 1 / (H gamma_0), and the clamp of solver.hpp:275-297
+  // after each Solve).  Rejected attempts only chose H, so they drop out.
+  //
+  // One accepted step, S stages, f-evaluations g_s = f(Y_s) at Y_0 = y and
+  // Y_s = y + sum_{j<s} a_sj K_j where new_function_evaluation[s], else g_s = g_{s-1}:
+  //     r_s = g_s + sum_{j<s} (c_sj / H) K_j,   K_s = A^-1 r_s,   ynew = y + sum_s m_s K_s.
+  // Reverse, with lam = dL/dynew:  Kbar_s = m_s lam;  for s = S-1 .. 0:
+  //     rbar = A^-T Kbar_s;  Jbar += rbar K_s^T (A = alpha I - J);  gbar += rbar;
+  //     Kbar_j += (c_sj / H) rbar;  at an f-evaluation: ybar_s = J(Y_s)^T gbar,
+  //     lam += ybar_s, Kbar_j += a_sj ybar_s, gbar = 0.
+  // Jbar reaches y and k through the second derivative of the mass-action rates.
+  // Rate parameters enter as per-cell log k (d/dlog k = k d/dk), which for every rate type
+  // is the derivative with respect to the log of a common factor on its A-prefactors.
+
+  // Planted mistakes for the tests that show the adjoint's checks are sharp (0: none;
+  // 1: drop the Jacobian's dependence on y and k; 2: solve with A instead of A^T;
+  // 3: flip the sign of the forcing's k-derivative).
+  int g_adjoint_fault = 0;
+
+  struct StepData
+  {
+    std::vector<double> J, L, U, K, Ys, f, v;
+  };
+
+  void step_data_init(const Mech& M, StepData& w)
+  {
+    const int64_t S = M.tab.stages;
+    w.J.assign(M.nnzA, 0.0);
+    w.L.assign(M.nnzL, 0.0);
+    w.U.assign(M.nnzU, 0.0);
+    w.K.assign(S * M.N, 0.0);
+    w.Ys.assign(S * M.N, 0.0);
+    w.f.assign(M.N, 0.0);
+    w.v.assign(std::max(M.N, M.nnzA), 0.0);
+  }
+
+  // One accepted step with a frozen H: the stage recurrences of rosenbrock.inl:140-207 on
+  // A = alpha I - J(y); keeps the factors, the solved K_s and the forcing inputs Y_s.
+  void frozen_step(const Mech& M, const double* k, const double* y, double H, StepData& w, double* ynew)
+  {
+    const Tableau& p = M.tab;
+    const int64_t N = M.N;
+    std::fill(w.J.begin(), w.J.end(), 0.0);
+    subtract_jacobian(M, k, y, w.J.data());
+    const double alpha = 1.0 / (H * p.gamma[0]);
+    for (int64_t i = 0; i < N; ++i)
+      w.J[M.diag[i]] += alpha * 1.0;
+    lu_decompose(M, w.J.data(), w.L.data(), w.U.data());
+    for (int s = 0; s < p.stages; ++s)
+    {
+      const int sc = s * (s - 1) / 2;
+      double* Ks = w.K.data() + s * N;
+      double* Ysv = w.Ys.data() + s * N;
+      if (s == 0 || p.new_f[s])
+      {
+        for (int64_t i = 0; i < N; ++i)
+        {
+          double v = y[i];
+          for (int j = 0; j < s; ++j)
+            v += p.a[sc + j] * w.K[j * N + i];
+          Ysv[i] = s == 0 ? y[i] : v;
+        }
+        std::fill(w.f.begin(), w.f.end(), 0.0);
+        add_forcing(M, k, Ysv, w.f.data());
+      }
+      else
+      {
+        std::memcpy(Ysv, Ysv - N, sizeof(double) * N);
+      }
+      for (int64_t i = 0; i < N; ++i)
+      {
+        double v = w.f[i];
+        for (int j = 0; j < s; ++j)
+          v += (p.c[sc + j] / H) * w.K[j * N + i];
+        Ks[i] = v;
+      }
+      lu_solve(M, w.L.data(), w.U.data(), Ks);
+    }
+    for (int64_t i = 0; i < N; ++i)
+    {
+      double v = y[i];
+      for (int s = 0; s < p.stages; ++s)
+        v += p.m[s] * w.K[s * N + i];
+      ynew[i] = v;
+    }
+  }
+
+  // x <- A^-T x with A = L U: U^T z = x (rows ascending), then L^T x = z (rows descending)
+  void lu_solve_transposed(const Mech& M, const double* L, const double* U, double* x)
+  {
+    const int64_t n = M.N;
+    for (int64_t i = 0; i < n; ++i)
+    {
+      for (int64_t j = 0; j < i; ++j)
+        if (M.u_pos[j * n + i] >= 0)
+          x[i] -= U[M.u_pos[j * n + i]] * x[j];
+      x[i] /= U[M.u_diag[i]];
+    }
+    for (int64_t i = n - 1; i >= 0; --i)
+    {
+      for (int64_t j = i + 1; j < n; ++j)
+        if (M.l_pos[j * n + i] >= 0)
+          x[i] -= L[M.l_pos[j * n + i]] * x[j];
+      x[i] /= L[M.l_diag[i]];
+    }
+  }
+
+  // The Jacobian term of `t` adds d_t = k_r prod_{deps} y to -J[dep rows and the
+  // independent row q, q] and -yield d_t to -J[product rows, q] (process_set.hpp:594-670).
+  // Its share of u^T J v is therefore d_t v_q (sum yield u_prod - sum u_dep - u_q).
+  double term_weight(const Mech& M, const Mech::JacTerm& t, const double* u, const double* v)
+  {
+    const auto& rx = M.rx[t.process];
+    double w = 0.0;
+    for (size_t j = 0; j < rx.products.size(); ++j)
+      w += rx.yields[j] * u[rx.products[j]];
+    for (auto q : t.dep_reactants)
+      w -= u[q];
+    w -= u[t.independent];
+    return w * v[t.independent];
+  }
+
+  // ybar += d/dy (u^T J(y) v), th += k d/dk (u^T J(y) v)
+  void jacobian_second_order(const Mech& M, const double* k, const double* y, const double* u, const double* v,
+                             double* ybar, double* th)
+  {
+    for (const auto& t : M.jac_terms)
+    {
+      const double s = term_weight(M, t, u, v);
+      const size_t nd = t.dep_reactants.size();
+      double d = k[t.process];
+      for (auto q : t.dep_reactants)
+        d *= y[q];
+      th[t.process] += s * d;
+      for (size_t a = 0; a < nd; ++a)
+      {
+        double part = k[t.process];
+        for (size_t b = 0; b < nd; ++b)
+          if (b != a)
+            part *= y[t.dep_reactants[b]];
+        ybar[t.dep_reactants[a]] += s * part;
+      }
+    }
+  }
+
+  // out = J(y)^T u from the assembled -J (a different route from the rates' derivatives);
+  // th += k d/dk (u^T f(y)) = rate_r (sum yield u_prod - sum u_reactant)
+  void forcing_vjp(const Mech& M, const double* k, const double* y, const double* u, StepData& w, double* out,
+                   double* th)
+  {
+    const int64_t n = M.N;
+    std::vector<double> mj(M.nnzA, 0.0);
+    subtract_jacobian(M, k, y, mj.data());
+    for (int64_t q = 0; q < n; ++q)
+    {
+      double acc = 0.0;
+      for (int64_t i = 0; i < n; ++i)
+        if (M.a_pos[i * n + q] >= 0)
+          acc -= mj[M.a_pos[i * n + q]] * u[i];
+      out[q] = acc;
+    }
+    (void)w;
+    for (int64_t r = 0; r < M.R; ++r)
+    {
+      const auto& rx = M.rx[r];
+      double rate = k[r];
+      for (auto s : rx.reactants)
+        rate *= y[s];
+      double wr = 0.0;
+      for (size_t j = 0; j < rx.products.size(); ++j)
+        wr += rx.yields[j] * u[rx.products[j]];
+      for (auto s : rx.reactants)
+        wr -= u[s];
+      th[r] += (g_adjoint_fault == 3 ? -rate : rate) * wr;
+    }
+  }
+
+  // Adjoint of one accepted step: lam in dL/dynew, out dL/dy; th += k dL/dk.
+  void step_adjoint(const Mech& M, const double* k, const double* y, double H, StepData& w, double* lam, double* th)
+  {
+    const Tableau& p = M.tab;
+    const int64_t N = M.N;
+    const int S = p.stages;
+    std::vector<double> ynew(N), Kb(S * N), gb(N, 0.0), rb(N), yb(N);
+    frozen_step(M, k, y, H, w, ynew.data());
+    for (int s = 0; s < S; ++s)
+      for (int64_t i = 0; i < N; ++i)
+        Kb[s * N + i] = p.m[s] * lam[i];
+    for (int s = S - 1; s >= 0; --s)
+    {
+      const int sc = s * (s - 1) / 2;
+      std::memcpy(rb.data(), Kb.data() + s * N, sizeof(double) * N);
+      if (g_adjoint_fault == 2)
+        lu_solve(M, w.L.data(), w.U.data(), rb.data());
+      else
+        lu_solve_transposed(M, w.L.data(), w.U.data(), rb.data());
+      if (g_adjoint_fault != 1)
+        jacobian_second_order(M, k, y, rb.data(), w.K.data() + s * N, lam, th);
+      for (int64_t i = 0; i < N; ++i)
+        gb[i] += rb[i];
+      for (int j = 0; j < s; ++j)
+        for (int64_t i = 0; i < N; ++i)
+          Kb[j * N + i] += (p.c[sc + j] / H) * rb[i];
+      if (s == 0 || p.new_f[s])
+      {
+        forcing_vjp(M, k, w.Ys.data() + s * N, gb.data(), w, yb.data(), th);
+        for (int64_t i = 0; i < N; ++i)
+          lam[i] += yb[i];
+        for (int j = 0; j < s; ++j)
+          for (int64_t i = 0; i < N; ++i)
+            Kb[j * N + i] += p.a[sc + j] * yb[i];
+        std::fill(gb.begin(), gb.end(), 0.0);
+      }
+    }
+  }
+
+  // Tangent of one accepted step along nd directions: dy [nd][N] in/out, dth [nd][R]
+  // (log k).  A different formulation from the adjoint: derivatives of the assembled
+  // -J and of the forcing, solved with A (not its transpose).
+  void step_tangent(const Mech& M, const double* k, double* y, double H, StepData& w, int64_t nd, double* dy,
+                    const double* dth)
+  {
+    const Tableau& p = M.tab;
+    const int64_t N = M.N, n = M.N;
+    const int S = p.stages;
+    std::vector<double> ynew(N), mj(M.nnzA), dK(S * N), dg(N), dr(N), dys(N);
+    frozen_step(M, k, y, H, w, ynew.data());
+    for (int64_t d = 0; d < nd; ++d)
+    {
+      double* dyd = dy + d * N;
+      const double* dt = dth + d * M.R;
+      for (int s = 0; s < S; ++s)
+      {
+        const int sc = s * (s - 1) / 2;
+        const double* ys = w.Ys.data() + s * N;
+        if (s == 0 || p.new_f[s])
+        {
+          for (int64_t i = 0; i < N; ++i)
+          {
+            double v = dyd[i];
+            for (int j = 0; j < s; ++j)
+              v += p.a[sc + j] * dK[j * N + i];
+            dys[i] = v;
+          }
+          // J(Y_s) dys from the assembled -J, plus the rates' log-k derivatives
+          std::fill(mj.begin(), mj.end(), 0.0);
+          subtract_jacobian(M, k, ys, mj.data());
+          for (int64_t i = 0; i < n; ++i)
+          {
+            double acc = 0.0;
+            for (int64_t q = 0; q < n; ++q)
+              if (M.a_pos[i * n + q] >= 0)
+                acc -= mj[M.a_pos[i * n + q]] * dys[q];
+            dg[i] = acc;
+          }
+          for (int64_t r = 0; r < M.R; ++r)
+          {
+            const auto& rx = M.rx[r];
+            double rate = k[r] * dt[r];
+            for (auto q : rx.reactants)
+              rate *= ys[q];
+            for (auto q : rx.reactants)
+              dg[q] -= rate;
+            for (size_t j = 0; j < rx.products.size(); ++j)
+              dg[rx.products[j]] += rx.yields[j] * rate;
+          }
+        }
+        for (int64_t i = 0; i < N; ++i)
+        {
+          double v = dg[i];
+          for (int j = 0; j < s; ++j)
+            v += (p.c[sc + j] / H) * dK[j * N + i];
+          dr[i] = v;
+        }
+        // -dA K_s = dJ K_s: each Jacobian term's d_t changes by
+        // dd_t = d_t dth_r + k_r sum_a (prod_{deps but a} y) dy_a
+        const double* Ks = w.K.data() + s * N;
+        for (const auto& t : M.jac_terms)
+        {
+          const auto& rx = M.rx[t.process];
+          const size_t ndp = t.dep_reactants.size();
+          double dd = k[t.process] * dt[t.process];
+          for (auto q : t.dep_reactants)
+            dd *= y[q];
+          for (size_t a = 0; a < ndp; ++a)
+          {
+            double part = k[t.process];
+            for (size_t b = 0; b < ndp; ++b)
+              if (b != a)
+                part *= y[t.dep_reactants[b]];
+            dd += part * dyd[t.dep_reactants[a]];
+          }
+          const double c = dd * Ks[t.independent];
+          for (auto q : t.dep_reactants)
+            dr[q] -= c;
+          dr[t.independent] -= c;
+          for (size_t j = 0; j < rx.products.size(); ++j)
+            dr[rx.products[j]] += rx.yields[j] * c;
+        }
+        lu_solve(M, w.L.data(), w.U.data(), dr.data());
+        std::memcpy(dK.data() + s * N, dr.data(), sizeof(double) * N);
+      }
+      for (int64_t i = 0; i < N; ++i)
+      {
+        double v = dyd[i];
+        for (int s = 0; s < S; ++s)
+          v += p.m[s] * dK[s * N + i];
+        dyd[i] = v;
+      }
+    }
+    std::memcpy(y, ynew.data(), sizeof(double) * N);
   }
 
   struct OracleSolver
@@ -1042,6 +1375,169 @@ extern "C"
     }
     return 0;
   }
+
+  // ---------------------------------------------------------------- discrete adjoint
+  // Tapes: per cell max_records records of N + 1 doubles (see solve_state), counts[c]
+  // records used; a count above max_records marks a tape that overflowed.
+
+  // n_solves x Solve(dt) per cell from given rate constants, recording each cell's tape.
+  int oracle_solve_record(void* h, int64_t n_cells, const double* k, double* y, double dt, int64_t n_solves, double rtol,
+                          const double* atol, int32_t* states, int64_t* stats, int64_t max_records, double* tape,
+                          int64_t* counts)
+  {
+    const Mech& M = static_cast<OracleSolver*>(h)->M;
+    const double* at = atol ? atol : M.atol.data();
+    if (rtol <= 0)
+      rtol = 1e-6;
+    const int64_t rec = M.N + 1;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t c = 0; c < n_cells; ++c)
+    {
+      std::vector<double> t;
+      Stats st;
+      int64_t acc[7] = {};
+      for (int64_t rep = 0; rep < n_solves; ++rep)
+      {
+        solve_state(M, 1, k + c * M.R, y + c * M.N, dt, rtol, at, st, &t);
+        for (int j = 0; j < 7; ++j)
+          acc[j] += st.v[j];
+      }
+      const int64_t n = static_cast<int64_t>(t.size()) / rec;
+      counts[c] = n;
+      std::memcpy(tape + c * max_records * rec, t.data(), sizeof(double) * std::min(n, max_records) * rec);
+      if (states)
+        states[c] = st.state;
+      if (stats)
+        for (int j = 0; j < 7; ++j)
+          stats[c * 7 + j] = acc[j];
+    }
+    return 0;
+  }
+
+  // The recorded map applied to y (in: initial, out: final): every step with its
+  // recorded H, every clamp; the tapes' own states are not read.
+  int oracle_replay(void* h, int64_t n_cells, const double* k, double* y, int64_t max_records, const double* tape,
+                    const int64_t* counts)
+  {
+    const Mech& M = static_cast<OracleSolver*>(h)->M;
+    const int64_t N = M.N, rec = N + 1;
+    for (int64_t c = 0; c < n_cells; ++c)
+      if (counts[c] > max_records)
+      {
+        g_error = "tape of cell " + std::to_string(c) + " overflowed";
+        return -1;
+      }
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t c = 0; c < n_cells; ++c)
+    {
+      StepData w;
+      step_data_init(M, w);
+      std::vector<double> ynew(N);
+      double* yc = y + c * N;
+      for (int64_t r = 0; r < counts[c]; ++r)
+      {
+        const double* rc = tape + (c * max_records + r) * rec;
+        if (rc[0] == kClampTag)
+          for (int64_t i = 0; i < N; ++i)
+            yc[i] = std::max(yc[i], 0.0);
+        else
+        {
+          frozen_step(M, k + c * M.R, yc, rc[0], w, ynew.data());
+          std::memcpy(yc, ynew.data(), sizeof(double) * N);
+        }
+      }
+    }
+    return 0;
+  }
+
+  // Reverse sweep over the tapes: lam_final = dL/dy_final [cells][N] -> grad_y0 [cells][N]
+  // and grad_logk [cells][R] = k dL/dk.
+  int oracle_adjoint(void* h, int64_t n_cells, const double* k, int64_t max_records, const double* tape,
+                     const int64_t* counts, const double* lam_final, double* grad_y0, double* grad_logk)
+  {
+    const Mech& M = static_cast<OracleSolver*>(h)->M;
+    const int64_t N = M.N, rec = N + 1;
+    for (int64_t c = 0; c < n_cells; ++c)
+      if (counts[c] > max_records)
+      {
+        g_error = "tape of cell " + std::to_string(c) + " overflowed";
+        return -1;
+      }
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t c = 0; c < n_cells; ++c)
+    {
+      StepData w;
+      step_data_init(M, w);
+      double* lam = grad_y0 + c * N;
+      double* th = grad_logk + c * M.R;
+      std::memcpy(lam, lam_final + c * N, sizeof(double) * N);
+      std::fill(th, th + M.R, 0.0);
+      for (int64_t r = counts[c] - 1; r >= 0; --r)
+      {
+        const double* rc = tape + (c * max_records + r) * rec;
+        if (rc[0] == kClampTag)
+        {
+          for (int64_t i = 0; i < N; ++i)
+            if (rc[1 + i] < 0.0)
+              lam[i] = 0.0;
+        }
+        else
+          step_adjoint(M, k + c * M.R, rc + 1, rc[0], w, lam, th);
+      }
+    }
+    return 0;
+  }
+
+  // Tangent-linear model of the recorded map along n_dir directions per cell:
+  // dy0 [cells][n_dir][N], dlogk [cells][n_dir][R] -> dy_final [cells][n_dir][N]; y0 [cells][N].
+  int oracle_tangent(void* h, int64_t n_cells, const double* k, const double* y0, int64_t max_records,
+                     const double* tape, const int64_t* counts, int64_t n_dir, const double* dy0, const double* dlogk,
+                     double* dy_final)
+  {
+    const Mech& M = static_cast<OracleSolver*>(h)->M;
+    const int64_t N = M.N, rec = N + 1;
+    for (int64_t c = 0; c < n_cells; ++c)
+      if (counts[c] > max_records)
+      {
+        g_error = "tape of cell " + std::to_string(c) + " overflowed";
+        return -1;
+      }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t c = 0; c < n_cells; ++c)
+    {
+      StepData w;
+      step_data_init(M, w);
+      std::vector<double> y(y0 + c * N, y0 + (c + 1) * N);
+      double* dy = dy_final + c * n_dir * N;
+      std::memcpy(dy, dy0 + c * n_dir * N, sizeof(double) * n_dir * N);
+      for (int64_t r = 0; r < counts[c]; ++r)
+      {
+        const double* rc = tape + (c * max_records + r) * rec;
+        if (rc[0] == kClampTag)
+        {
+          for (int64_t i = 0; i < N; ++i)
+            if (y[i] < 0.0)
+            {
+              y[i] = 0.0;
+              for (int64_t d = 0; d < n_dir; ++d)
+                dy[d * N + i] = 0.0;
+            }
+        }
+        else
+          step_tangent(M, k + c * M.R, y.data(), rc[0], w, n_dir, dy, dlogk + c * n_dir * M.R);
+      }
+    }
+    return 0;
+  }
+}
+
+// Plants a mistake in the adjoint (tests of the checks only; 0 restores it).
+extern "C" int oracle_set_adjoint_fault(int32_t which)
+{
+  if (which < 0 || which > 3)
+    return -1;
+  g_adjoint_fault = which;
+  return 0;
 }
 
 // Worker threads of the per-cell solves (OpenMP).
