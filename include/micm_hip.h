@@ -254,6 +254,19 @@ int micm_hip_normalized_error(const micm_hip_mechanism* mechanism, int64_t n_cel
                               const double* errors, const double* absolute_tolerances,
                               double relative_tolerance, double* error, void* stream);
 
+/* NormalizedErrorDriver (cuda_rosenbrock.cuh:29-35) / NormalizedError (rosenbrock.inl:353-402) for a
+ * whole State: *error = max(sqrt(sum / (n_cells * n_species)), 1e-10), the sum added one term at a
+ * time in the order the reference's dense matrix policy visits the terms -- vector_length 1:
+ * Matrix<Real> (cell by cell, species inner); L > 1: VectorMatrix<L> (each group of L cells species
+ * by species with the group's cells inner, a partial last group over its own cells;
+ * vector_matrix.hpp:860-867).  Arrays grouped as above (the device layout is always 64-wide; the
+ * order is the reference State's); absolute_tolerances a plain [n_species] device array; error one
+ * device double. */
+int micm_hip_normalized_error_state(const micm_hip_mechanism* mechanism, int64_t n_cells,
+                                    const double* variables, const double* new_variables, const double* errors,
+                                    const double* absolute_tolerances, double relative_tolerance,
+                                    int64_t vector_length, double* error, void* stream);
+
 /* ------------------------------------------------------------ state */
 /* Solver::GetState (solver.hpp:174-195): device arrays for n_cells plus
  * integrator scratch; tolerances start at the mechanism's defaults with
@@ -272,6 +285,9 @@ int micm_hip_state_set_tolerances(micm_hip_state* state, double relative_toleran
                                   const double* absolute_tolerances /* [n_species] or NULL */);
 /* Per-cell result of the last solve: states [n_cells], stats [n_cells][7], final_time [n_cells]; any may be NULL. */
 int micm_hip_state_get_results(const micm_hip_state* state, int32_t* states, int64_t* stats, double* final_time);
+/* Summation order of the MICM_HIP_SOLVE_LOCKSTEP form's error norm: that of a reference State over
+ * VectorMatrix<vector_length> (1, the default: Matrix<Real>); see micm_hip_normalized_error_state. */
+int micm_hip_state_set_lockstep_order(micm_hip_state* state, int64_t vector_length);
 /* Stream used by the state's solves (hipStream_t; NULL = default stream). */
 int micm_hip_state_set_stream(micm_hip_state* state, void* stream);
 /* Step count at which the persistent form hands a cell to the cooperative form

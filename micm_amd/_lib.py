@@ -95,6 +95,8 @@ _SIGNATURES = {
     "micm_hip_mechanism_cooperative_supported": [_vp, ctypes.c_int32, _i32p],
     "micm_hip_update_state_parameters": [_vp, _vp],
     "micm_hip_solve": [_vp, _vp, ctypes.c_double, ctypes.POINTER(RosenbrockParameters), ctypes.c_int32],
+    "micm_hip_normalized_error_state": [_vp, _i64, _vp, _vp, _vp, _vp, ctypes.c_double, _i64, _vp, _vp],
+    "micm_hip_state_set_lockstep_order": [_vp, _i64],
     "micm_hip_state_set_variables_device": [_vp, _vp],
     "micm_hip_state_get_variables_device": [_vp, _vp],
     "micm_hip_state_set_rate_constants_device": [_vp, _vp],
@@ -264,6 +266,12 @@ class Mechanism:
     def normalized_error(self, n_cells, y, ynew, yerr, atol, rtol, error, stream=None):
         check(lib().micm_hip_normalized_error(self.h, n_cells, y, ynew, yerr, atol, float(rtol), error, stream))
 
+    def normalized_error_state(self, n_cells, y, ynew, yerr, atol, rtol, vector_length, error, stream=None):
+        """One norm for the whole State, summed in the order of VectorMatrix<vector_length>
+        (1: Matrix<Real>); `error` is the address of one device double."""
+        check(lib().micm_hip_normalized_error_state(self.h, n_cells, y, ynew, yerr, atol, float(rtol),
+                                                    int(vector_length), error, stream))
+
 
 class DeviceState:
     """Device-resident per-cell arrays and integrator scratch (micm_hip_state)."""
@@ -319,6 +327,11 @@ class DeviceState:
 
     def set_stream(self, stream):
         check(lib().micm_hip_state_set_stream(self.h, stream))
+
+    def set_lockstep_order(self, vector_length):
+        """Summation order of the lockstep form's error norm: that of VectorMatrix<vector_length>
+        (1, the default: Matrix<Real>)."""
+        check(lib().micm_hip_state_set_lockstep_order(self.h, int(vector_length)))
 
     def set_handoff(self, steps):
         """Step count at which the persistent form hands a cell to the cooperative form

@@ -117,6 +117,7 @@ namespace micm_hip
     int32_t new_f[6];
     double a[15], c[15], m[6], e[6];
     int64_t handoff_steps;  // per-lane form hands a cell to the cooperative form at this step count (0: never)
+    int64_t error_group;    // lockstep error norm: vector length of the summation order (1: Matrix<Real>)
   };
 
   // Per-cell integrator state (rosenbrock.inl:75-78 plus SolverStats).

@@ -949,16 +949,20 @@ namespace
 
   constexpr int kLockstepErrorThreads = 1024;
 
-  // Lockstep error: the reference's single running sum over cells (outer) and
-  // species (inner) of one State (rosenbrock.inl:372-401).  The terms of the
-  // flattened (cell, species) pairs are formed by the whole workgroup a block at a
-  // time and added one after another by thread 0, in that order.  Every thread of
-  // the workgroup calls it; the sum is thread 0's result.
-  __device__ inline double lockstep_error(const DevMech& m, const DevParams& p, int64_t n_cells, const Work& w,
-                                          const double* __restrict__ atol, double* terms)
+  // The reference's NormalizedError for one State (rosenbrock.inl:353-402): one running
+  // sum of the squared scaled errors in the order the State's dense matrix policy visits
+  // them, then max(sqrt(sum / (cells * species)), 1e-10).  `group` is that policy's
+  // vector length: 1 for Matrix<Real> (cell by cell, species inner), L for
+  // VectorMatrix<L> (each group of L cells species by species, the group's cells inner,
+  // a partial last group over its own cells; vector_matrix.hpp:860-867).  The terms are
+  // formed by the whole workgroup a block at a time and added one after another by
+  // thread 0.  Every thread of the workgroup calls it; the result is thread 0's.
+  __device__ inline double ordered_error(int n, int64_t n_cells, int64_t group, const double* __restrict__ y,
+                                         const double* __restrict__ ynew, const double* __restrict__ yerr,
+                                         const double* __restrict__ atol, double rtol, double* terms)
   {
-    const int n = m.n_species;
     const int64_t n_terms = n_cells * n;
+    const int64_t per_group = group * n;
     double acc = 0.0;
     for (int64_t base = 0; base < n_terms; base += kLockstepErrorThreads)
     {
@@ -966,13 +970,17 @@ namespace
       double q = 0.0;
       if (f < n_terms)
       {
-        const int64_t c = f / n;
-        const int i = static_cast<int>(f - c * n);
+        const int64_t gi = f / per_group;
+        const int64_t off = f - gi * per_group;
+        const int64_t rows = n_cells - gi * group < group ? n_cells - gi * group : group;
+        const int i = static_cast<int>(off / rows);
+        const int64_t c = gi * group + off % rows;
         const size_t g = c / kGroup;
         const int cl = c % kGroup;
-        const double a = fabs(lane_view(w.y, g, n, cl)[i]), b = fabs(lane_view(w.ynew, g, n, cl)[i]);
+        const double a = fabs(lane_view(const_cast<double*>(y), g, n, cl)[i]);
+        const double b = fabs(lane_view(const_cast<double*>(ynew), g, n, cl)[i]);
         const double ymax = a > b ? a : b;
-        const double v = lane_view(w.yerr, g, n, cl)[i] / (atol[i] + p.rtol * ymax);
+        const double v = lane_view(const_cast<double*>(yerr), g, n, cl)[i] / (atol[i] + rtol * ymax);
         q = v * v;
       }
       terms[threadIdx.x] = q;
@@ -1000,6 +1008,24 @@ namespace
     }
     const double N = static_cast<double>(n_terms > 0 ? n_terms : 1);
     return smax(sqrt(acc / N), 1.0e-10);
+  }
+
+  // Lockstep error of the state's attempt, in the order of the state's dense matrix policy.
+  __device__ inline double lockstep_error(const DevMech& m, const DevParams& p, int64_t n_cells, const Work& w,
+                                          const double* __restrict__ atol, double* terms)
+  {
+    return ordered_error(m.n_species, n_cells, p.error_group, w.y, w.ynew, w.yerr, atol, p.rtol, terms);
+  }
+
+  __global__ void __launch_bounds__(kLockstepErrorThreads) k_state_error(int n, int64_t n_cells, int64_t group,
+                                                                         const double* y, const double* ynew,
+                                                                         const double* yerr, const double* atol,
+                                                                         double rtol, double* error_out)
+  {
+    __shared__ double terms[kLockstepErrorThreads];
+    const double e = ordered_error(n, n_cells, group, y, ynew, yerr, atol, rtol, terms);
+    if (threadIdx.x == 0)
+      *error_out = e;
   }
 
   __global__ void __launch_bounds__(kLockstepErrorThreads) k_lockstep_error(DevMech m, DevParams p, int64_t n_cells,
@@ -1655,6 +1681,9 @@ struct micm_hip_state
   int coop_threads = 0;
   // the tape a recorded Solve appends to (records == nullptr otherwise)
   Tape recording{ nullptr, nullptr, 0 };
+  // vector length of the dense matrix policy whose summation order the lockstep form's
+  // error norm follows (1: Matrix<Real>)
+  int64_t lockstep_group = 1;
 };
 
 struct micm_hip_tape
@@ -1676,7 +1705,7 @@ struct micm_hip_packed
   bool captured = false, replayed = false;
   double dt = 0.0;
   micm_hip_rosenbrock_parameters params{};
-  std::vector<int64_t> shapes;  // per state: handoff, cooperative threads, staging
+  std::vector<int64_t> shapes;  // per state: launch shapes, rtol and scratch buffers the graph bakes in
 };
 
 namespace
@@ -1762,6 +1791,7 @@ namespace
     p.gamma0 = in.gamma[0];
     p.dt = dt;
     p.rtol = rtol;
+    p.error_group = 1;
     for (int i = 0; i < 6; ++i)
     {
       p.new_f[i] = in.new_function_evaluation[i];
@@ -2700,6 +2730,30 @@ extern "C"
     HIP_TRY(hipGetLastError());
     return MICM_HIP_OK;
   }
+  int micm_hip_normalized_error_state(const micm_hip_mechanism* m, int64_t n_cells, const double* y,
+                                      const double* ynew, const double* errors, const double* atol, double rtol,
+                                      int64_t vector_length, double* error, void* stream)
+  {
+    if (int rc = check_device_mech(m))
+      return rc;
+    if (n_cells < 0 || vector_length < 1)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "negative cell count or vector length below 1");
+    if (!error || (n_cells > 0 && (!y || !ynew || !errors || !atol)))
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null buffer");
+    hipLaunchKernelGGL(k_state_error, dim3(1), dim3(kLockstepErrorThreads), 0, static_cast<hipStream_t>(stream),
+                       m->dev.n_species, n_cells, vector_length, y, ynew, errors, atol, rtol, error);
+    HIP_TRY(hipGetLastError());
+    return MICM_HIP_OK;
+  }
+
+  int micm_hip_state_set_lockstep_order(micm_hip_state* s, int64_t vector_length)
+  {
+    if (!s || vector_length < 1)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null state or vector length below 1");
+    s->lockstep_group = vector_length;
+    return MICM_HIP_OK;
+  }
+
 
   // ---- state
 
@@ -2918,7 +2972,7 @@ extern "C"
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "bad state or parameters");
     if (params->stages < 1 || params->stages > 6)
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "stages must be 1..6");
-    const DevParams p = make_params(*params, time_step, s->rtol);
+    DevParams p = make_params(*params, time_step, s->rtol);
     const DevMech& d = m->dev;
     hipStream_t st = s->stream;
     const unsigned grid = grid_for(s->n_cells);
@@ -2933,6 +2987,7 @@ extern "C"
         HIP_TRY(hipGetLastError());
       }
       const bool lockstep = form == MICM_HIP_SOLVE_LOCKSTEP;
+      p.error_group = s->lockstep_group;
       Work w = state_work(s);
       hipLaunchKernelGGL(k_plain_init, dim3(grid), dim3(kBlock), 0, st, p, s->n_cells, s->ctl);
       HIP_TRY(hipGetLastError());
@@ -3255,10 +3310,18 @@ extern "C"
     {
       if (int rc = prepare_solve(s, pk->form, static_cast<int>(params->stages)))
         return rc;
+      // everything the captured launches bake in: launch shapes, the relative tolerance
+      // (a kernel argument) and the scratch buffers prepare_solve may have reallocated
+      int64_t rtol_bits = 0;
+      std::memcpy(&rtol_bits, &s->rtol, sizeof rtol_bits);
       shapes.push_back(s->handoff_steps);
       shapes.push_back(s->coop_threads);
       shapes.push_back(s->coop_stage);
       shapes.push_back(static_cast<int64_t>(s->n_slots));
+      shapes.push_back(rtol_bits);
+      shapes.push_back(static_cast<int64_t>(reinterpret_cast<intptr_t>(s->coop_k)));
+      shapes.push_back(s->coop_k_rows);
+      shapes.push_back(static_cast<int64_t>(reinterpret_cast<intptr_t>(s->slot_blob)));
     }
     const bool same = pk->captured && pk->dt == time_step && pk->shapes == shapes &&
                       std::memcmp(&pk->params, params, sizeof *params) == 0;

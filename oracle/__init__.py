@@ -86,6 +86,8 @@ class _Base:
                     "adjoint": [_vp, ctypes.c_int64, _dp, ctypes.c_int64, _dp, _ip, _dp, _dp, _dp],
                     "tangent": [_vp, ctypes.c_int64, _dp, _dp, ctypes.c_int64, _dp, _ip, ctypes.c_int64, _dp, _dp,
                                 _dp],
+                    "normalized_error_grouped": [_vp, ctypes.c_int64, _dp, _dp, _dp, ctypes.c_double, _dp,
+                                                 ctypes.c_int64, _dp],
                 })
             for name, args in sig.items():
                 f = getattr(lib, p + name)
@@ -166,6 +168,15 @@ class _Base:
         self._call("lu_solve", x.shape[0], _d(np.ascontiguousarray(lower)), _d(np.ascontiguousarray(upper)), _d(x))
         return x
 
+    def normalized_error_grouped(self, y, ynew, yerr, rtol, atol, group):
+        """The norm over all cells of one State, summed in VectorMatrix<group> order (oracle only)."""
+        y, ynew, yerr = (np.ascontiguousarray(a, dtype=np.float64) for a in (y, ynew, yerr))
+        atol = np.ascontiguousarray(atol, dtype=np.float64)
+        e = np.zeros(1)
+        self._call("normalized_error_grouped", y.shape[0], _d(y), _d(ynew), _d(yerr), float(rtol), _d(atol),
+                   int(group), _d(e))
+        return float(e[0])
+
     def normalized_error(self, y, ynew, yerr, rtol, atol):
         """The reference's norm over all cells of one State."""
         y, ynew, yerr = (np.ascontiguousarray(a, dtype=np.float64) for a in (y, ynew, yerr))
@@ -175,8 +186,11 @@ class _Base:
         return float(e[0])
 
     def solve(self, conditions, custom, y, dt, n_solves=1, lockstep=False, rtol=1e-6, atol=None):
-        """n_solves x (UpdateStateParameters, Solve(dt)); per-cell States unless lockstep.
-        Returns (y_final, states, stats summed over solves, final_time of the last solve)."""
+        """n_solves x (UpdateStateParameters, Solve(dt)); per-cell States unless lockstep
+        (True or 1: one State of every cell over Matrix<Real>; an int L > 1: over
+        VectorMatrix<L>, whose error norm is summed group by group -- the reference build
+        offers L = 4).  Returns (y_final, states, stats summed over solves, final_time of
+        the last solve)."""
         conditions = np.ascontiguousarray(conditions, dtype=np.float64)
         n = conditions.shape[0]
         custom = np.ascontiguousarray(custom, dtype=np.float64).reshape(n, -1)
@@ -185,7 +199,7 @@ class _Base:
         stats = np.zeros((n, 7), dtype=np.int64)
         ft = np.zeros(n)
         atol = None if atol is None else np.ascontiguousarray(atol, dtype=np.float64)
-        self._call("solve", n, int(bool(lockstep)), _d(conditions), _d(custom), _d(y), float(dt), int(n_solves),
+        self._call("solve", n, int(lockstep), _d(conditions), _d(custom), _d(y), float(dt), int(n_solves),
                    float(rtol), _d(atol), _i32(states), _i(stats), _d(ft))
         return y, states, stats, ft
 

@@ -683,8 +683,11 @@ namespace
   // before the clamp); each record is N + 1 doubles.
   constexpr double kClampTag = -1.0;
 
+  // `group`: vector length of the State's dense matrix policy, which fixes the order of the
+  // error norm's sum -- 1 for Matrix<Real> (cells outer, species inner), L for VectorMatrix<L>
+  // (groups of L cells, species, then the group's cells; vector_matrix.hpp:860-867).
   void solve_state(const Mech& M, int64_t nc, const double* k, double* Y, double dt, double rtol,
-                   const double* atol, Stats& st, std::vector<double>* tape = nullptr)
+                   const double* atol, Stats& st, std::vector<double>* tape = nullptr, int64_t group = 1)
   {
     const Tableau& p = M.tab;
     const int64_t N = M.N, nJ = M.nnzA;
@@ -778,16 +781,20 @@ namespace
         std::fill(Yerr.begin(), Yerr.end(), 0.0);
         for (int stage = 0; stage < p.stages; ++stage)
           axpy(Yerr.data(), p.e[stage], K[stage].data());
-        // NormalizedError: cells outer, species inner (Matrix<Real> row order)
+        // NormalizedError in the State's summation order
         double error = 0;
-        for (int64_t c = 0; c < nc; ++c)
+        for (int64_t g0 = 0; g0 < nc; g0 += group)
+        {
+          const int64_t rows = std::min(group, nc - g0);
           for (int64_t i = 0; i < N; ++i)
-          {
-            double y = Y[c * N + i], yn = Ynew[c * N + i];
-            double ymax = (std::abs(y) > std::abs(yn) ? std::abs(y) : std::abs(yn));
-            double q = Yerr[c * N + i] / (atol[i] + rtol * ymax);
-            error += q * q;
-          }
+            for (int64_t c = g0; c < g0 + rows; ++c)
+            {
+              double y = Y[c * N + i], yn = Ynew[c * N + i];
+              double ymax = (std::abs(y) > std::abs(yn) ? std::abs(y) : std::abs(yn));
+              double q = Yerr[c * N + i] / (atol[i] + rtol * ymax);
+              error += q * q;
+            }
+        }
         const double Ntot = static_cast<double>(std::max<int64_t>(1, nc * N));
         error = std::max(std::sqrt(error / Ntot), 1.0e-10);
         double fac = std::min(p.factor_max, std::max(p.factor_min, p.safety / RC_POW(error, 1 / p.elo)));
@@ -1321,6 +1328,32 @@ extern "C"
     return 0;
   }
 
+  // The same in the summation order of VectorMatrix<group> (group 1: Matrix<Real>).
+  int oracle_normalized_error_grouped(void* h, int64_t n_cells, const double* y, const double* ynew,
+                                      const double* yerr, double rtol, const double* atol, int64_t group,
+                                      double* error)
+  {
+    const Mech& M = static_cast<OracleSolver*>(h)->M;
+    double e = 0;
+    for (int64_t g0 = 0; g0 < n_cells; g0 += group)
+    {
+      const int64_t rows = std::min(group, n_cells - g0);
+      for (int64_t i = 0; i < M.N; ++i)
+        for (int64_t c = g0; c < g0 + rows; ++c)
+        {
+          double a = y[c * M.N + i], b = ynew[c * M.N + i];
+          double ymax = (std::abs(a) > std::abs(b) ? std::abs(a) : std::abs(b));
+          double q = yerr[c * M.N + i] / (atol[i] + rtol * ymax);
+          e += q * q;
+        }
+    }
+    const double Ntot = static_cast<double>(std::max<int64_t>(1, n_cells * M.N));
+    *error = std::max(std::sqrt(e / Ntot), 1.0e-10);
+    return 0;
+  }
+
+  // lockstep: 0 one 1-cell State per cell; 1 one State of every cell (Matrix<Real> order);
+  // L > 1 one State of every cell summing its error norm in VectorMatrix<L> order.
   int oracle_solve(void* h, int64_t n_cells, int32_t lockstep, const double* conditions, const double* custom, double* y,
                    double dt, int64_t n_solves, double rtol, const double* atol, int32_t* states, int64_t* stats,
                    double* final_time)
@@ -1342,7 +1375,7 @@ extern "C"
             const double* cd = conditions + 3 * (c0 + c);
             k[c * M.R + r] = rate_constant(M.rx[r], cd[0], cd[1], cd[2], custom + (c0 + c) * M.P);
           }
-        solve_state(M, nc, k.data(), y + c0 * M.N, dt, rtol, at, st);
+        solve_state(M, nc, k.data(), y + c0 * M.N, dt, rtol, at, st, nullptr, lockstep > 1 ? lockstep : 1);
         for (int j = 0; j < 7; ++j)
           acc[j] += st.v[j];
       }

@@ -30,6 +30,12 @@ namespace
   using Builder = micm::CpuSolverBuilder<Params>;
   using SolverT = decltype(std::declval<Builder&>().Build());
   using StateT = decltype(std::declval<SolverT&>().GetState(1));
+  // the same solver over vector-ordered matrices (VectorMatrix<L>, SparseMatrixVectorOrdering<L>),
+  // whose multi-cell State sums its error norm group by group
+  constexpr std::size_t kRefVectorLength = 4;
+  using BuilderV = micm::CpuSolverBuilder<Params, micm::VectorMatrix<double, kRefVectorLength>,
+                                          micm::SparseMatrix<double, micm::SparseMatrixVectorOrdering<kRefVectorLength>>>;
+  using SolverV = decltype(std::declval<BuilderV&>().Build());
 
   Params preset(int32_t p)
   {
@@ -47,6 +53,7 @@ namespace
   struct RefSolver
   {
     std::unique_ptr<SolverT> solver;
+    std::unique_ptr<SolverV> solver_v;
     int64_t n_species = 0, n_reactions = 0, n_custom = 0;
     std::vector<std::string> species_names;          // declared order
     std::vector<std::string> custom_labels;          // in our custom-parameter order
@@ -142,10 +149,11 @@ namespace
     return b.Build();
   }
 
-  StateT make_state(RefSolver* s, int64_t n_cells, const double* conditions, const double* custom,
-                    double rtol, const double* atol)
+  template<class Solver>
+  auto make_state_of(RefSolver* s, Solver& solver, int64_t n_cells, const double* conditions, const double* custom,
+                     double rtol, const double* atol)
   {
-    StateT state = s->solver->GetState(n_cells);
+    auto state = solver.GetState(n_cells);
     for (int64_t c = 0; c < n_cells; ++c)
     {
       state.conditions_[c].temperature_ = conditions[c * 3 + 0];
@@ -163,7 +171,72 @@ namespace
     }
     return state;
   }
+
+  StateT make_state(RefSolver* s, int64_t n_cells, const double* conditions, const double* custom,
+                    double rtol, const double* atol)
+  {
+    return make_state_of(s, *s->solver, n_cells, conditions, custom, rtol, atol);
+  }
 }  // namespace
+
+
+namespace
+{
+  template<class Solver>
+  int solve_with(RefSolver* s, Solver& solver, bool lockstep, int64_t n_cells, const double* conditions,
+                 const double* custom, double* y, double dt, int64_t n_solves, double rtol, const double* atol,
+                 int32_t* states, int64_t* stats, double* final_time)
+  {
+    const int64_t N = s->n_species;
+    auto run = [&](int64_t c0, int64_t nc)
+    {
+      auto st = make_state_of(s, solver, nc, conditions + 3 * c0, custom + s->n_custom * c0, rtol, atol);
+      for (int64_t c = 0; c < nc; ++c)
+        for (int64_t i = 0; i < N; ++i)
+          st.variables_[c][i] = y[(c0 + c) * N + i];
+      std::vector<int64_t> acc(7, 0);
+      micm::SolverResult res{};
+      for (int64_t k = 0; k < n_solves; ++k)
+      {
+        solver.UpdateStateParameters(st);
+        res = solver.Solve(dt, st);
+        acc[0] += res.stats_.function_calls_;
+        acc[1] += res.stats_.jacobian_updates_;
+        acc[2] += res.stats_.number_of_steps_;
+        acc[3] += res.stats_.accepted_;
+        acc[4] += res.stats_.rejected_;
+        acc[5] += res.stats_.decompositions_;
+        acc[6] += res.stats_.solves_;
+      }
+      for (int64_t c = 0; c < nc; ++c)
+      {
+        for (int64_t i = 0; i < N; ++i)
+          y[(c0 + c) * N + i] = st.variables_[c][i];
+        if (states)
+          states[c0 + c] = static_cast<int32_t>(res.state_);
+        if (stats)
+          for (int j = 0; j < 7; ++j)
+            stats[(c0 + c) * 7 + j] = acc[j];
+        if (final_time)
+          final_time[c0 + c] = res.stats_.final_time_;
+      }
+    };
+    try
+    {
+      if (lockstep)
+        run(0, n_cells);
+      else
+        for (int64_t c = 0; c < n_cells; ++c)
+          run(c, 1);
+    }
+    catch (const std::exception& e)
+    {
+      g_error = e.what();
+      return -1;
+    }
+    return 0;
+  }
+}
 
 extern "C"
 {
@@ -227,6 +300,12 @@ extern "C"
                                                 .SetReorderState(d->reorder_state != 0)
                                                 .SetIgnoreUnusedSpecies(d->ignore_unused_species != 0)
                                                 .Build());
+      s->solver_v = std::make_unique<SolverV>(BuilderV(s->params)
+                                                  .SetSystem(micm::System(gas))
+                                                  .SetReactions(processes)
+                                                  .SetReorderState(d->reorder_state != 0)
+                                                  .SetIgnoreUnusedSpecies(d->ignore_unused_species != 0)
+                                                  .Build());
       return s.release();
     }
     catch (const std::exception& e)
@@ -402,6 +481,7 @@ extern "C"
 
   // n_solves consecutive (UpdateStateParameters, Solve(dt)) per cell.
   // lockstep = 1: one State holding every cell (the reference's multi-cell semantics);
+  // lockstep = 4: the same over VectorMatrix<4> (the error norm summed group by group);
   // lockstep = 0: one 1-cell State per cell.  y is in/out.  states [cell] is the last
   // solve's SolverState, stats [cell][7] are summed over solves, final_time [cell] of the last solve.
   int ref_solve(void* h, int64_t n_cells, int32_t lockstep, const double* conditions, const double* custom,
@@ -409,53 +489,15 @@ extern "C"
                 int64_t* stats, double* final_time)
   {
     auto* s = static_cast<RefSolver*>(h);
-    const int64_t N = s->n_species;
-    auto run = [&](int64_t c0, int64_t nc)
+    if (lockstep == static_cast<int32_t>(kRefVectorLength))
+      return solve_with(s, *s->solver_v, true, n_cells, conditions, custom, y, dt, n_solves, rtol, atol, states,
+                        stats, final_time);
+    if (lockstep != 0 && lockstep != 1)
     {
-      StateT st = make_state(s, nc, conditions + 3 * c0, custom + s->n_custom * c0, rtol, atol);
-      for (int64_t c = 0; c < nc; ++c)
-        for (int64_t i = 0; i < N; ++i)
-          st.variables_[c][i] = y[(c0 + c) * N + i];
-      std::vector<int64_t> acc(7, 0);
-      micm::SolverResult res{};
-      for (int64_t k = 0; k < n_solves; ++k)
-      {
-        s->solver->UpdateStateParameters(st);
-        res = s->solver->Solve(dt, st);
-        acc[0] += res.stats_.function_calls_;
-        acc[1] += res.stats_.jacobian_updates_;
-        acc[2] += res.stats_.number_of_steps_;
-        acc[3] += res.stats_.accepted_;
-        acc[4] += res.stats_.rejected_;
-        acc[5] += res.stats_.decompositions_;
-        acc[6] += res.stats_.solves_;
-      }
-      for (int64_t c = 0; c < nc; ++c)
-      {
-        for (int64_t i = 0; i < N; ++i)
-          y[(c0 + c) * N + i] = st.variables_[c][i];
-        if (states)
-          states[c0 + c] = static_cast<int32_t>(res.state_);
-        if (stats)
-          for (int j = 0; j < 7; ++j)
-            stats[(c0 + c) * 7 + j] = acc[j];
-        if (final_time)
-          final_time[c0 + c] = res.stats_.final_time_;
-      }
-    };
-    try
-    {
-      if (lockstep)
-        run(0, n_cells);
-      else
-        for (int64_t c = 0; c < n_cells; ++c)
-          run(c, 1);
-    }
-    catch (const std::exception& e)
-    {
-      g_error = e.what();
+      g_error = "lockstep: 0, 1 or " + std::to_string(kRefVectorLength);
       return -1;
     }
-    return 0;
+    return solve_with(s, *s->solver, lockstep != 0, n_cells, conditions, custom, y, dt, n_solves, rtol, atol, states,
+                      stats, final_time);
   }
 }

@@ -18,7 +18,7 @@ sys.path.insert(0, os.path.dirname(HERE))
 
 import subprocess  # noqa: E402
 
-from mechanisms import TS1_JSON, all_rate_types, ts1_inputs, ts1_mechanism  # noqa: E402
+from mechanisms import TS1_JSON, all_rate_types, all_types_inputs, ts1_inputs, ts1_mechanism  # noqa: E402
 from micm_amd import mechanism_io, synthetic  # noqa: E402
 from oracle import Reference, reference_available  # noqa: E402
 

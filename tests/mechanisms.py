@@ -30,6 +30,19 @@ def all_rate_types():
     return MechanismDesc(8, rx, parameterized=[(0.0, 0.0, 0.0, 1.0), (0.5, 1.0e-3, 0.0, 0.0)])
 
 
+def all_types_inputs(desc, n, seed=5):
+    """Cells for `all_rate_types`: T 200-320 K, P 5-110 kPa, log-uniform custom parameters
+    (photolysis 1e-6..0.1 s-1, surface radius 1e-8..1e-5 m and number density 1e6..1e10)
+    and concentrations 1e-8..1."""
+    st = synthetic.CellStreams(seed, n)
+    T = st.uniform(200.0, 320.0)
+    P = st.uniform(5e3, 1.1e5)
+    cond = np.stack([T, P, P / (synthetic.GAS_CONSTANT * T)], 1)
+    custom = np.stack([st.log_uniform(1e-6, 1e-1), st.log_uniform(1e-8, 1e-5), st.log_uniform(1e6, 1e10)], 1)
+    y = np.stack([st.log_uniform(1e-8, 1.0) for _ in range(desc.n_species)], 1)
+    return synthetic.CellInputs(cond, y, custom, seed)
+
+
 TS1_JSON = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ts1_mechanism.json")
 
 

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from helpers import bits_equal, dev, dev_zeros, host
-from mechanisms import all_rate_types, ts1_inputs, ts1_mechanism
+from mechanisms import all_rate_types, all_types_inputs, ts1_inputs, ts1_mechanism
 from micm_amd import _lib, synthetic
 from oracle import OraclePortableMath, Tape
 
@@ -23,19 +23,9 @@ pytestmark = pytest.mark.gpu
 TARGETS = [3, 10, 20, 33, 50]
 
 
-def _all_types_inputs(desc, n, seed=5):
-    st = synthetic.CellStreams(seed, n)
-    T = st.uniform(200.0, 320.0)
-    P = st.uniform(5e3, 1.1e5)
-    cond = np.stack([T, P, P / (synthetic.GAS_CONSTANT * T)], 1)
-    custom = np.stack([st.log_uniform(1e-6, 1e-1), st.log_uniform(1e-8, 1e-5), st.log_uniform(1e6, 1e10)], 1)
-    y = np.stack([st.log_uniform(1e-8, 1.0) for _ in range(desc.n_species)], 1)
-    return synthetic.CellInputs(cond, y, custom, seed)
-
-
 CASES = {
     "readme": (synthetic.readme_mechanism(), lambda d, n: synthetic.readme_inputs(n, seed=3), 3, 500.0, 3),
-    "all_types": (all_rate_types(), _all_types_inputs, 4, 60.0, 2),
+    "all_types": (all_rate_types(), all_types_inputs, 4, 60.0, 2),
     "60_200": (synthetic.mechanism_60_200(), synthetic.inputs_60_200, 4, 300.0, 2),
     "60_200_2stage": (synthetic.mechanism_60_200(), synthetic.inputs_60_200, 2, 300.0, 1),
     "210_560": (synthetic.mechanism_210_560(), synthetic.inputs_210_560, 4, 900.0, 1),
@@ -68,6 +58,16 @@ def _adjoint(state, tape, params, lam):
     gy, gk = dev_zeros(n, N), dev_zeros(n, R)
     state.adjoint(params, tape, seed.data_ptr(), gy.data_ptr(), gk.data_ptr())
     return host(gy, n, N), host(gk, n, R)
+
+
+def _smooth_cells(tape, dt):
+    """Cells none of whose recorded steps is below h_min (1e-15 dt, rosenbrock.inl:33)."""
+    counts = tape.counts()
+    out = np.zeros(tape.n_cells, dtype=bool)
+    for c in range(tape.n_cells):
+        h = tape.records([c])[0, :counts[c], 0]
+        out[c] = not np.any((h > 0) & (h < 1e-15 * dt))
+    return out
 
 
 def _host_tape(tape, cells_idx=None):
@@ -111,7 +111,9 @@ def test_adjoint_matches_oracle(name):
     n = 24 if name in ("210_560", "ts1") else 70
     print(f"adjoint vs oracle: {name} n={n}", flush=True)
     cells = inputs(desc, n)
-    state, tape, params = _record(mech, cells, tableau, dt, n_solves, 1024)
+    # room for every accepted step a Solve can take (max_number_of_steps) and its clamp
+    state, tape, params = _record(mech, cells, tableau, dt, n_solves, 1002 * n_solves)
+    assert tape.overflowed() == 0
     rng = np.random.default_rng(3)
     lam = rng.standard_normal((n, mech.n_species))
     gy, gk = _adjoint(state, tape, params, lam)
@@ -119,12 +121,20 @@ def test_adjoint_matches_oracle(name):
     k = state.get_rate_constants()
     gy_ref, gk_ref = o.adjoint(k, _host_tape(tape), lam)
     y0 = cells.variables[:, mech.species_order()]
-    # relative sensitivities (d/dlog y0) and d/dlog k, normwise per cell
+    # relative sensitivities (d/dlog y0) and d/dlog k, normwise per cell.  A cell whose
+    # step size fell below h_min had steps accepted whatever their error (rosenbrock.inl:
+    # 229: `H < h_min`); its map is unstable (sensitivities of 1e30 and more, overflowing
+    # on some TS1 cells, where the oracle's own tangent and adjoint part ways), so such
+    # cells are left out.
+    smooth = _smooth_cells(tape, dt)
+    assert smooth.mean() >= 0.75
     for a, b in ((gy * y0, gy_ref * y0), (gk, gk_ref)):
+        assert np.all(np.isfinite(a[smooth]))
         scale = np.maximum(np.abs(b).max(axis=1), 1e-300)
         err = np.abs(a - b).max(axis=1) / scale
-        print(f"  max normwise difference {err.max():.2e}", flush=True)
-        assert np.all(err <= 1e-9)
+        print(f"  max normwise difference {err[smooth].max():.2e} ({smooth.sum()} cells without forced steps), "
+              f"{err.max():.2e} (all)", flush=True)
+        assert np.all(err[smooth] <= 1e-9)
 
 
 def test_overflowed_tape_gives_nan():

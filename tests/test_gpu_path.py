@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 from helpers import bits_equal, dev, dev_zeros, host, stream
-from mechanisms import all_rate_types, ts1_inputs, ts1_mechanism
+from mechanisms import all_rate_types, all_types_inputs, ts1_inputs, ts1_mechanism
 from micm_amd import _lib, synthetic
 from oracle import Oracle, OraclePortableMath
 
@@ -29,22 +29,12 @@ pytestmark = pytest.mark.gpu
 def _cases():
     return {
         "readme": (synthetic.readme_mechanism(), lambda d, n: synthetic.readme_inputs(n, seed=3), 3, 500.0),
-        "all_types": (all_rate_types(), _all_types_inputs, 4, 60.0),
+        "all_types": (all_rate_types(), all_types_inputs, 4, 60.0),
         "60_200": (synthetic.mechanism_60_200(), synthetic.inputs_60_200, 4, 300.0),
         "210_560": (synthetic.mechanism_210_560(), synthetic.inputs_210_560, 4, 900.0),
         # the reference's TS1 benchmark mechanism with its benchmark's 3-stage solver and 30 s step
         "ts1": (ts1_mechanism(), ts1_inputs, 3, 30.0),
     }
-
-
-def _all_types_inputs(desc, n, seed=5):
-    st = synthetic.CellStreams(seed, n)
-    T = st.uniform(200.0, 320.0)
-    P = st.uniform(5e3, 1.1e5)
-    cond = np.stack([T, P, P / (synthetic.GAS_CONSTANT * T)], 1)
-    custom = np.stack([st.log_uniform(1e-6, 1e-1), st.log_uniform(1e-8, 1e-5), st.log_uniform(1e6, 1e10)], 1)
-    y = np.stack([st.log_uniform(1e-8, 1.0) for _ in range(desc.n_species)], 1)
-    return synthetic.CellInputs(cond, y, custom, seed)
 
 
 CASES = _cases()
@@ -123,10 +113,12 @@ def test_kernels_match_oracle(case, n_cells):
     assert bits_equal(e_dev.cpu().numpy(), e_ref)
 
 
-def _solve(mech, cells, order, tableau, dt, form, n_solves=1, rate_constants=None, handoff=-1, coop_shape=(0, -1)):
+def _solve(mech, cells, order, tableau, dt, form, n_solves=1, rate_constants=None, handoff=-1, coop_shape=(0, -1),
+           lockstep_order=1):
     """n_solves x (UpdateStateParameters, Solve); with rate_constants given, those are
     loaded instead of being computed on the device."""
     state = _lib.DeviceState(mech, cells.n_cells)
+    state.set_lockstep_order(lockstep_order)
     state.set_handoff(handoff)
     state.set_cooperative_shape(*coop_shape)
     state.set_variables(cells.variables[:, order])
@@ -358,3 +350,60 @@ def test_lockstep_solve_matches_oracle():
     assert np.all(st_gpu == 2) and np.all(st_ref == 2)
     assert np.array_equal(stats_gpu, stats_ref)
     assert bits_equal(y_gpu, y_ref)
+
+
+@pytest.mark.parametrize("name,n_cells,group", [("60_200", 130, 4), ("readme", 1000, 4), ("all_types", 70, 16),
+                                                ("60_200", 130, 64)])
+def test_lockstep_solve_vector_matrix_order(name, n_cells, group):
+    """A multi-cell State over VectorMatrix<L> sums its error norm group by group; the
+    lockstep form in that order equals the oracle's State of the same order bit for bit
+    (itself bitwise with the reference's own VectorMatrix<4> State, tests/test_oracle.py),
+    and differs from the Matrix<Real> order."""
+    desc, inputs, tableau, dt = CASES[name]
+    mech = _lib.Mechanism(desc)
+    cells = inputs(desc, n_cells)
+    order = mech.species_order()
+    print(f"lockstep VectorMatrix<{group}> order: {name} n_cells={n_cells}", flush=True)
+    y_gpu, st_gpu, stats_gpu, ft_gpu = _solve(mech, cells, order, tableau, dt, _lib.SOLVE_LOCKSTEP, n_solves=2,
+                                              lockstep_order=group)
+    y_ref, st_ref, stats_ref, ft_ref = OraclePortableMath(desc, tableau).solve(
+        cells.conditions, cells.custom, cells.variables[:, order], dt, 2, lockstep=group)
+    assert np.array_equal(st_gpu, st_ref) and np.array_equal(stats_gpu, stats_ref)
+    assert bits_equal(ft_gpu, ft_ref) and bits_equal(y_gpu, y_ref)
+
+
+@pytest.mark.parametrize("n_cells", [1, 64, 301, 5000])
+@pytest.mark.parametrize("group", [1, 4, 64, 100])
+def test_state_normalized_error(n_cells, group):
+    """micm_hip_normalized_error_state: one scalar per State, bit for bit with the
+    oracle's norm in the same order (Matrix<Real> for 1: the reference's own norm,
+    tests/test_oracle.py)."""
+    desc = synthetic.mechanism_60_200()
+    mech = _lib.Mechanism(desc)
+    o = OraclePortableMath(desc, 4)
+    rng = np.random.default_rng(n_cells + group)
+    N = mech.n_species
+    y = rng.lognormal(-8, 3, (n_cells, N))
+    ynew = y * (1 + 1e-3 * rng.standard_normal((n_cells, N)))
+    err = 1e-6 * rng.standard_normal((n_cells, N)) * y
+    atol = np.full(N, 1e-12)
+    e_dev = torch_zeros(1)
+    bufs = [dev(a) for a in (y, ynew, err)]
+    atol_dev = torch_tensor(atol)
+    mech.normalized_error_state(n_cells, *[b.data_ptr() for b in bufs], atol_dev.data_ptr(), 1e-4, group,
+                                e_dev.data_ptr(), stream())
+    e_gpu = float(e_dev.cpu().numpy()[0])
+    e_ref = o.normalized_error_grouped(y, ynew, err, 1e-4, atol, group)
+    assert bits_equal(np.array([e_gpu]), np.array([e_ref]))
+    if group == 1:
+        assert e_ref == o.normalized_error(y, ynew, err, 1e-4, atol)
+
+
+def torch_zeros(n):
+    import torch
+    return torch.zeros(n, dtype=torch.float64, device="cuda")
+
+
+def torch_tensor(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to("cuda")

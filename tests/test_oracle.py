@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from helpers import bits_equal
-from mechanisms import all_rate_types, ts1_inputs, ts1_mechanism
+from mechanisms import all_rate_types, all_types_inputs, ts1_inputs, ts1_mechanism
 from micm_amd import synthetic
 from oracle import Oracle, Reference, reference_available
 
@@ -67,8 +67,7 @@ def test_oracle_matches_live_reference(name):
     if name == "readme":
         cells = synthetic.readme_inputs(n, seed=9)
     elif name == "all_types":
-        from test_gpu_path import _all_types_inputs
-        cells = _all_types_inputs(desc, n, seed=9)
+        cells = all_types_inputs(desc, n, seed=9)
     elif name == "ts1":
         cells = ts1_inputs(desc, n, seed=9)
     else:
@@ -109,8 +108,7 @@ def test_readme_table():
 
 
 def _all_types_cells(n=32, seed=13):
-    from test_gpu_path import _all_types_inputs
-    return _all_types_inputs(all_rate_types(), n, seed)
+    return all_types_inputs(all_rate_types(), n, seed)
 
 
 def test_rate_constants_closed_forms():
@@ -250,3 +248,36 @@ def test_analytical_two_step_decay():
             got[:, order] = y
             assert np.all(st == 2)
             np.testing.assert_allclose(got, np.stack([A, B, C], 1), atol=2e-7, rtol=0)
+
+
+@pytest.mark.skipif(not reference_available(), reason="reference build (oracle/_ref) not available")
+@pytest.mark.parametrize("group", [1, 4])
+def test_lockstep_orders_match_live_reference(group):
+    """A multi-cell State over Matrix<Real> and over VectorMatrix<4>: the oracle's lockstep
+    solve in the same summation order equals the reference's own State bit for bit, and
+    the two orders give different step sequences on this mechanism (so the order is
+    observable and pinned)."""
+    desc = synthetic.mechanism_60_200()
+    cells = synthetic.inputs_60_200(desc, 130)
+    o, r = Oracle(desc, 4), Reference(desc, 4)
+    y0 = cells.variables[:, o.species_order()]
+    yo, so, sto, fto = o.solve(cells.conditions, cells.custom, y0, 300.0, 1, lockstep=group)
+    yr, sr, str_, ftr = r.solve(cells.conditions, cells.custom, y0, 300.0, 1, lockstep=group)
+    assert bits_equal(yo, yr) and np.array_equal(sto, str_) and np.array_equal(so, sr)
+    other = 4 if group == 1 else 1
+    _, _, st_other, _ = o.solve(cells.conditions, cells.custom, y0, 300.0, 1, lockstep=other)
+    assert not np.array_equal(sto, st_other)
+
+
+def test_grouped_norm_with_group_one_is_the_reference_norm():
+    desc = synthetic.mechanism_60_200()
+    o = Oracle(desc, 4)
+    rng = np.random.default_rng(2)
+    y = rng.lognormal(-8, 3, (77, o.n_species))
+    ynew = y * 1.001
+    err = 1e-6 * y * rng.standard_normal(y.shape)
+    atol = np.full(o.n_species, 1e-12)
+    assert o.normalized_error_grouped(y, ynew, err, 1e-4, atol, 1) == o.normalized_error(y, ynew, err, 1e-4, atol)
+    if reference_available():
+        assert o.normalized_error(y, ynew, err, 1e-4, atol) == Reference(desc, 4).normalized_error(y, ynew, err, 1e-4,
+                                                                                                  atol)
