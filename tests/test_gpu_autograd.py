@@ -43,6 +43,11 @@ def test_forward_is_the_solve():
 
 
 def test_gradients_match_directional_differences():
+    """Directional central differences (h = 1e-6) of the binding's own forward.  Each
+    forward records anew, so these differences include how the adaptive step sizes move
+    with the input, which the frozen-step adjoint leaves out (measured ~1e-6 relative here);
+    hence 1e-5.  The 1e-6 check against differences of the recorded map itself is
+    tests/test_gpu_adjoint.py::test_differentiation_workload."""
     import torch
     desc, cells, solve, y0 = _setup()
     y0 = y0.clone().requires_grad_(True)
@@ -59,12 +64,12 @@ def test_gradients_match_directional_differences():
             fd = (_loss(solve(y0 + h * vy, log_a)) - _loss(solve(y0 - h * vy, log_a))) / (2 * h)
             ad = (gy * vy).sum()
             print(f"  y0 direction: fd {fd.item():.10e} adjoint {ad.item():.10e}", flush=True)
-            assert abs(fd.item() - ad.item()) <= 1e-6 * abs(fd.item())
+            assert abs(fd.item() - ad.item()) <= 1e-5 * abs(fd.item())
             va = torch.randn(solve.n_reactions, generator=g, dtype=torch.float64).to("cuda")
             fd = (_loss(solve(y0, log_a + h * va)) - _loss(solve(y0, log_a - h * va))) / (2 * h)
             ad = (ga * va).sum()
             print(f"  log A direction: fd {fd.item():.10e} adjoint {ad.item():.10e}", flush=True)
-            assert abs(fd.item() - ad.item()) <= 1e-6 * abs(fd.item())
+            assert abs(fd.item() - ad.item()) <= 1e-5 * abs(fd.item())
 
 
 def test_per_cell_parameters_sum_to_the_shared_gradient():
