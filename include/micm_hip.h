@@ -316,6 +316,17 @@ int micm_hip_update_state_parameters(const micm_hip_mechanism* mechanism, micm_h
 int micm_hip_solve(const micm_hip_mechanism* mechanism, micm_hip_state* state, double time_step,
                    const micm_hip_rosenbrock_parameters* parameters, int32_t form);
 
+/* Solver::Solve in the cooperative form (MICM_HIP_SOLVE_COOPERATIVE, flags 0 or
+ * MICM_HIP_SOLVE_UPDATE_RATES) without allocating: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under stream capture, tight serving loops).  The
+ * launch's scratch (queue counter, the compact layout's rate-constant rows) lives in one static
+ * 4 MiB device array per GPU; a launch shape needing more is refused with
+ * MICM_HIP_ERR_INVALID_ARGUMENT.  Enqueued on the state's stream without waiting; results are
+ * those of micm_hip_solve once the stream has finished.  Every call on one GPU shares that
+ * array: calls that overlap on a GPU (other threads or streams) race. */
+int micm_hip_solve_noalloc(const micm_hip_mechanism* mechanism, micm_hip_state* state, double time_step,
+                           const micm_hip_rosenbrock_parameters* parameters, int32_t flags);
+
 /* ---------------------------------------------------------------- device-to-device transfers
  * Grouped device arrays of n_groups * 64 rows, on the state's stream; the call waits for the copy.
  * CudaDenseMatrix::CopyToDevice / CopyToHost (cuda_dense_matrix.hpp:46-50) for callers whose

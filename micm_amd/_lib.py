@@ -97,6 +97,7 @@ _SIGNATURES = {
     "micm_hip_solve": [_vp, _vp, ctypes.c_double, ctypes.POINTER(RosenbrockParameters), ctypes.c_int32],
     "micm_hip_normalized_error_state": [_vp, _i64, _vp, _vp, _vp, _vp, ctypes.c_double, _i64, _vp, _vp],
     "micm_hip_state_set_lockstep_order": [_vp, _i64],
+    "micm_hip_solve_noalloc": [_vp, _vp, ctypes.c_double, ctypes.POINTER(RosenbrockParameters), ctypes.c_int32],
     "micm_hip_state_set_variables_device": [_vp, _vp],
     "micm_hip_state_get_variables_device": [_vp, _vp],
     "micm_hip_state_set_rate_constants_device": [_vp, _vp],
@@ -355,6 +356,12 @@ class DeviceState:
 
     def solve(self, time_step, parameters, form=SOLVE_PLAIN):
         check(lib().micm_hip_solve(self.mechanism.h, self.h, float(time_step), ctypes.byref(parameters), int(form)))
+
+    def solve_noalloc(self, time_step, parameters, update_rates=False):
+        """The cooperative Solve without device allocation, enqueued on the state's stream
+        (results valid once that stream has finished)."""
+        check(lib().micm_hip_solve_noalloc(self.mechanism.h, self.h, float(time_step), ctypes.byref(parameters),
+                                           SOLVE_UPDATE_RATES if update_rates else 0))
 
     # ---- device-resident transfers (grouped device arrays given by address)
     def set_variables_device(self, ptr):

@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1635,6 +1636,41 @@ namespace
 
 // ====================================================================== C-ABI
 
+// Scratch of micm_hip_solve_noalloc: the cooperative launch's queue counter and the compact
+// layout's per-workgroup rate-constant rows.  Each GPU holds its own copy until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress reports for the
+// current device (noalloc_scratch).
+constexpr size_t kNoAllocScratchBytes = size_t(4) << 20;
+constexpr size_t kNoAllocCounterBytes = 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace
+{
+  constexpr int kMaxDevices = 64;
+  std::mutex g_scratch_mutex;
+  void* g_scratch_address[kMaxDevices] = {};
+
+  // The current device's scratch address, looked up once per device.
+  int noalloc_scratch(void** out)
+  {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+      return fail(MICM_HIP_ERR_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_address[dev])
+    {
+      void* p = nullptr;
+      HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      if (!p)
+        return fail(MICM_HIP_ERR_DEVICE, "no address for the scratch array");
+      g_scratch_address[dev] = p;
+    }
+    *out = g_scratch_address[dev];
+    return MICM_HIP_OK;
+  }
+}  // namespace
+
 struct micm_hip_mechanism
 {
   MechanismTables t;
@@ -1684,6 +1720,10 @@ struct micm_hip_state
   // vector length of the dense matrix policy whose summation order the lockstep form's
   // error norm follows (1: Matrix<Real>)
   int64_t lockstep_group = 1;
+  // scratch of the launch being enqueued by micm_hip_solve_noalloc (nullptr otherwise)
+  int* ext_next = nullptr;
+  double* ext_k = nullptr;
+  int64_t ext_k_rows = 0;
 };
 
 struct micm_hip_tape
@@ -2246,16 +2286,21 @@ namespace
     int64_t blocks = 0;
     if (int rc = coop_blocks(plan, blocks))
       return rc;
-    if (COMPACT && s->coop_k_rows < blocks)
+    // the queue counter and the compact layout's rate-constant rows: the state's own, or
+    // the no-allocation solve's static scratch
+    const bool ext = s->ext_next != nullptr;
+    int* next = ext ? s->ext_next : s->coop_next;
+    double* k_rows = ext ? s->ext_k : s->coop_k;
+    if (COMPACT && (ext ? s->ext_k_rows : s->coop_k_rows) < blocks)
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "cooperative scratch not prepared");
     if (!list)
       blocks = std::max<int64_t>(1, std::min(blocks, n_fixed));
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, s->coop_next);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, next);
     hipLaunchKernelGGL((k_coop<NT, STAGE, COMPACT, MINW>), dim3(static_cast<unsigned>(blocks)), dim3(NT), plan.lds, st,
                        m->dev,
-                       m->coop, p, plan.p_capacity, n_fixed, list, count, s->coop_next, resume, s->y, s->k, s->atol,
+                       m->coop, p, plan.p_capacity, n_fixed, list, count, next, resume, s->y, s->k, s->atol,
                        s->status, s->stats, s->final_time, update_rates ? s->cond : nullptr,
-                       update_rates ? s->custom : nullptr, s->coop_k, s->recording);
+                       update_rates ? s->custom : nullptr, k_rows, s->recording);
     HIP_TRY(hipGetLastError());
     return MICM_HIP_OK;
   }
@@ -3087,6 +3132,45 @@ extern "C"
       return MICM_HIP_OK;
     }
     return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "unknown solve form");
+  }
+
+  int micm_hip_solve_noalloc(const micm_hip_mechanism* m, micm_hip_state* s, double time_step,
+                             const micm_hip_rosenbrock_parameters* params, int32_t flags)
+  {
+    if (int rc = check_mech(m))
+      return rc;
+    if (!s || s->mech != m || !params)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "bad state or parameters");
+    if ((flags & ~MICM_HIP_SOLVE_UPDATE_RATES) != 0)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "flags: only MICM_HIP_SOLVE_UPDATE_RATES");
+    if (params->stages < 1 || params->stages > 6)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "stages must be 1..6");
+    const CoopPlan plan = plan_coop(m, static_cast<int>(params->stages), s->coop_stage, s->coop_threads);
+    if (plan.lds == 0)
+      return fail(MICM_HIP_ERR_UNSUPPORTED, "the cooperative form does not fit this mechanism in LDS");
+    int64_t rows = 0;
+    if (plan.compact)
+      if (int rc = coop_blocks(plan, rows))
+        return rc;
+    const size_t need = kNoAllocCounterBytes + static_cast<size_t>(rows) *
+                                                   static_cast<size_t>(std::max(m->dev.n_reactions, 1)) * sizeof(double);
+    if (need > kNoAllocScratchBytes)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT,
+                  "this launch shape needs " + std::to_string(need) + " bytes of scratch; the no-allocation solve has " +
+                      std::to_string(kNoAllocScratchBytes) + " (choose a shape without the compact layout)");
+    void* base = nullptr;
+    if (int rc = noalloc_scratch(&base))
+      return rc;
+    const DevParams p = make_params(*params, time_step, s->rtol);
+    s->ext_next = static_cast<int*>(base);
+    s->ext_k = reinterpret_cast<double*>(static_cast<char*>(base) + kNoAllocCounterBytes);
+    s->ext_k_rows = rows;
+    const int rc = enqueue_solve(s, p, MICM_HIP_SOLVE_COOPERATIVE, (flags & MICM_HIP_SOLVE_UPDATE_RATES) != 0,
+                                 s->stream);
+    s->ext_next = nullptr;
+    s->ext_k = nullptr;
+    s->ext_k_rows = 0;
+    return rc;
   }
 
   // ---------------------------------------------------------------- discrete adjoint
