@@ -94,8 +94,9 @@ def algorithmic_flops(per, stats):
                s[2] * per["error"])
 
 
-def cpu_baseline(desc, cells, order, dt, sample, threads, tableau=4):
-    """Cell-solves/s of the reference CPU solver (per-cell States) on `sample` cells."""
+def cpu_baseline(desc, cells, order, dt, sample, threads, tableau=4, results=None):
+    """Cell-solves/s of the reference CPU solver (per-cell States) on `sample` cells.
+    With a dict `results`, its y, states and stats for those cells are stored there."""
     from oracle import Oracle, Reference, reference_available
     kind = "reference" if reference_available() else "port"
     idx = np.arange(sample)
@@ -105,16 +106,19 @@ def cpu_baseline(desc, cells, order, dt, sample, threads, tableau=4):
         solver = Oracle(desc, tableau)
         solver.lib.oracle_set_threads(int(threads))
         t0 = time.perf_counter()
-        solver.solve(sub.conditions, sub.custom, y, dt, 1)
+        out = solver.solve(sub.conditions, sub.custom, y, dt, 1)
         el = time.perf_counter() - t0
+        if results is not None:
+            results.update(y=out[0], states=out[1], stats=out[2])
         return kind, sample / el, el
     solvers = [Reference(desc, tableau) for _ in range(threads)]
     chunks = np.array_split(idx, threads)
+    outs = [None] * threads
 
     def work(k):
         c = chunks[k]
         if len(c):
-            solvers[k].solve(sub.conditions[c], sub.custom[c], y[c], dt, 1)
+            outs[k] = solvers[k].solve(sub.conditions[c], sub.custom[c], y[c], dt, 1)
 
     t0 = time.perf_counter()
     ts = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
@@ -123,7 +127,52 @@ def cpu_baseline(desc, cells, order, dt, sample, threads, tableau=4):
     for t in ts:
         t.join()
     el = time.perf_counter() - t0
+    if results is not None:
+        done = [o for o in outs if o is not None]
+        results.update(y=np.concatenate([o[0] for o in done]), states=np.concatenate([o[1] for o in done]),
+                       stats=np.concatenate([o[2] for o in done]))
     return kind, sample / el, el
+
+
+def parity(y_gpu, st_gpu, stats_gpu, ref):
+    """The GPU's results against the reference's on the same cells
+    (1e-8 relative on species above 1e-20 mol m-3), bitwise equality and SolverStats."""
+    y_ref = ref["y"]
+    n = y_ref.shape[0]
+    g = y_gpu[:n]
+    mask = np.abs(y_ref) > 1e-20
+    rel = np.where(mask, np.abs(g - y_ref) / np.where(mask, np.abs(y_ref), 1.0), 0.0)
+    return {"cells": int(n),
+            "within_1e-8": round(float(np.mean(rel.max(axis=1) <= 1e-8)), 6),
+            "bitwise_equal": round(float(np.mean(np.all(g.view(np.int64) == y_ref.view(np.int64), axis=1))), 6),
+            "identical_stats": round(float(np.mean(np.all(stats_gpu[:n] == ref["stats"], axis=1))), 6),
+            "identical_outcome": round(float(np.mean(st_gpu[:n] == ref["states"])), 6),
+            "max_rel_error": float(rel.max())}
+
+
+def accuracy(desc, cells, order, dt, y_gpu, tableau, n_sample, threads):
+    """Error of the GPU's results against a tight-tolerance FP64 CPU solve (rtol 1e-12,
+    atol 1e-20, step limit 10^7) on `n_sample` evenly spaced cells: the largest relative
+    error on species above 1e-20 mol m-3 and the error in the solver's own norm
+    (|y - y_tight| / (atol + rtol |y_tight|) with the run's tolerances)."""
+    from oracle import Oracle
+    idx = np.linspace(0, cells.n_cells - 1, n_sample).round().astype(np.int64)
+    sub = cells.subset(idx)
+    o = Oracle(desc, tableau)
+    o.lib.oracle_set_threads(int(threads))
+    o.set_max_steps(10 ** 7)
+    t0 = time.perf_counter()
+    y_t, st_t, _, _ = o.solve(sub.conditions, sub.custom, sub.variables[:, order], dt, 1, rtol=1e-12,
+                              atol=np.full(o.n_species, 1e-20))
+    el = time.perf_counter() - t0
+    g = y_gpu[idx]
+    mask = np.abs(y_t) > 1e-20
+    rel = np.where(mask, np.abs(g - y_t) / np.where(mask, np.abs(y_t), 1.0), 0.0).max(axis=1)
+    w = (np.abs(g - y_t) / (1e-3 + 1e-6 * np.abs(y_t))).max(axis=1)
+    return {"cells": int(n_sample), "tight_solve": "rtol 1e-12, atol 1e-20", "seconds": round(el, 1),
+            "tight_converged": int(np.sum(st_t == 2)),
+            "max_rel_error_species_above_1e-20": {"median": float(np.median(rel)), "max": float(rel.max())},
+            "weighted_error_atol_1e-3_rtol_1e-6": {"median": float(np.median(w)), "max": float(w.max())}}
 
 
 def cpu_lockstep(desc, sub, order, dt, tableau):
@@ -258,7 +307,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="210_560", choices=["210_560", "60_200", "readme", "ts1", "packed"])
+    ap.add_argument("--workload", default="210_560",
+                    choices=["210_560", "60_200", "readme", "ts1", "packed", "adjoint", "adjoint_210_560"])
     ap.add_argument("--cells", type=int, default=0,
                     help="cells (0: 100000 for 210_560, 10000 for 60_200 and ts1, 1000 for readme)")
     ap.add_argument("--form", default="autotune", choices=["autotune", "persistent", "plain", "cooperative"])
@@ -271,6 +321,12 @@ def main():
                     help="cells in the CPU-baseline sample (0: skip; -1: about 15 core-seconds of CPU work, "
                          "sized from the single-core leg)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU-baseline threads (0: all host cores)")
+    ap.add_argument("--accuracy-cells", type=int, default=0,
+                    help="cells whose results are compared with a tight-tolerance CPU solve (rtol 1e-12; "
+                         "tens of CPU-seconds per 210-species cell)")
+    ap.add_argument("--no-full-state-forms", action="store_true",
+                    help="skip timing the persistent per-lane form on the whole state")
+    ap.add_argument("--check-cells", type=int, default=4, help="adjoint workloads: cells of the gradient checks")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device-to-device copy of 1 GiB (counter calibration for profiles)")
     args = ap.parse_args()
@@ -290,6 +346,9 @@ def main():
     if args.workload == "packed":
         main_packed(args)
         return
+    if args.workload.startswith("adjoint"):
+        main_adjoint(args)
+        return
 
     import torch
 
@@ -297,6 +356,8 @@ def main():
 
     n_cells = args.cells or {"210_560": 100000, "readme": 1000}.get(args.workload, 10000)
     torch.cuda.set_device(0)
+    # a stream of our own: the Solve's graph is captured on it (the legacy default stream cannot be)
+    torch.cuda.set_stream(torch.cuda.Stream())
     desc, cells, tableau, dt, label = workload(args.workload, n_cells)
     log(f"workload {args.workload}: {n_cells} cells generated")
     mech = _lib.Mechanism(desc)
@@ -340,9 +401,19 @@ def main():
     y_initial = torch.empty(n_y, dtype=torch.float64, device="cuda")
     _copy_device(y_initial.data_ptr(), ptr.variables, n_y * 8, stream)  # pristine initial state
 
+    # the per-cell forms run as one graph (the Solve's launches and the cost-order pass),
+    # captured at the first step and replayed after
+    packed = None
+    if form in (_lib.SOLVE_COOPERATIVE, _lib.SOLVE_PERSISTENT):
+        packed = _lib.PackedSolve([state], form)
+        packed.set_stream(stream.cuda_stream)
+
     def step():
         _copy_device(ptr.variables, y_initial.data_ptr(), n_y * 8, stream)
-        state.solve(dt, params, form)
+        if packed is not None:
+            packed.solve(dt, params)
+        else:
+            state.solve(dt, params, form)
 
     log(f"form: {form_label}")
     for i in range(args.warmup):
@@ -362,6 +433,7 @@ def main():
     status, stats, _ = state.results()
     ms_per_step = wall * 1e3 / args.steps
     value = n_cells * args.steps / wall
+    graph_replayed = packed.replayed if packed is not None else False
 
     t0 = time.perf_counter()
     y_out = state.get_variables()
@@ -373,6 +445,7 @@ def main():
     flops_per_step = algorithmic_flops(flops_per_invocation(desc, mech), stats)
     tflops = flops_per_step / (kernel_ms * 1e-3) / 1e12
     traffic, traffic_source = measured_traffic(args.workload, n_cells, coop_threads) if coop_threads else (None, None)
+    conv = status == 2
     result = {
         "metric": {"210_560": "cell-solves/s, 210-species x 100k cells, batched Rosenbrock (FP64)",
                    "60_200": f"cell-solves/s, 60-species x {n_cells} cells, Rosenbrock (FP64)",
@@ -391,16 +464,25 @@ def main():
         "data": "the reference's TS1 benchmark mechanism and initial state" if args.workload == "ts1"
                 else "synthetic (seeded mechanism and cells)",
         "config": {"workload": label, "form": form_label, "cells": n_cells, "tableau_stages": int(params.stages),
-                   "time_step_s": dt},
+                   "time_step_s": dt, "graph_captured": packed is not None, "graph_replayed": graph_replayed},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "traffic_source": traffic_source,
+                     "traffic_gbs": round(traffic / (kernel_ms * 1e-3) / 1e9, 2) if traffic else None,
+                     "limiter": ("latency: the factorisation's and substitutions' level schedules, a few dependent "
+                                 "LDS round trips per level, two cells per CU" if coop_threads else
+                                 "latency and HBM: one cell per lane over the grouped arrays"),
                      "kernel": "the Solve launch (one persistent or cooperative kernel per step); algorithmic bytes "
-                               f"per step {bytes_per_step} summed over the attempts the cells took"},
+                               f"per step {bytes_per_step} summed over the attempts the cells took; `achieved` "
+                               "prices them as per-kernel HBM streams, `traffic_gbs` is what the fused kernel "
+                               "moved (rocprofv3 PMC)"},
         "fp64": {"achieved": round(tflops, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                  "frac": round(tflops / FP64_PEAK_TFLOPS, 6), "flops_per_step": flops_per_step},
         "attempts_per_step": int(stats[:, 2].sum()),
-        "converged_fraction": round(float(np.mean(status == 2)), 4),
+        "attempts_by_outcome": {"converged": int(stats[conv, 2].sum()), "not_converged": int(stats[~conv, 2].sum())},
+        "converged_fraction": round(float(np.mean(conv)), 4),
+        "converged_value": {"value": round(float(conv.sum()) * args.steps / wall, 2), "unit": "cell-solves/s",
+                            "note": "converged cells only, over the same wall time"},
         "update_state_parameters_ms": round(update_ms, 3),
         "end_to_end": {"value": round(end_to_end, 2), "unit": "cell-solves/s",
                        "note": "host inputs uploaded, rate constants, one Solve, results downloaded (PCIe included)"},
@@ -448,17 +530,38 @@ def main():
         result["lockstep_form"] = {"value": round(ml / el, 2), "unit": "cell-solves/s", "cells": ml,
                                    "steps": int(stats_l[0, 2]), "converged": bool(np.all(st_l == 2)),
                                    "note": "one step size for all cells (the reference's multi-cell State)"}
+    if not args.no_full_state_forms and form == _lib.SOLVE_COOPERATIVE and args.workload == "210_560":
+        # the per-lane persistent form on every cell (the autotuner decides on samples)
+        full = {}
+        for lbl, handoff in (("persistent+handoff48", 48), ("persistent+handoff16", 16)):
+            state.set_handoff(handoff)
+            _copy_device(ptr.variables, y_initial.data_ptr(), n_y * 8, stream)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            state.solve(dt, params, _lib.SOLVE_PERSISTENT)
+            full[lbl] = round(n_cells / (time.perf_counter() - t0), 2)
+            log(f"full state, {lbl}: {full[lbl]} cell-solves/s")
+        if choice is not None:
+            autotune.apply(state, choice)
+        result["forms_full_state"] = {**full, "chosen": round(value, 2), "unit": "cell-solves/s"}
     if not args.no_kernels:
         result["kernels"] = kernel_rooflines(mech, state, stream, n_cells)
         log("single-phase kernels timed")
+    if args.accuracy_cells > 0:
+        threads = args.cpu_threads or len(os.sched_getaffinity(0))
+        result["accuracy"] = accuracy(desc, cells, order, dt, y_out, tableau, args.accuracy_cells, threads)
+        log("accuracy against the tight CPU solve done")
     if args.cpu_sample != 0:
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
         # one core first; its rate sizes the all-core sample when none is given
         sample1 = min(n_cells, 16 if args.cpu_sample < 0 else max(8, args.cpu_sample // threads))
         kind1, cps1, el1 = cpu_baseline(desc, cells, order, dt, sample1, 1, tableau)
         sample = min(n_cells, args.cpu_sample if args.cpu_sample > 0 else max(256, int(15.0 * cps1)))
-        kind, cps, el = cpu_baseline(desc, cells, order, dt, sample, threads, tableau)
+        ref = {}
+        kind, cps, el = cpu_baseline(desc, cells, order, dt, sample, threads, tableau, results=ref)
         log(f"CPU baseline: {sample} cells on {threads} cores in {el:.1f} s")
+        # the timed Solve's own results on those cells against the reference's
+        result["parity_sample"] = parity(y_out, status, stats, ref)
         result["cpu_baseline"] = {"value": round(cps, 2), "unit": "cell-solves/s", "cores": threads, "kind": kind,
                                   "cpu": cpu_model(),
                                   "sample": f"first {sample} cells of the same workload, one 1-cell State per "
@@ -577,6 +680,179 @@ def main_packed(args):
                                   "sample": "each mechanism's first cells, one 1-cell State per cell, combined as "
                                             "all cells over the summed times; " + "; ".join(parts_cpu)}
     print(json.dumps(result), flush=True)
+
+
+ADJOINT_WORKLOADS = {
+    # name: (mechanism, cells, Solves, time step, label)
+    "adjoint": ("mechanism_60_200", 4096, 10, 300.0,
+                "60/200 mechanism, 4096 cells (T 220-310 K, P 101325 Pa), 10 x 300 s, 4-stage, atol 1e-12"),
+    "adjoint_210_560": ("mechanism_210_560", 1024, 1, 900.0,
+                        "210/560 mechanism, 1024 cells of the packed workload's 210-species part (T 180-320 K, "
+                        "P 1-101 kPa, 50 % night), 1 x 900 s, 4-stage, atol 1e-12"),
+}
+ADJOINT_TARGETS = (3, 10, 20, 33, 50)
+
+
+def main_adjoint(args):
+    """Gradients of loss = sum over cells of the squared final concentrations of five
+    seeded species, with respect to every initial concentration and the log A of every
+    reaction: a recorded Solve sequence (per-accepted-step checkpoints on the device) and
+    one reverse sweep (micm_hip_adjoint).  `value` counts cell-solves with their gradients
+    per second (forward and reverse).  atol 1e-12 throughout: the default 1e-3 exceeds
+    every initial concentration of these workloads, and the map being differentiated is
+    then not smooth at the scale of the difference checks."""
+    import torch
+
+    from micm_amd import _lib, synthetic
+    from oracle import OraclePortableMath
+    from oracle import Tape as HostTape
+    torch.cuda.set_device(0)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    maker, n, n_solves, dt, label = ADJOINT_WORKLOADS[args.workload]
+    desc = getattr(synthetic, maker)()
+    if maker == "mechanism_60_200":
+        cells = synthetic.inputs_60_200(desc, n)
+    else:
+        cells = synthetic.random_inputs(desc, n, 33, (180.0, 320.0), log_pressure=(1.0e3, 1.01e5),
+                                        night_fraction=0.5)
+    tableau = 4
+    params = _lib.rosenbrock_parameters(tableau)
+    mech = _lib.Mechanism(desc)
+    order = mech.species_order()
+    N, R = mech.n_species, mech.n_reactions
+    y0 = np.ascontiguousarray(cells.variables[:, order])
+    atol = np.full(N, 1e-12)
+    state = _lib.DeviceState(mech, n)
+    state.set_stream(stream.cuda_stream)
+    state.set_variables(y0)
+    state.set_conditions(cells.conditions)
+    state.set_custom_parameters(cells.custom)
+    state.set_tolerances(1e-6, atol)
+    state.update_state_parameters()
+    state.set_cooperative_shape(256, -1)
+    max_records = 1002 * n_solves
+    tape = _lib.Tape(mech, n, max_records)
+    ptr = state.pointers()
+    n_y = ptr.n_groups * 64 * N
+    y_init = torch.empty(n_y, dtype=torch.float64, device="cuda")
+    _copy_device(y_init.data_ptr(), ptr.variables, n_y * 8, stream)
+    seed = torch.zeros(n_y, dtype=torch.float64, device="cuda")
+    gy = torch.zeros_like(seed)
+    gk = torch.zeros(ptr.n_groups * 64 * R, dtype=torch.float64, device="cuda")
+    targets = [t for t in ADJOINT_TARGETS if t < N]
+
+    def forward():
+        _copy_device(ptr.variables, y_init.data_ptr(), n_y * 8, stream)
+        tape.reset()
+        for _ in range(n_solves):
+            state.solve_recorded(dt, params, tape)
+
+    def reverse():
+        state.adjoint(params, tape, seed.data_ptr(), gy.data_ptr(), gk.data_ptr())
+
+    forward()
+    y_final = state.get_variables()
+    lam = np.zeros((n, N))
+    lam[:, targets] = 2.0 * y_final[:, targets]
+    from micm_amd._lib import to_grouped
+    seed.copy_(torch.from_numpy(to_grouped(lam)))
+    reverse()
+    log("warm-up forward and reverse done")
+    fwd_ms, rev_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        a.record(stream)
+        forward()
+        b.record(stream)
+        reverse()
+        c.record(stream)
+        torch.cuda.synchronize()
+        fwd_ms.append(a.elapsed_time(b))
+        rev_ms.append(b.elapsed_time(c))
+    wall = time.perf_counter() - t0
+    log(f"{args.steps} timed forward+reverse: {wall:.2f} s")
+    # the same Solves unrecorded: the cost of recording
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _copy_device(ptr.variables, y_init.data_ptr(), n_y * 8, stream)
+    for _ in range(n_solves):
+        state.solve(dt, params, _lib.SOLVE_COOPERATIVE)
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t1
+    counts = tape.counts()
+    status = state.results()[0]
+    g_y = from_grouped_host(gy, n, N)
+    g_k = from_grouped_host(gk, n, R)
+    # gradient checks on sampled cells: directional central differences (h = 1e-6) of the
+    # recorded map and the CPU tangent-linear model, three directions each in y0 and log A
+    sample = np.linspace(0, n - 1, args.check_cells).astype(int)
+    o = OraclePortableMath(desc, tableau)
+    o.lib.oracle_set_threads(int(len(os.sched_getaffinity(0))))
+    k = state.get_rate_constants()[sample]
+    ht = HostTape(np.ascontiguousarray(tape.records(sample)), np.ascontiguousarray(counts[sample]))
+    ys = y0[sample]
+    rng = np.random.default_rng(17)
+    h = 1e-6
+
+    def loss(y):
+        return float(np.sum(y[:, targets] ** 2))
+
+    errs = {"fd_y0": [], "fd_logA": [], "tangent_y0": [], "tangent_logA": []}
+    t2 = time.perf_counter()
+    for _ in range(3):
+        # unit directions (relative in y0), so that h is the size of the perturbation
+        v = rng.standard_normal(ys.shape)
+        v = v / np.linalg.norm(v) * ys
+        w = rng.standard_normal(R)
+        w = w / np.linalg.norm(w)
+        ad_y = float(np.sum(g_y[sample] * v))
+        ad_k = float(np.sum(g_k[sample] @ w))
+        fd_y = (loss(o.replay(k, ys + h * v, ht)) - loss(o.replay(k, ys - h * v, ht))) / (2 * h)
+        fd_k = (loss(o.replay(k * np.exp(h * w), ys, ht)) - loss(o.replay(k * np.exp(-h * w), ys, ht))) / (2 * h)
+        dyf = o.tangent(k, ys, ht, np.stack([v, np.zeros_like(v)], 1),
+                        np.stack([np.zeros((len(sample), R)), np.broadcast_to(w, (len(sample), R))], 1))
+        lam_s = lam[sample]
+        tl_y = float(np.sum(dyf[:, 0] * lam_s))
+        tl_k = float(np.sum(dyf[:, 1] * lam_s))
+        errs["fd_y0"].append(abs(ad_y - fd_y) / abs(fd_y))
+        errs["fd_logA"].append(abs(ad_k - fd_k) / abs(fd_k))
+        errs["tangent_y0"].append(abs(ad_y - tl_y) / abs(tl_y))
+        errs["tangent_logA"].append(abs(ad_k - tl_k) / abs(tl_k))
+    log(f"gradient checks on {len(sample)} cells: {time.perf_counter() - t2:.1f} s")
+    fwd, rev = float(np.mean(fwd_ms)), float(np.mean(rev_ms))
+    rec_bytes = (N + 1) * 8
+    result = {
+        "metric": "cell-solves/s with gradients (recorded Solves + discrete adjoint, FP64)",
+        "value": round(n * n_solves * args.steps / wall, 2), "unit": "cell-solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded mechanism and cells)",
+        "config": {"workload": label, "cells": n, "solves": n_solves, "loss": "sum of squared final "
+                   f"concentrations of species {targets} (state order)", "gradients": f"{N} initial concentrations "
+                   f"and {R} log A per cell (summed over cells for a shared log A)"},
+        "forward_recorded_ms": round(fwd, 2), "adjoint_ms": round(rev, 2),
+        "forward_unrecorded_ms": round(plain_s * 1e3, 2),
+        "memory": {"tape_allocated_bytes": int(n * max_records * rec_bytes + 8 * n),
+                   "tape_used_bytes": int(counts.sum() * rec_bytes),
+                   "records_per_cell": {"mean": round(float(counts.mean()), 1), "max": int(counts.max())},
+                   "note": "one checkpoint (H, y) per accepted step and one per clamp; stage vectors recomputed"},
+        "converged_fraction": round(float(np.mean(status == 2)), 4),
+        "gradient_check": {"cells": int(len(sample)), "directions": 3, "h": h,
+                           **{key: float(max(v)) for key, v in errs.items()},
+                           "note": "max relative error of directional derivatives of the loss over the sampled "
+                                   "cells: central differences of the recorded map, and the CPU tangent-linear "
+                                   "model"},
+    }
+    print(json.dumps(result), flush=True)
+
+
+def from_grouped_host(t, n_cells, n_values):
+    import torch
+    from micm_amd._lib import from_grouped
+    torch.cuda.synchronize()
+    return from_grouped(t.cpu().numpy(), n_cells, n_values)
 
 
 def _copy_device(dst, src, nbytes, stream):

@@ -390,6 +390,9 @@ typedef struct micm_hip_packed micm_hip_packed;
 int micm_hip_packed_create(int32_t n_states, micm_hip_state* const* states, int32_t form, micm_hip_packed** out);
 /* Blocks until every state has finished; per-cell results via micm_hip_state_get_results. */
 int micm_hip_packed_solve(micm_hip_packed* packed, double time_step, const micm_hip_rosenbrock_parameters* params);
+/* Capture and launch the packed solves on `stream` (a hipStream_t other than the legacy default
+ * stream, which cannot be captured) instead of the object's own stream. */
+int micm_hip_packed_set_stream(micm_hip_packed* packed, void* stream);
 /* 1 if the last micm_hip_packed_solve replayed a graph captured earlier, else 0. */
 int micm_hip_packed_replayed(const micm_hip_packed* packed);
 void micm_hip_packed_destroy(micm_hip_packed* packed);

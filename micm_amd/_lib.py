@@ -91,6 +91,7 @@ _SIGNATURES = {
     "micm_hip_packed_create": [_i32, ctypes.POINTER(_vp), _i32, ctypes.POINTER(_vp)],
     "micm_hip_packed_solve": [_vp, ctypes.c_double, ctypes.POINTER(RosenbrockParameters)],
     "micm_hip_packed_replayed": [_vp],
+    "micm_hip_packed_set_stream": [_vp, _vp],
     "micm_hip_packed_destroy": [_vp],
     "micm_hip_mechanism_cooperative_supported": [_vp, ctypes.c_int32, _i32p],
     "micm_hip_update_state_parameters": [_vp, _vp],
@@ -456,6 +457,10 @@ class PackedSolve:
 
     def solve(self, time_step, parameters):
         check(lib().micm_hip_packed_solve(self.h, float(time_step), ctypes.byref(parameters)))
+
+    def set_stream(self, stream):
+        """Capture and launch on `stream` (hipStream_t, not the legacy default stream)."""
+        check(lib().micm_hip_packed_set_stream(self.h, stream))
 
     @property
     def replayed(self):

@@ -76,17 +76,19 @@ def _time_candidates(mech, y, k, dt, params, candidates, repeats, verbose, give_
     return times
 
 
-def autotune(mech, state, dt, params, sample=16384, first_sample=2048, pilot=256, finalists=3, repeats=1,
+def autotune(mech, state, dt, params, sample=65536, first_sample=2048, pilot=256, finalists=3, repeats=1,
              candidates=CANDIDATES, verbose=False):
     """Pick the fastest per-cell form for `state` (whose inputs and rate constants
     are set), on strided samples of its cells: every candidate on `pilot` cells,
     those within twice the best (the plain form: four times) on `first_sample` cells,
-    the `finalists` fastest of
-    those on `sample` cells.  The last samples must hold many cells per resident
-    workgroup: on a few cells per workgroup the time is that of the slowest cells,
-    which favours the shapes that run one cell fastest over those that run the most
-    cells at once.  Returns a dict with keys label, form, handoff, threads, stage,
-    sample, seconds (label -> seconds on the largest sample the candidate reached)."""
+    the `finalists` fastest of those, with the persistent forms that hand slow cells
+    on, on `sample` cells.  The last sample fills the chip for the per-lane forms (64
+    lanes x 4 SIMDs x 256 CUs = 65,536 cells): on fewer cells a per-lane form waits for
+    its slowest cell with most lanes idle, and the cooperative shapes' time is that of
+    their slowest cells, which favours the shapes that run one cell fastest over those
+    that run the most cells at once.  Returns a dict with keys label, form, handoff,
+    threads, stage, sample, seconds (label -> seconds on the largest sample the
+    candidate reached)."""
     n = state.n_cells
     m = min(sample, n)
     key = _key(mech, params.stages, dt, m)
@@ -115,15 +117,24 @@ def autotune(mech, state, dt, params, sample=16384, first_sample=2048, pilot=256
             slow_handoff = True
         best = t if best is None else min(best, t)
     # the plain form's time grows slowly with the cell count (its launches serve every
-    # cell at once), so a pilot understates it: it stays in at four times the best
+    # cell at once), so a pilot understates it: it stays in at four times the best.  The
+    # persistent forms with a hand-off are timed on the last sample whatever the pilot
+    # says: a few hundred cells leave almost every lane idle behind the slowest one.
+    def per_lane(c):
+        return c[1] == _lib.SOLVE_PERSISTENT and c[2] > 0
+    lane_forms = [c for c in cands if per_lane(c)]
     cands = [c for c in cands
-             if c[0] in times and times[c[0]] <= (4.0 if c[1] == _lib.SOLVE_PLAIN else 2.0) * best]
+             if c[0] in times and not per_lane(c) and times[c[0]] <= (4.0 if c[1] == _lib.SOLVE_PLAIN else 2.0) * best]
     m1 = min(first_sample, m)
     if m1 > m0 and len(cands) > 1:
         times.update(_time_candidates(mech, *pick(m1), dt, params, cands, repeats, verbose))
         cands = sorted(cands, key=lambda c: times[c[0]])[:finalists]
-    if m > m1 and len(cands) > 1:
-        times.update(_time_candidates(mech, *pick(m), dt, params, cands, repeats, verbose))
+    last = cands + lane_forms
+    if m > m1:
+        times.update(_time_candidates(mech, *pick(m), dt, params, last, repeats, verbose))
+    elif m > m0 and lane_forms:  # the others were timed on these m cells already
+        times.update(_time_candidates(mech, *pick(m), dt, params, lane_forms, repeats, verbose))
+    cands = [c for c in last if c[0] in times]
     label, form, handoff, threads, stage = min(cands, key=lambda c: times[c[0]])
     choice = {"label": label, "form": form, "handoff": handoff, "threads": threads, "stage": stage,
               "sample": m, "seconds": times}

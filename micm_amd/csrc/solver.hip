@@ -1738,7 +1738,8 @@ struct micm_hip_packed
 {
   std::vector<micm_hip_state*> states;
   int32_t form = MICM_HIP_SOLVE_COOPERATIVE;
-  hipStream_t origin = nullptr;
+  hipStream_t origin = nullptr;      // capture and launch stream
+  hipStream_t own_stream = nullptr;  // the stream created with the object
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   // what the captured graph was built for
@@ -3378,7 +3379,8 @@ extern "C"
     auto p = std::make_unique<micm_hip_packed>();
     p->states.assign(states, states + n_states);
     p->form = form;
-    HIP_TRY(hipStreamCreate(&p->origin));
+    HIP_TRY(hipStreamCreate(&p->own_stream));
+    p->origin = p->own_stream;
     *out = p.release();
     return MICM_HIP_OK;
   }
@@ -3449,6 +3451,15 @@ extern "C"
     return MICM_HIP_OK;
   }
 
+  int micm_hip_packed_set_stream(micm_hip_packed* pk, void* stream)
+  {
+    if (!pk || !stream)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null packed object or stream (a graph cannot be captured on the "
+                                                 "legacy default stream)");
+    pk->origin = static_cast<hipStream_t>(stream);
+    return MICM_HIP_OK;
+  }
+
   int micm_hip_packed_replayed(const micm_hip_packed* pk)
   {
     return pk && pk->replayed ? 1 : 0;
@@ -3462,8 +3473,11 @@ extern "C"
       (void)hipGraphExecDestroy(pk->exec);
     if (pk->graph)
       (void)hipGraphDestroy(pk->graph);
-    if (pk->origin)
-      (void)hipStreamDestroy(pk->origin);
+    if (pk->own_stream)
+    {
+      (void)hipStreamSynchronize(pk->own_stream);
+      (void)hipStreamDestroy(pk->own_stream);
+    }
     delete pk;
   }
 

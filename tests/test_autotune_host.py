@@ -1,8 +1,8 @@
 """The autotuner's selection logic on the host, with the GPU timings replaced by a
 model: the pilot prunes candidates at twice the best (the plain form at four times),
 the per-lane persistent form without hand-off is skipped once a hand-off variant
-was four times slower than the best, and the fastest of the finalists on the
-largest sample wins."""
+was four times slower than the best, the hand-off variants always reach the last
+(chip-filling) sample, and the fastest there wins."""
 import types
 
 import numpy as np
@@ -73,7 +73,19 @@ def test_plain_form_pruned_beyond_four_times(monkeypatch):
     choice, calls = _run(monkeypatch, model, 100000, tag=102)
     assert choice["label"] == "cooperative256"
     assert [n for label, n in calls if label == "plain"] == [256]
-    assert max(n for _, n in calls) == 16384
+    assert max(n for _, n in calls) == 65536
+
+
+def test_persistent_hand_off_forms_are_timed_at_chip_filling_size(monkeypatch):
+    """A per-lane form that loses the 256-cell pilot (its lanes idle behind the slowest
+    cell) but wins once 65,536 cells fill the chip is the choice."""
+    model = {label: _linear(0.5, 1e-4) for label, *_ in autotune.CANDIDATES}
+    model["cooperative256"] = _linear(0.3, 8.5e-5)
+    # slow on few cells, faster per cell at scale
+    model["persistent+handoff16"] = lambda n: 1.4 if n <= 2048 else 0.2 + 8.2e-5 * n
+    choice, calls = _run(monkeypatch, model, 100000, tag=104)
+    assert ("persistent+handoff16", 65536) in calls and ("persistent+handoff48", 65536) in calls
+    assert choice["label"] == "persistent+handoff16"
 
 
 def test_without_cooperative_support_only_per_lane_forms_compete(monkeypatch):
