@@ -862,7 +862,8 @@ namespace micm_hip
         p.stages = 2;
         p.a[0] = 1.0 / g;
         p.c[0] = -2.0 / g;
-        p.new_function_evaluation[0] = p.new_function_evaluation[1] = 1;
+        for (int i = 0; i < 6; ++i)
+          p.new_function_evaluation[i] = 1;
         p.m[0] = 3.0 / (2.0 * g);
         p.m[1] = 1.0 / (2.0 * g);
         p.e[0] = 1.0 / (2.0 * g);

@@ -220,6 +220,19 @@ class Tape:
 
 
 class _AdjointMixin:
+    def set_control(self, max_steps=1000, h_min=0.0, h_max=0.0, h_start=0.0):
+        """max_number_of_steps, h_min, h_max and h_start of this solver's parameters."""
+        f = self.lib.oracle_set_control
+        f.argtypes = [_vp, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        if f(self.h, int(max_steps), float(h_min), float(h_max), float(h_start)) != 0:
+            raise ValueError(max_steps)
+
+    def set_max_steps(self, n):
+        """max_number_of_steps of this solver's Solves (for tight-tolerance references)."""
+        self.lib.oracle_set_max_steps.argtypes = [_vp, ctypes.c_int64]
+        if self.lib.oracle_set_max_steps(self.h, int(n)) != 0:
+            raise ValueError(n)
+
     """The discrete adjoint of the recorded map (each accepted step with its H frozen),
     its tangent-linear model and the map itself, for finite differences."""
 
@@ -278,6 +291,26 @@ class Reference(_Base):
     """The reference's compiled CPU solver (needs oracle/_ref)."""
     prefix = "ref_"
     path = REF_LIB
+
+    @classmethod
+    def tableau(cls, preset):
+        """Every field of the reference's RosenbrockSolverParameters preset, as arrays."""
+        from micm_amd._lib import RosenbrockParameters
+        lib = cls._lib()
+        lib.ref_tableau.argtypes = [ctypes.c_int32, ctypes.POINTER(RosenbrockParameters)]
+        p = RosenbrockParameters()
+        if lib.ref_tableau(int(preset), ctypes.byref(p)) != 0:
+            raise ValueError(preset)
+        return tableau_fields(p)
+
+
+def tableau_fields(p):
+    """The fields of a micm_hip_rosenbrock_parameters struct as numpy arrays."""
+    out = {}
+    for name, _ in p._fields_:
+        v = getattr(p, name)
+        out[name] = np.array(list(v)) if hasattr(v, "_length_") else np.array([v])
+    return out
 
 
 class Oracle(_Base, _AdjointMixin):

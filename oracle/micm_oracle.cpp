@@ -1564,6 +1564,30 @@ extern "C"
   }
 }
 
+// Step limit of the solver's Solves (max_number_of_steps, rosenbrock_solver_parameters.hpp:31),
+// raised for tight-tolerance accuracy references.
+extern "C" int oracle_set_max_steps(void* h, int64_t n)
+{
+  if (!h || n < 1)
+    return -1;
+  static_cast<OracleSolver*>(h)->M.tab.max_steps = n;
+  return 0;
+}
+
+// Step-control fields of the solver's parameters (rosenbrock_solver_parameters.hpp:31-41):
+// max_number_of_steps, h_min, h_max, h_start; tests force the solver's failure states with them.
+extern "C" int oracle_set_control(void* h, int64_t max_steps, double h_min, double h_max, double h_start)
+{
+  if (!h || max_steps < 0)
+    return -1;
+  Tableau& t = static_cast<OracleSolver*>(h)->M.tab;
+  t.max_steps = max_steps;
+  t.h_min = h_min;
+  t.h_max = h_max;
+  t.h_start = h_start;
+  return 0;
+}
+
 // Plants a mistake in the adjoint (tests of the checks only; 0 restores it).
 extern "C" int oracle_set_adjoint_fault(int32_t which)
 {

@@ -315,6 +315,46 @@ extern "C"
     }
   }
 
+  // The reference's RosenbrockSolverParameters preset, every field, in the C-ABI's struct.
+  int ref_tableau(int32_t preset_id, micm_hip_rosenbrock_parameters* out)
+  {
+    try
+    {
+      const Params p = preset(preset_id);
+      *out = micm_hip_rosenbrock_parameters{};
+      out->stages = static_cast<int64_t>(p.stages_);
+      out->max_number_of_steps = static_cast<int64_t>(p.max_number_of_steps_);
+      out->round_off = p.round_off_;
+      out->factor_min = p.factor_min_;
+      out->factor_max = p.factor_max_;
+      out->rejection_factor_decrease = p.rejection_factor_decrease_;
+      out->safety_factor = p.safety_factor_;
+      out->h_min = p.h_min_;
+      out->h_max = p.h_max_;
+      out->h_start = p.h_start_;
+      for (int i = 0; i < 6; ++i)
+      {
+        out->new_function_evaluation[i] = p.new_function_evaluation_[i] ? 1 : 0;
+        out->m[i] = p.m_[i];
+        out->e[i] = p.e_[i];
+        out->alpha[i] = p.alpha_[i];
+        out->gamma[i] = p.gamma_[i];
+      }
+      out->estimator_of_local_order = p.estimator_of_local_order_;
+      for (int i = 0; i < 15; ++i)
+      {
+        out->a[i] = p.a_[i];
+        out->c[i] = p.c_[i];
+      }
+      return 0;
+    }
+    catch (const std::exception& e)
+    {
+      g_error = e.what();
+      return -1;
+    }
+  }
+
   void ref_solver_destroy(void* h)
   {
     delete static_cast<RefSolver*>(h);

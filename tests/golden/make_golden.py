@@ -79,9 +79,21 @@ def make(name):
     print(name, {kk: v.shape for kk, v in out.items() if v.size > 1})
 
 
+def make_tableaus():
+    """tests/golden/tableaus.npz: every field of the reference's five RosenbrockSolverParameters
+    presets (ref_tableau in oracle/ref_driver.cpp)."""
+    out = {}
+    for preset in (2, 3, 4, 41, 61):
+        for key, value in Reference.tableau(preset).items():
+            out[f"{preset}_{key}"] = value
+    np.savez_compressed(os.path.join(HERE, "tableaus.npz"), **out)
+    print("tableaus", sorted({k.split("_", 1)[1] for k in out}))
+
+
 if __name__ == "__main__":
     if not reference_available():
         sys.exit("oracle/_ref/libmicm_ref.so is not built (make -C oracle ref)")
+    make_tableaus()
     names = sys.argv[1:] or list(CASES)
     if "ts1" in names:
         export_ts1()

@@ -407,3 +407,124 @@ def torch_zeros(n):
 def torch_tensor(a):
     import torch
     return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to("cuda")
+
+
+ALL_FORMS = [_lib.SOLVE_PLAIN, _lib.SOLVE_PERSISTENT, _lib.SOLVE_COOPERATIVE, _lib.SOLVE_LOCKSTEP]
+
+
+@pytest.mark.parametrize("tableau", [2, 3, 4, 41, 61])
+@pytest.mark.parametrize("form", ALL_FORMS)
+def test_every_tableau_matches_reference_math(tableau, form):
+    """The five presets (2-, 3-, 4-stage, 4-stage DA, 6-stage DA; fields equal the
+    reference's, tests/test_host.py) drive every form to the C-library oracle's result
+    bit for bit: 60/200, 130 cells, 2 x 300 s (lockstep: one State of all cells)."""
+    desc = synthetic.mechanism_60_200()
+    mech = _lib.Mechanism(desc)
+    if form == _lib.SOLVE_COOPERATIVE and not mech.cooperative_supported(6 if tableau == 61 else 4):
+        pytest.skip("mechanism does not fit the cooperative form")
+    cells = synthetic.inputs_60_200(desc, 130, seed=41)
+    order = mech.species_order()
+    print(f"tableau {tableau} form {form}", flush=True)
+    y, st, stats, ft = _solve(mech, cells, order, tableau, 300.0, form, n_solves=2)
+    y_ref, st_ref, stats_ref, ft_ref = Oracle(desc, tableau).solve(
+        cells.conditions, cells.custom, cells.variables[:, order], 300.0, 2, lockstep=form == _lib.SOLVE_LOCKSTEP)
+    assert np.array_equal(st, st_ref) and np.array_equal(stats, stats_ref)
+    assert bits_equal(ft, ft_ref) and bits_equal(y, y_ref)
+
+
+@pytest.mark.parametrize("name", ["60_200", "210_560", "ts1"])
+@pytest.mark.parametrize("form", [_lib.SOLVE_PERSISTENT, _lib.SOLVE_COOPERATIVE])
+def test_fused_rate_constants_match_reference_math(name, form):
+    """UpdateStateParameters fused into the Solve launch (MICM_HIP_SOLVE_UPDATE_RATES):
+    rate constants, outcomes, counters and concentrations equal the C-library oracle's."""
+    desc, inputs, tableau, dt = CASES[name]
+    mech = _lib.Mechanism(desc)
+    if form == _lib.SOLVE_COOPERATIVE and not mech.cooperative_supported(tableau):
+        pytest.skip("mechanism does not fit the cooperative form")
+    cells = inputs(desc, 150)
+    order = mech.species_order()
+    print(f"fused rates vs oracle: {name} form {form}", flush=True)
+    state = _lib.DeviceState(mech, cells.n_cells)
+    state.set_variables(cells.variables[:, order])
+    state.set_conditions(cells.conditions)
+    state.set_custom_parameters(cells.custom)
+    params = _lib.rosenbrock_parameters(tableau)
+    state.solve(dt, params, form | _lib.SOLVE_UPDATE_RATES)
+    o = Oracle(desc, tableau)
+    y_ref, st_ref, stats_ref, ft_ref = o.solve(cells.conditions, cells.custom, cells.variables[:, order], dt, 1)
+    st, stats, ft = state.results()
+    assert bits_equal(state.get_rate_constants(), o.rate_constants(cells.conditions, cells.custom))
+    assert np.array_equal(st, st_ref) and np.array_equal(stats, stats_ref)
+    assert bits_equal(ft, ft_ref) and bits_equal(state.get_variables(), y_ref)
+
+
+def test_ten_step_sequence_of_the_60_species_workload():
+    """The 60/200 workload's sequence -- 10 consecutive Solves of 300 s, rate constants
+    updated before each -- on 515 cells, cooperative and persistent forms: every cell bit
+    for bit against the C-library oracle after all ten."""
+    desc = synthetic.mechanism_60_200()
+    mech = _lib.Mechanism(desc)
+    cells = synthetic.inputs_60_200(desc, 515)
+    order = mech.species_order()
+    y_ref, st_ref, stats_ref, ft_ref = Oracle(desc, 4).solve(cells.conditions, cells.custom,
+                                                             cells.variables[:, order], 300.0, 10)
+    for form in (_lib.SOLVE_COOPERATIVE, _lib.SOLVE_PERSISTENT):
+        print(f"10 x 300 s: form {form}", flush=True)
+        y, st, stats, ft = _solve(mech, cells, order, 4, 300.0, form, n_solves=10)
+        assert np.array_equal(st, st_ref) and np.array_equal(stats, stats_ref)
+        assert bits_equal(ft, ft_ref) and bits_equal(y, y_ref)
+
+
+def _forced(case):
+    """(mechanism, cells, y0 mutator, rtol, atol, parameter overrides, expected state) of
+    the inputs that force each of the reference's failure states (solver_result.hpp:13-35)."""
+    readme = synthetic.readme_mechanism()
+    if case == "nan":
+        def mutate(y):
+            y[1, 0] = np.nan
+            return y
+        return readme, synthetic.readme_inputs(70, seed=3), mutate, None, None, {}, 6
+    if case == "inf":  # the scaled errors overflow
+        return readme, synthetic.readme_inputs(70, seed=3), None, 1e-300, 1e-300, {}, 7
+    if case == "max_steps":
+        return synthetic.mechanism_60_200(), synthetic.inputs_60_200(synthetic.mechanism_60_200(), 70), None, \
+            None, None, {"max_number_of_steps": 1}, 3
+    # step size below the round-off
+    return readme, synthetic.readme_inputs(70, seed=3), None, None, None, {"h_max": 1e-300}, 4
+
+
+@pytest.mark.parametrize("case", ["nan", "inf", "max_steps", "step_size"])
+@pytest.mark.parametrize("form", ALL_FORMS)
+def test_forced_failure_states_match_reference_math(case, form):
+    """NaNDetected (a NaN initial concentration), InfDetected (tolerances of 1e-300, whose
+    scaled errors overflow), ConvergenceExceededMaxSteps (max_number_of_steps = 1) and
+    StepSizeTooSmall (h_max below the round-off): every form reports the oracle's state,
+    counters and concentrations for every cell."""
+    desc, cells, mutate, rtol, atol, overrides, expected = _forced(case)
+    mech = _lib.Mechanism(desc)
+    order = mech.species_order()
+    y0 = np.array(cells.variables[:, order])
+    if mutate:
+        y0 = mutate(y0)
+    print(f"forced {case}: form {form}", flush=True)
+    params = _lib.rosenbrock_parameters(4 if desc.n_species > 3 else 3)
+    for key, value in overrides.items():
+        setattr(params, key, value)
+    state = _lib.DeviceState(mech, cells.n_cells)
+    state.set_variables(y0)
+    state.set_conditions(cells.conditions)
+    state.set_custom_parameters(cells.custom)
+    if rtol is not None:
+        state.set_tolerances(rtol, np.full(mech.n_species, atol))
+    state.update_state_parameters()
+    state.solve(300.0 if desc.n_species > 3 else 500.0, params, form)
+    st, stats, ft = state.results()
+    o = Oracle(desc, int(params.stages))
+    o.set_control(int(params.max_number_of_steps), params.h_min, params.h_max, params.h_start)
+    y_ref, st_ref, stats_ref, ft_ref = o.solve(
+        cells.conditions, cells.custom, y0, 300.0 if desc.n_species > 3 else 500.0, 1,
+        lockstep=form == _lib.SOLVE_LOCKSTEP, rtol=rtol if rtol is not None else 1e-6,
+        atol=None if atol is None else np.full(mech.n_species, atol))
+    assert np.any(st_ref == expected)
+    assert np.array_equal(st, st_ref) and np.array_equal(stats, stats_ref)
+    assert bits_equal(ft, ft_ref) and bits_equal(state.get_variables(), y_ref)

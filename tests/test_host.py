@@ -190,3 +190,26 @@ def test_fp64_operation_count():
     assert (per["lu_multiply_subtracts"], per["lu_divisions"]) == (6986, 1244)
     assert per["lu"] == 2 * 6986 + 1244 + 209
     assert per["solve"] == 2 * ((1453 - 209) + (1137 - 209)) + 2 * 209
+
+
+def test_tableau_presets_equal_the_reference():
+    """Every field of micm_hip_rosenbrock_parameters_preset for the five presets equals the
+    reference's RosenbrockSolverParameters (tests/golden/tableaus.npz, written by the
+    reference build; and the live reference where it is built), bit for bit."""
+    import os
+
+    import numpy as np
+
+    from micm_amd import _lib
+    from oracle import Reference, reference_available, tableau_fields
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "tableaus.npz"))
+    for preset in (2, 3, 4, 41, 61):
+        ours = tableau_fields(_lib.rosenbrock_parameters(preset))
+        refs = [{k.split("_", 1)[1]: g[k] for k in g.files if k.split("_", 1)[0] == str(preset)}]
+        if reference_available():
+            refs.append(Reference.tableau(preset))
+        for ref in refs:
+            assert set(ref) == set(ours)
+            for key, value in ours.items():
+                assert np.array_equal(np.asarray(value).view(np.uint8), np.asarray(ref[key]).view(np.uint8)), \
+                    (preset, key)
