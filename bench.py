@@ -327,6 +327,7 @@ def main():
     ap.add_argument("--no-full-state-forms", action="store_true",
                     help="skip timing the persistent per-lane form on the whole state")
     ap.add_argument("--check-cells", type=int, default=4, help="adjoint workloads: cells of the gradient checks")
+    ap.add_argument("--no-graph", action="store_true", help="launch the Solve directly instead of as a graph")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device-to-device copy of 1 GiB (counter calibration for profiles)")
     args = ap.parse_args()
@@ -404,7 +405,7 @@ def main():
     # the per-cell forms run as one graph (the Solve's launches and the cost-order pass),
     # captured at the first step and replayed after
     packed = None
-    if form in (_lib.SOLVE_COOPERATIVE, _lib.SOLVE_PERSISTENT):
+    if form in (_lib.SOLVE_COOPERATIVE, _lib.SOLVE_PERSISTENT) and not args.no_graph:
         packed = _lib.PackedSolve([state], form)
         packed.set_stream(stream.cuda_stream)
 

@@ -1362,7 +1362,9 @@ namespace
     return NT == 256 ? 3 : 1;
   }
 
-  template<int NT, int STAGE, bool COMPACT = false, int MINW = coop_min_waves_per_eu<NT>()>
+  // RECORD: a recorded Solve, appending each cell's checkpoints to `tape` (a separate
+  // build, so that the ordinary Solve's register allocation does not carry it)
+  template<int NT, int STAGE, bool COMPACT = false, int MINW = coop_min_waves_per_eu<NT>(), bool RECORD = false>
   __global__ void __launch_bounds__(NT, MINW) k_coop(DevMech m, DevCoop c, DevParams p, int p_capacity, int64_t n_fixed,
                                                const int32_t* __restrict__ list, const int* list_count, int* next,
                                                const CellControl* resume, double* __restrict__ y_state,
@@ -1455,10 +1457,9 @@ namespace
       else
         control_init(p, cc);
       // recorded Solve: the cell's checkpoints are appended to its tape
-      int64_t n_rec = tape.records ? tape.counts[cell] : 0;
+      int64_t n_rec = RECORD ? tape.counts[cell] : 0;
       double* const rec_base =
-          tape.records ? tape.records + static_cast<size_t>(cell) * tape.max_records * (static_cast<size_t>(n) + 1)
-                       : nullptr;
+          RECORD ? tape.records + static_cast<size_t>(cell) * tape.max_records * (static_cast<size_t>(n) + 1) : nullptr;
       auto record = [&](double tag) {
         if (n_rec < tape.max_records)
         {
@@ -1602,7 +1603,7 @@ namespace
         const double h_used = cc.H;
         if (control_update(p, cc, error))
         {
-          if (rec_base)
+          if constexpr (RECORD)
             record(h_used);
           for (int i = lane; i < n; i += NT)
             L.y[i] = COMPACT ? ynew_r : L.ynew[i];
@@ -1611,7 +1612,7 @@ namespace
         if (cc.status != MICM_HIP_STATE_RUNNING)
           break;
       }
-      if (rec_base)
+      if constexpr (RECORD)
       {
         record(kClampTag);
         if (lane == 0)
@@ -2183,9 +2184,12 @@ namespace
   // more cells per CU; 4: the compact layout; threads_request 0: one wavefront.
   CoopPlan plan_coop(const micm_hip_mechanism* m, int stages, int stage_request = -1, int threads_request = 0)
   {
-    CoopPlan plan = plan_coop_full(m, stages, stage_request == 4 ? -1 : stage_request, threads_request);
+    // 4: the compact layout, its register budget chosen by occupancy; 5 / 6: the compact
+    // layout built for three waves per SIMD (registers spill) / without that bound (256 threads)
+    const bool compact_request = stage_request >= 4;
+    CoopPlan plan = plan_coop_full(m, stages, compact_request ? -1 : stage_request, threads_request);
     const DevCoop& c = m->coop;
-    if (!m->t.coop_levels_ok || (stage_request >= 0 && stage_request != 4))
+    if (!m->t.coop_levels_ok || (stage_request >= 0 && !compact_request))
       return plan;
     // the compact layout keeps one species per thread in registers and publishes the
     // rate constants through global memory, so it needs a workgroup barrier
@@ -2204,7 +2208,7 @@ namespace
     const int occ_unbounded = threads == 256 ? coop_occupancy(threads, 0, true, compact, true) : 0;
     const int occ_compact = std::max(occ_bounded, occ_unbounded);
     const int occ_full = plan.lds > 0 ? coop_occupancy(threads, plan.stage, false, plan.lds) : 0;
-    if (stage_request == 4 ? occ_compact > 0 : occ_compact > occ_full)
+    if (compact_request ? occ_compact > 0 : occ_compact > occ_full)
     {
       plan.lds = compact;
       plan.stage = 0;
@@ -2212,6 +2216,10 @@ namespace
       plan.p_capacity = c.bwd.max_level_terms;
       plan.compact = true;
       plan.unbounded = occ_unbounded >= occ_bounded;  // no register spills when it costs no cell per CU
+      if (threads == 256 && stage_request == 5)
+        plan.unbounded = false;
+      else if (threads == 256 && stage_request == 6 && occ_unbounded > 0)
+        plan.unbounded = true;
     }
     return plan;
   }
@@ -2277,12 +2285,12 @@ namespace
     return MICM_HIP_OK;
   }
 
-  template<int NT, int STAGE, bool COMPACT = false, int MINW = coop_min_waves_per_eu<NT>()>
-  int launch_coop_shape(micm_hip_state* s, const DevParams& p, const CoopPlan& plan, const int32_t* list,
+  template<int NT, int STAGE, bool COMPACT, int MINW, bool RECORD>
+  int launch_coop_build(micm_hip_state* s, const DevParams& p, const CoopPlan& plan, const int32_t* list,
                         const int* count, int64_t n_fixed, const CellControl* resume, bool update_rates, hipStream_t st)
   {
     const micm_hip_mechanism* m = s->mech;
-    const void* fn = reinterpret_cast<const void*>(k_coop<NT, STAGE, COMPACT, MINW>);
+    const void* fn = reinterpret_cast<const void*>(k_coop<NT, STAGE, COMPACT, MINW, RECORD>);
     HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(plan.lds)));
     int64_t blocks = 0;
     if (int rc = coop_blocks(plan, blocks))
@@ -2297,13 +2305,23 @@ namespace
     if (!list)
       blocks = std::max<int64_t>(1, std::min(blocks, n_fixed));
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, next);
-    hipLaunchKernelGGL((k_coop<NT, STAGE, COMPACT, MINW>), dim3(static_cast<unsigned>(blocks)), dim3(NT), plan.lds, st,
-                       m->dev,
-                       m->coop, p, plan.p_capacity, n_fixed, list, count, next, resume, s->y, s->k, s->atol,
+    hipLaunchKernelGGL((k_coop<NT, STAGE, COMPACT, MINW, RECORD>), dim3(static_cast<unsigned>(blocks)), dim3(NT),
+                       plan.lds, st, m->dev, m->coop, p, plan.p_capacity, n_fixed, list, count, next, resume, s->y, s->k, s->atol,
                        s->status, s->stats, s->final_time, update_rates ? s->cond : nullptr,
                        update_rates ? s->custom : nullptr, k_rows, s->recording);
     HIP_TRY(hipGetLastError());
     return MICM_HIP_OK;
+  }
+
+  template<int NT, int STAGE, bool COMPACT = false, int MINW = coop_min_waves_per_eu<NT>()>
+  int launch_coop_shape(micm_hip_state* s, const DevParams& p, const CoopPlan& plan, const int32_t* list,
+                        const int* count, int64_t n_fixed, const CellControl* resume, bool update_rates, hipStream_t st)
+  {
+    if (s->recording.records)
+      return launch_coop_build<NT, STAGE, COMPACT, MINW, true>(s, p, plan, list, count, n_fixed, resume, update_rates,
+                                                             st);
+    return launch_coop_build<NT, STAGE, COMPACT, MINW, false>(s, p, plan, list, count, n_fixed, resume, update_rates,
+                                                            st);
   }
 
   template<int NT>
@@ -2980,8 +2998,8 @@ extern "C"
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null state");
     if (threads != 0 && threads != 64 && threads != 128 && threads != 256 && threads != 512 && threads != 1024)
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "threads per cell must be 0, 64, 128, 256, 512 or 1024");
-    if (stage < -1 || stage > 4)
-      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "stage must be -1 .. 4");
+    if (stage < -1 || stage > 6)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "stage must be -1 .. 6");
     s->coop_threads = threads;
     s->coop_stage = stage;
     return MICM_HIP_OK;
