@@ -403,7 +403,7 @@ def main():
     _copy_device(y_initial.data_ptr(), ptr.variables, n_y * 8, stream)  # pristine initial state
 
     # the per-cell forms run as one graph (the Solve's launches and the cost-order pass),
-    # captured at the first step and replayed after
+    # captured at the first step, again at the second (the cost order exists from then on), replayed after
     packed = None
     if form in (_lib.SOLVE_COOPERATIVE, _lib.SOLVE_PERSISTENT) and not args.no_graph:
         packed = _lib.PackedSolve([state], form)

@@ -3426,6 +3426,9 @@ extern "C"
       shapes.push_back(static_cast<int64_t>(reinterpret_cast<intptr_t>(s->coop_k)));
       shapes.push_back(s->coop_k_rows);
       shapes.push_back(static_cast<int64_t>(reinterpret_cast<intptr_t>(s->slot_blob)));
+      // the first Solve of a state runs its cells in natural order, the later ones in the
+      // previous Solve's cost order: the graph captured first is replaced once the order exists
+      shapes.push_back(s->has_order ? 1 : 0);
     }
     const bool same = pk->captured && pk->dt == time_step && pk->shapes == shapes &&
                       std::memcmp(&pk->params, params, sizeof *params) == 0;
