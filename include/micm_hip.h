@@ -299,8 +299,8 @@ int micm_hip_state_set_handoff(micm_hip_state* state, int64_t steps);
  * do not fit next to the cell's data stay in global memory), or 4: the compact
  * per-cell layout (no tables staged, fewer arrays in LDS so that more cells fit a CU;
  * needs more than 64 threads and at least one thread per species, else ignored), its
- * register budget chosen from occupancy; 5 / 6: the compact layout with 256 threads built
- * for three waves per SIMD (registers spill to scratch) / without that bound.  A tuning
+ * register budget chosen from occupancy; 5 / 6 / 7: the compact layout with 256 threads built
+ * for three waves per SIMD (168 registers) / two (256) / no register bound.  A tuning
  * knob in the role of the vector length L of the reference's GPU policies
  * (cuda_solver_builder.hpp:24-38); results do not depend on it. */
 int micm_hip_state_set_cooperative_shape(micm_hip_state* state, int32_t threads, int32_t stage);

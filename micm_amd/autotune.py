@@ -26,9 +26,9 @@ CANDIDATES = (
     ("cooperative256", _lib.SOLVE_COOPERATIVE, -1, 256, -1),
     ("cooperative64", _lib.SOLVE_COOPERATIVE, -1, 64, -1),
     ("cooperative512", _lib.SOLVE_COOPERATIVE, -1, 512, -1),
-    # the compact layout's two register budgets: three waves per SIMD with spills, or none
-    ("cooperative256-compact-bounded", _lib.SOLVE_COOPERATIVE, -1, 256, 5),
-    ("cooperative256-compact-unbounded", _lib.SOLVE_COOPERATIVE, -1, 256, 6),
+    # the compact layout's register budgets: three waves per SIMD (168 registers) or two (256)
+    ("cooperative256-compact-w3", _lib.SOLVE_COOPERATIVE, -1, 256, 5),
+    ("cooperative256-compact-w2", _lib.SOLVE_COOPERATIVE, -1, 256, 6),
     ("cooperative64-global-tables", _lib.SOLVE_COOPERATIVE, -1, 64, 0),
     ("cooperative128-global-tables", _lib.SOLVE_COOPERATIVE, -1, 128, 0),
     ("persistent+handoff16", _lib.SOLVE_PERSISTENT, 16, 0, -1),

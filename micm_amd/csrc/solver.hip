@@ -1354,8 +1354,9 @@ namespace
   //
   // With 256 threads per cell the kernel is compiled for three waves per SIMD (at most
   // 168 VGPRs, a few of them spilled), which lets a third cell onto a CU where LDS
-  // allows (TS1: 1.26x); the compact layout also has a build without that bound
-  // (MINW 1), taken where LDS allows two cells per CU anyway (210/560).
+  // allows (TS1: 1.26x); the compact layout also has builds for two waves per SIMD (256
+  // registers) and without a bound (MINW 2, 1): where LDS allows two cells per CU anyway
+  // (210/560) the one that keeps two cells per CU with the fewest spills is taken.
   template<int NT>
   constexpr int coop_min_waves_per_eu()
   {
@@ -2066,20 +2067,23 @@ namespace
     int threads = kWave;
     int p_capacity = 0;     // doubles of the per-level product buffer
     bool compact = false;   // coop_layout_compact
-    bool unbounded = false;  // compact, 256 threads: the build without the three-waves bound
+    // compact, 256 threads: the build's register budget as waves per SIMD (3: 168 registers,
+    // 2: 256, 1: unbounded); 0: the NT's default build
+    int min_waves = 0;
   };
 
   // Workgroups of the cooperative kernel that fit one CU (registers and LDS).
   template<int NT>
-  int coop_occupancy_nt(int stage, bool compact, size_t lds, bool unbounded = false)
+  int coop_occupancy_nt(int stage, bool compact, size_t lds, int min_waves = 0)
   {
     int per_cu = 0;
     hipError_t e = hipSuccess;
     if constexpr (NT == 256)
     {
-      if (compact && unbounded)
+      if (compact && (min_waves == 1 || min_waves == 2))
       {
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coop<NT, 0, true, 1>, NT, lds);
+        e = min_waves == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coop<NT, 0, true, 1>, NT, lds)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coop<NT, 0, true, 2>, NT, lds);
         return e == hipSuccess ? per_cu : 0;
       }
     }
@@ -2096,12 +2100,12 @@ namespace
     return e == hipSuccess ? per_cu : 0;
   }
 
-  int coop_occupancy(int threads, int stage, bool compact, size_t lds, bool unbounded = false)
+  int coop_occupancy(int threads, int stage, bool compact, size_t lds, int min_waves = 0)
   {
     switch (threads)
     {
       case 128: return coop_occupancy_nt<128>(stage, compact, lds);
-      case 256: return coop_occupancy_nt<256>(stage, compact, lds, unbounded);
+      case 256: return coop_occupancy_nt<256>(stage, compact, lds, min_waves);
       case 512: return coop_occupancy_nt<512>(stage, compact, lds);
       case 1024: return coop_occupancy_nt<1024>(stage, compact, lds);
       default: return coop_occupancy_nt<kWave>(stage, compact, lds);
@@ -2184,8 +2188,8 @@ namespace
   // more cells per CU; 4: the compact layout; threads_request 0: one wavefront.
   CoopPlan plan_coop(const micm_hip_mechanism* m, int stages, int stage_request = -1, int threads_request = 0)
   {
-    // 4: the compact layout, its register budget chosen by occupancy; 5 / 6: the compact
-    // layout built for three waves per SIMD (registers spill) / without that bound (256 threads)
+    // 4: the compact layout, its register budget chosen by occupancy; 5 / 6 / 7: the compact
+    // layout built for three / two waves per SIMD / without a register bound (256 threads)
     const bool compact_request = stage_request >= 4;
     CoopPlan plan = plan_coop_full(m, stages, compact_request ? -1 : stage_request, threads_request);
     const DevCoop& c = m->coop;
@@ -2204,9 +2208,19 @@ namespace
                            sizeof(double);
     if (compact > lds_limit())
       return plan;
-    const int occ_bounded = coop_occupancy(threads, 0, true, compact);
-    const int occ_unbounded = threads == 256 ? coop_occupancy(threads, 0, true, compact, true) : 0;
-    const int occ_compact = std::max(occ_bounded, occ_unbounded);
+    // the 256-thread compact build in three register budgets: the one that fits the most cells
+    // per CU, the largest budget among equals (fewer spills)
+    int occ[4] = { 0, 0, 0, 0 };
+    int best_waves = 0;
+    for (int w = 1; w <= 3; ++w)
+    {
+      if (threads != 256 && w != 3)
+        continue;
+      occ[w] = coop_occupancy(threads, 0, true, compact, w);
+      if (best_waves == 0 || occ[w] > occ[best_waves])
+        best_waves = w;
+    }
+    const int occ_compact = occ[best_waves];
     const int occ_full = plan.lds > 0 ? coop_occupancy(threads, plan.stage, false, plan.lds) : 0;
     if (compact_request ? occ_compact > 0 : occ_compact > occ_full)
     {
@@ -2215,11 +2229,11 @@ namespace
       plan.threads = threads;
       plan.p_capacity = c.bwd.max_level_terms;
       plan.compact = true;
-      plan.unbounded = occ_unbounded >= occ_bounded;  // no register spills when it costs no cell per CU
-      if (threads == 256 && stage_request == 5)
-        plan.unbounded = false;
-      else if (threads == 256 && stage_request == 6 && occ_unbounded > 0)
-        plan.unbounded = true;
+      plan.min_waves = threads == 256 ? best_waves : 0;
+      // 5 / 6 / 7: three / two waves per SIMD / no register bound, where that build fits
+      const int forced = stage_request == 5 ? 3 : stage_request == 6 ? 2 : stage_request == 7 ? 1 : 0;
+      if (threads == 256 && forced && occ[forced] > 0)
+        plan.min_waves = forced;
     }
     return plan;
   }
@@ -2281,7 +2295,7 @@ namespace
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     blocks = static_cast<int64_t>(cus) *
-             std::max(coop_occupancy(plan.threads, plan.stage, plan.compact, plan.lds, plan.unbounded), 1);
+             std::max(coop_occupancy(plan.threads, plan.stage, plan.compact, plan.lds, plan.min_waves), 1);
     return MICM_HIP_OK;
   }
 
@@ -2330,8 +2344,10 @@ namespace
   {
     if constexpr (NT == 256)
     {
-      if (plan.compact && plan.unbounded)
+      if (plan.compact && plan.min_waves == 1)
         return launch_coop_shape<NT, 0, true, 1>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
+      if (plan.compact && plan.min_waves == 2)
+        return launch_coop_shape<NT, 0, true, 2>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
     }
     if (plan.compact)
       return launch_coop_shape<NT, 0, true>(s, p, plan, list, count, n_fixed, resume, update_rates, st);
@@ -2998,8 +3014,8 @@ extern "C"
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null state");
     if (threads != 0 && threads != 64 && threads != 128 && threads != 256 && threads != 512 && threads != 1024)
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "threads per cell must be 0, 64, 128, 256, 512 or 1024");
-    if (stage < -1 || stage > 6)
-      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "stage must be -1 .. 6");
+    if (stage < -1 || stage > 7)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "stage must be -1 .. 7");
     s->coop_threads = threads;
     s->coop_stage = stage;
     return MICM_HIP_OK;

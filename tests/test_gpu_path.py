@@ -209,7 +209,7 @@ def test_per_cell_forms_agree_bitwise(name):
             _solve(mech, cells, order, tableau, dt, _lib.SOLVE_PERSISTENT, handoff=3, coop_shape=(256, 4))]
     if mech.cooperative_supported(tableau):
         for threads, stage in ((64, -1), (128, 0), (256, 3), (64, 1), (512, -1), (1024, 2), (128, 4), (256, 4),
-                              (256, 5), (256, 6)):
+                              (256, 5), (256, 6), (256, 7)):
             runs.append(_solve(mech, cells, order, tableau, dt, _lib.SOLVE_COOPERATIVE, coop_shape=(threads, stage)))
     for r in runs:
         assert bits_equal(r[0], ref[0])
@@ -259,7 +259,7 @@ def test_cooperative_shape_arguments_checked():
     cells = inputs(desc, 70, seed=3)
     order = mech.species_order()
     state = _lib.DeviceState(mech, cells.n_cells)
-    for threads, stage in ((96, -1), (2048, -1), (256, 7), (256, -2)):
+    for threads, stage in ((96, -1), (2048, -1), (256, 8), (256, -2)):
         with pytest.raises(_lib.MicmHipError):
             state.set_cooperative_shape(threads, stage)
     ref = _solve(mech, cells, order, tableau, dt, _lib.SOLVE_PLAIN)
