@@ -131,7 +131,11 @@ def autotune(mech, state, dt, params, sample=65536, first_sample=2048, pilot=256
     m1 = min(first_sample, m)
     if m1 > m0 and len(cands) > 1:
         times.update(_time_candidates(mech, *pick(m1), dt, params, cands, repeats, verbose))
-        cands = sorted(cands, key=lambda c: times[c[0]])[:finalists]
+        ranked = sorted(cands, key=lambda c: times[c[0]])
+        best1 = times[ranked[0][0]]
+        # the plain form gains the most from more cells: it stays in at twice the best
+        cands = ranked[:finalists] + [c for c in ranked[finalists:]
+                                      if c[1] == _lib.SOLVE_PLAIN and times[c[0]] <= 2.0 * best1]
     last = cands + lane_forms
     if m > m1:
         times.update(_time_candidates(mech, *pick(m), dt, params, last, repeats, verbose))

@@ -100,3 +100,17 @@ def _fresh_cache():
     autotune._cache.clear()
     yield
     autotune._cache.clear()
+
+
+def test_plain_form_reaches_the_last_sample_within_twice_the_best(monkeypatch):
+    """Fourth at 2,048 cells but within twice the best, the plain form is timed on the
+    last sample, where its launches amortise over every cell and it wins (TS1's identical
+    cells)."""
+    model = {label: _linear(0.5, 1e-4) for label, *_ in autotune.CANDIDATES}
+    model["cooperative256"] = _linear(0.004, 1.3e-5)
+    model["cooperative128"] = _linear(0.004, 1.7e-5)
+    model["cooperative512"] = _linear(0.004, 1.5e-5)
+    model["plain"] = lambda n: 0.03 + 0.9e-5 * n if n > 256 else 0.024
+    choice, calls = _run(monkeypatch, model, 10000, tag=105)
+    assert ("plain", 10000) in calls
+    assert choice["label"] == "plain"
