@@ -386,7 +386,8 @@ def main():
     if args.form == "autotune":
         choice = autotune.autotune(mech, state, dt, params, verbose=True)
         form = autotune.apply(state, choice)
-        form_label = f"autotuned: {choice['label']}"
+        form_label = f"autotuned: {choice['label']}" + (f" (hand-off to {choice['handoff_to']})"
+                                                         if choice.get("handoff_to") else "")
         coop_threads = choice["threads"] if form == _lib.SOLVE_COOPERATIVE else 0
     else:
         form = {"persistent": _lib.SOLVE_PERSISTENT, "plain": _lib.SOLVE_PLAIN,
@@ -800,7 +801,8 @@ def main_adjoint(args):
     def loss(y):
         return float(np.sum(y[:, targets] ** 2))
 
-    errs = {"fd_y0": [], "fd_logA": [], "tangent_y0": [], "tangent_logA": []}
+    errs = {"fd_y0": [], "fd_logA": [], "fd_y0_h1e-4": [], "fd_logA_h1e-4": [], "tangent_y0": [],
+            "tangent_logA": []}
     t2 = time.perf_counter()
     for _ in range(3):
         # unit directions (relative in y0), so that h is the size of the perturbation
@@ -810,8 +812,14 @@ def main_adjoint(args):
         w = w / np.linalg.norm(w)
         ad_y = float(np.sum(g_y[sample] * v))
         ad_k = float(np.sum(g_k[sample] @ w))
-        fd_y = (loss(o.replay(k, ys + h * v, ht)) - loss(o.replay(k, ys - h * v, ht))) / (2 * h)
-        fd_k = (loss(o.replay(k * np.exp(h * w), ys, ht)) - loss(o.replay(k * np.exp(-h * w), ys, ht))) / (2 * h)
+        def central(step):
+            return ((loss(o.replay(k, ys + step * v, ht)) - loss(o.replay(k, ys - step * v, ht))) / (2 * step),
+                    (loss(o.replay(k * np.exp(step * w), ys, ht)) - loss(o.replay(k * np.exp(-step * w), ys, ht)))
+                    / (2 * step))
+        fd_y, fd_k = central(h)
+        # at h = 1e-6 the differences' rounding (amplified through the recorded steps) dominates
+        # their error; at 1e-4 their truncation error is still far smaller
+        fd_y4, fd_k4 = central(1e-4)
         dyf = o.tangent(k, ys, ht, np.stack([v, np.zeros_like(v)], 1),
                         np.stack([np.zeros((len(sample), R)), np.broadcast_to(w, (len(sample), R))], 1))
         lam_s = lam[sample]
@@ -823,6 +831,8 @@ def main_adjoint(args):
         sk = float(np.sum(np.abs(g_k[sample] * w)))
         errs["fd_y0"].append(abs(ad_y - fd_y) / sy)
         errs["fd_logA"].append(abs(ad_k - fd_k) / sk)
+        errs["fd_y0_h1e-4"].append(abs(ad_y - fd_y4) / sy)
+        errs["fd_logA_h1e-4"].append(abs(ad_k - fd_k4) / sk)
         errs["tangent_y0"].append(abs(ad_y - tl_y) / sy)
         errs["tangent_logA"].append(abs(ad_k - tl_k) / sk)
     log(f"gradient checks on {len(sample)} cells: {time.perf_counter() - t2:.1f} s")
@@ -848,8 +858,9 @@ def main_adjoint(args):
                            **{key: float(max(v)) for key, v in errs.items()},
                            "note": "largest error of a directional derivative of the loss over the sampled cells, "
                                    "relative to the sum of its terms' magnitudes: central differences of the recorded "
-                                   "map, and the CPU tangent-linear model (per-component checks of every gradient: "
-                                   "tests/test_gpu_adjoint.py)"},
+                                   "map (h = 1e-6 relative, and 1e-4: the error at 1e-6 is the differences' own "
+                                   "rounding), and the CPU tangent-linear model (per-component checks of every "
+                                   "gradient: tests/test_gpu_adjoint.py)"},
     }
     print(json.dumps(result), flush=True)
 

@@ -85,13 +85,15 @@ def autotune(mech, state, dt, params, sample=65536, first_sample=2048, pilot=256
     are set), on strided samples of its cells: every candidate on `pilot` cells,
     those within twice the best (the plain form: four times) on `first_sample` cells,
     the `finalists` fastest of those, with the persistent forms that hand slow cells
-    on, on `sample` cells.  The last sample fills the chip for the per-lane forms (64
+    on, on `sample` cells; a persistent form hands its slow cells to the fastest
+    cooperative shape timed before it.  The last sample fills the chip for the per-lane forms (64
     lanes x 4 SIMDs x 256 CUs = 65,536 cells): on fewer cells a per-lane form waits for
     its slowest cell with most lanes idle, and the cooperative shapes' time is that of
     their slowest cells, which favours the shapes that run one cell fastest over those
     that run the most cells at once.  Returns a dict with keys label, form, handoff,
-    threads, stage, sample, seconds (label -> seconds on the largest sample the
-    candidate reached)."""
+    threads, stage (for a persistent form: the hand-off's shape), sample, seconds (label -> seconds on the largest sample the
+    candidate reached), handoff_to (the cooperative candidate a persistent choice hands its
+    slow cells to)."""
     n = state.n_cells
     m = min(sample, n)
     key = _key(mech, params.stages, dt, m)
@@ -106,13 +108,28 @@ def autotune(mech, state, dt, params, sample=65536, first_sample=2048, pilot=256
         idx = np.linspace(0, n - 1, count).round().astype(np.int64)
         return y_all[idx], k_all[idx]
 
+    def per_lane(c):
+        return c[1] == _lib.SOLVE_PERSISTENT and c[2] > 0
+
+    def handing_to(c, pool):
+        """`c` with the hand-off shape of the fastest cooperative candidate of `pool` timed
+        so far (the cells a persistent form parks are finished by that shape)."""
+        coop = [d for d in pool if d[1] == _lib.SOLVE_COOPERATIVE and d[0] in times]
+        if not per_lane(c) or not coop:
+            return c
+        best_coop = min(coop, key=lambda d: times[d[0]])
+        hand_to[c[0]] = best_coop[0]
+        return (c[0], c[1], c[2], best_coop[3], best_coop[4])
+
     m0 = min(pilot, m)
     times = {}
+    hand_to = {}
     best = None
     slow_handoff = False
     for c in cands:  # pilot: each candidate gives up once it is four times slower than the best so far
         if c[1] == _lib.SOLVE_PERSISTENT and c[2] == 0 and slow_handoff:
             continue  # without hand-off the slowest cells run even longer one per lane
+        c = handing_to(c, cands)
         t = _time_candidates(mech, *pick(m0), dt, params, [c], repeats, verbose,
                              give_up=None if best is None else 4.0 * best)[c[0]]
         times[c[0]] = t
@@ -123,8 +140,6 @@ def autotune(mech, state, dt, params, sample=65536, first_sample=2048, pilot=256
     # cell at once), so a pilot understates it: it stays in at four times the best.  The
     # persistent forms with a hand-off are timed on the last sample whatever the pilot
     # says: a few hundred cells leave almost every lane idle behind the slowest one.
-    def per_lane(c):
-        return c[1] == _lib.SOLVE_PERSISTENT and c[2] > 0
     lane_forms = [c for c in cands if per_lane(c)]
     cands = [c for c in cands
              if c[0] in times and not per_lane(c) and times[c[0]] <= (4.0 if c[1] == _lib.SOLVE_PLAIN else 2.0) * best]
@@ -136,6 +151,8 @@ def autotune(mech, state, dt, params, sample=65536, first_sample=2048, pilot=256
         # the plain form gains the most from more cells: it stays in at twice the best
         cands = ranked[:finalists] + [c for c in ranked[finalists:]
                                       if c[1] == _lib.SOLVE_PLAIN and times[c[0]] <= 2.0 * best1]
+    # the hand-off goes to the fastest cooperative shape on the largest sample timed so far
+    lane_forms = [handing_to(c, cands) for c in lane_forms]
     last = cands + lane_forms
     if m > m1:
         times.update(_time_candidates(mech, *pick(m), dt, params, last, repeats, verbose))
@@ -144,6 +161,6 @@ def autotune(mech, state, dt, params, sample=65536, first_sample=2048, pilot=256
     cands = [c for c in last if c[0] in times]
     label, form, handoff, threads, stage = min(cands, key=lambda c: times[c[0]])
     choice = {"label": label, "form": form, "handoff": handoff, "threads": threads, "stage": stage,
-              "sample": m, "seconds": times}
+              "handoff_to": hand_to.get(label) if handoff > 0 else None, "sample": m, "seconds": times}
     _cache[key] = dict(choice)
     return choice

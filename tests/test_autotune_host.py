@@ -114,3 +114,27 @@ def test_plain_form_reaches_the_last_sample_within_twice_the_best(monkeypatch):
     choice, calls = _run(monkeypatch, model, 10000, tag=105)
     assert ("plain", 10000) in calls
     assert choice["label"] == "plain"
+
+
+def test_hand_off_goes_to_the_fastest_cooperative_shape(monkeypatch):
+    """The persistent forms finish their slow cells in the cooperative form: they are
+    timed, and chosen, with the shape of the fastest cooperative candidate, not with the
+    one-wavefront default."""
+    seen = {}
+    model = {label: _linear(0.5, 1e-4) for label, *_ in autotune.CANDIDATES}
+    model["cooperative256-compact-w2"] = _linear(0.3, 8.5e-5)
+    model["persistent+handoff16"] = lambda n: 1.4 if n <= 2048 else 0.2 + 8.2e-5 * n
+
+    def fake(mech, y, k, dt, params, candidates, repeats, verbose, give_up=None):
+        out = {}
+        for c in candidates:
+            seen.setdefault(c[0], []).append((y.shape[0], c[3], c[4]))
+            out[c[0]] = model[c[0]](y.shape[0])
+        return out
+
+    monkeypatch.setattr(autotune, "_time_candidates", fake)
+    choice = autotune.autotune(_Mech(106), _State(100000), 30.0, types.SimpleNamespace(stages=4))
+    w2 = next(c for c in autotune.CANDIDATES if c[0] == "cooperative256-compact-w2")
+    assert (65536, w2[3], w2[4]) in seen["persistent+handoff16"]
+    assert choice["label"] == "persistent+handoff16" and choice["handoff_to"] == "cooperative256-compact-w2"
+    assert (choice["threads"], choice["stage"]) == (w2[3], w2[4])
