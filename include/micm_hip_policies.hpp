@@ -59,10 +59,13 @@ namespace micm
 {
   namespace hip_detail
   {
+    // A non-zero status of the library as the CUDA backend reports a failed CUDA call
+    // (src/util/cuda_util.cu:9-16): a MicmException of the internal-error category.
     inline void check(int rc, const char* what)
     {
       if (rc != MICM_HIP_OK)
-        throw std::runtime_error(std::string(what) + ": " + micm_hip_last_error());
+        throw MicmException(
+            MICM_ERROR_CATEGORY_INTERNAL, MICM_INTERNAL_ERROR_CODE_GENERAL, std::string(micm_hip_last_error()) + " : " + what);
     }
 
     // The processes in the order the builder sorted them, species by the builder's state
@@ -163,7 +166,8 @@ namespace micm
           device_(hip_detail::CompileProcesses(processes, variable_map))
     {
       if (!external_process_sets.empty())
-        throw std::runtime_error("HipProcessSet: external model process sets are not supported on the device");
+        throw MicmException(MICM_ERROR_CATEGORY_INTERNAL, MICM_INTERNAL_ERROR_CODE_GENERAL,
+                            "HipProcessSet: external model process sets are not supported on the device");
     }
   };
 
@@ -272,7 +276,8 @@ namespace micm
           constraints_(std::move(constraints))
     {
       if (constraints_.Size() > 0)
-        throw std::runtime_error("HipRosenbrockSolver: constraints are not supported on the device");
+        throw MicmException(MICM_ERROR_CATEGORY_SOLVER, MICM_SOLVER_ERROR_CODE_CUDA_CONSTRAINTS_UNSUPPORTED,
+                            "HipRosenbrockSolver: constraints are not supported on the device");
     }
     HipRosenbrockSolver(HipRosenbrockSolver&&) noexcept = default;
     HipRosenbrockSolver& operator=(HipRosenbrockSolver&&) noexcept = default;
