@@ -190,8 +190,9 @@ def cpu_lockstep(desc, sub, order, dt, tableau):
 
 
 def kernel_rooflines(mech, state, stream, n_cells, repeats=5):
-    """Per-kernel time of the single-phase entry points (one cell per lane) over the
-    whole State, and their algorithmic bytes as a fraction of the HBM peak."""
+    """Per-kernel time of the single-phase entry points over the whole State, and their
+    algorithmic bytes as a fraction of the HBM peak (the accumulating forcing and Jacobian
+    entries also read their output: 8 N and 8 nnzJ bytes per cell beyond the count)."""
     import torch
     N, R, P = mech.n_species, mech.n_reactions, mech.n_custom
     i = mech.info
@@ -204,7 +205,9 @@ def kernel_rooflines(mech, state, stream, n_cells, repeats=5):
     per_cell = {
         "rate_constants": 8 * (3 + P + R),
         "add_forcing": 8 * (2 * N + R),
+        "fill_and_add_forcing": 8 * (2 * N + R),
         "subtract_jacobian": 8 * (N + R + i.nnz_jacobian),
+        "fill_and_subtract_jacobian": 8 * (N + R + i.nnz_jacobian),
         "alpha_minus_jacobian": 8 * 2 * N,
         "lu_decompose": 8 * (i.nnz_jacobian + i.nnz_lower + i.nnz_upper),
         "lu_solve": 8 * (i.nnz_lower + i.nnz_upper),
@@ -216,6 +219,12 @@ def kernel_rooflines(mech, state, stream, n_cells, repeats=5):
         "add_forcing": lambda: mech.add_forcing_terms(n_cells, ptr.rate_constants, ptr.variables, f.data_ptr(), st),
         "subtract_jacobian": lambda: mech.subtract_jacobian_terms(n_cells, ptr.rate_constants, ptr.variables,
                                                                   ptr.jacobian, st),
+        # the pairs Solve issues at a step start: Fill(0) then AddForcingTerms /
+        # SubtractJacobianTerms, the output only written
+        "fill_and_add_forcing": lambda: mech.fill_and_add_forcing_terms(n_cells, ptr.rate_constants, ptr.variables,
+                                                                        f.data_ptr(), st),
+        "fill_and_subtract_jacobian": lambda: mech.fill_and_subtract_jacobian_terms(
+            n_cells, ptr.rate_constants, ptr.variables, ptr.jacobian, st),
         "alpha_minus_jacobian": lambda: mech.alpha_minus_jacobian(n_cells, ptr.jacobian, 1.0e3, st),
         "lu_decompose": lambda: mech.lu_decompose(n_cells, ptr.jacobian, ptr.lower, ptr.upper, st),
         "lu_solve": lambda: mech.lu_solve(n_cells, ptr.lower, ptr.upper, x.data_ptr(), st),

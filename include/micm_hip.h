@@ -233,10 +233,21 @@ int micm_hip_calculate_rate_constants(const micm_hip_mechanism* mechanism, int64
 int micm_hip_add_forcing_terms(const micm_hip_mechanism* mechanism, int64_t n_cells,
                                const double* rate_constants, const double* variables,
                                double* forcing, void* stream);
+/* The pair Rosenbrock::Solve issues at each step start, initial_forcing.Fill(0) then
+ * AddForcingTerms (rosenbrock.inl:98-99), in one pass: forcing = f(y), bit for bit what the
+ * pair gives; forcing is only written. */
+int micm_hip_fill_and_add_forcing_terms(const micm_hip_mechanism* mechanism, int64_t n_cells,
+                                        const double* rate_constants, const double* variables,
+                                        double* forcing, void* stream);
 /* SubtractJacobianTermsKernelDriver (cuda_process_set.cuh:37-41): jacobian -= J(y). */
 int micm_hip_subtract_jacobian_terms(const micm_hip_mechanism* mechanism, int64_t n_cells,
                                      const double* rate_constants, const double* variables,
                                      double* jacobian, void* stream);
+/* The pair jacobian_.Fill(0) then SubtractJacobianTerms (rosenbrock.inl:109-110) in one
+ * pass: jacobian = -J(y), bit for bit what the pair gives; jacobian is only written. */
+int micm_hip_fill_and_subtract_jacobian_terms(const micm_hip_mechanism* mechanism, int64_t n_cells,
+                                              const double* rate_constants, const double* variables,
+                                              double* jacobian, void* stream);
 /* AlphaMinusJacobianDriver (cuda_rosenbrock.cuh:18-22): jacobian[diag] += alpha. */
 int micm_hip_alpha_minus_jacobian(const micm_hip_mechanism* mechanism, int64_t n_cells,
                                   double* jacobian, double alpha, void* stream);

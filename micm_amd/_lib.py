@@ -69,7 +69,9 @@ _SIGNATURES = {
     "micm_hip_mechanism_get_jacobian_diagonal": [_vp, _ip],
     "micm_hip_calculate_rate_constants": [_vp, _i64, _vp, _vp, _vp, _vp],
     "micm_hip_add_forcing_terms": [_vp, _i64, _vp, _vp, _vp, _vp],
+    "micm_hip_fill_and_add_forcing_terms": [_vp, _i64, _vp, _vp, _vp, _vp],
     "micm_hip_subtract_jacobian_terms": [_vp, _i64, _vp, _vp, _vp, _vp],
+    "micm_hip_fill_and_subtract_jacobian_terms": [_vp, _i64, _vp, _vp, _vp, _vp],
     "micm_hip_alpha_minus_jacobian": [_vp, _i64, _vp, ctypes.c_double, _vp],
     "micm_hip_lu_decompose": [_vp, _i64, _vp, _vp, _vp, _vp],
     "micm_hip_lu_solve": [_vp, _i64, _vp, _vp, _vp, _vp],
@@ -250,6 +252,14 @@ class Mechanism:
 
     def subtract_jacobian_terms(self, n_cells, k, y, jac, stream=None):
         check(lib().micm_hip_subtract_jacobian_terms(self.h, n_cells, k, y, jac, stream))
+
+    def fill_and_add_forcing_terms(self, n_cells, k, y, f, stream=None):
+        """f = f(y): Fill(0) and AddForcingTerms in one pass."""
+        check(lib().micm_hip_fill_and_add_forcing_terms(self.h, n_cells, k, y, f, stream))
+
+    def fill_and_subtract_jacobian_terms(self, n_cells, k, y, jac, stream=None):
+        """jac = -J(y): Fill(0) and SubtractJacobianTerms in one pass."""
+        check(lib().micm_hip_fill_and_subtract_jacobian_terms(self.h, n_cells, k, y, jac, stream))
 
     def alpha_minus_jacobian(self, n_cells, jac, alpha, stream=None):
         check(lib().micm_hip_alpha_minus_jacobian(self.h, n_cells, jac, float(alpha), stream))

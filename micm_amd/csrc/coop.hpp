@@ -151,26 +151,27 @@ namespace micm_hip
   }
 
   // v + sum_q coef[q] * src[idx[q] * TC + cc] for q in [b, e), added in order, in batches
-  // of eight (the lists end in eight padding entries)
-  template<int TC = 1>
+  // of B <= 8 (the lists end in eight padding entries)
+  template<int TC = 1, int B = 8>
   __device__ __forceinline__ double gather_sum(double v, const int32_t* idx, const double* coef, int b, int e,
                                                const double* src, int cc = 0)
   {
-    for (int q = b; q < e; q += 8)
+    static_assert(B >= 1 && B <= 8, "the lists end in eight padding entries");
+    for (int q = b; q < e; q += B)
     {
-      int ix[8];
-      double cf[8], sv[8];
+      int ix[B];
+      double cf[B], sv[B];
 #pragma unroll
-      for (int r = 0; r < 8; ++r)
+      for (int r = 0; r < B; ++r)
       {
         ix[r] = idx[q + r];
         cf[r] = coef[q + r];
       }
 #pragma unroll
-      for (int r = 0; r < 8; ++r)
+      for (int r = 0; r < B; ++r)
         sv[r] = src[ix[r] * TC + cc];
 #pragma unroll
-      for (int r = 0; r < 8; ++r)
+      for (int r = 0; r < B; ++r)
         if (q + r < e)
           v += cf[r] * sv[r];
     }

@@ -97,6 +97,8 @@ namespace
       kk[r] = rate_constant(m, r, T, P, rho, cu);
   }
 
+  // FROM_ZERO: the output is cleared first (the reference's Fill(0) before the call)
+  template<bool FROM_ZERO = false>
   __global__ void __launch_bounds__(kBlock) k_add_forcing(DevMech m, int64_t n_cells, const double* __restrict__ k,
                                                           const double* __restrict__ y, double* __restrict__ f)
   {
@@ -105,10 +107,15 @@ namespace
       return;
     const size_t g = c / kGroup;
     const int l = c % kGroup;
+    auto out = lane_view(f, g, m.n_species, l);
+    if constexpr (FROM_ZERO)
+      for (int i = 0; i < m.n_species; ++i)
+        out[i] = 0.0;
     add_forcing(m, lane_view(const_cast<double*>(k), g, m.n_reactions, l), lane_view(const_cast<double*>(y), g, m.n_species, l),
-                lane_view(f, g, m.n_species, l));
+                out);
   }
 
+  template<bool FROM_ZERO = false>
   __global__ void __launch_bounds__(kBlock) k_subtract_jacobian(DevMech m, int64_t n_cells, const double* __restrict__ k,
                                                                 const double* __restrict__ y, double* __restrict__ jac)
   {
@@ -117,8 +124,12 @@ namespace
       return;
     const size_t g = c / kGroup;
     const int l = c % kGroup;
+    auto out = lane_view(jac, g, m.nnz_jac, l);
+    if constexpr (FROM_ZERO)
+      for (int i = 0; i < m.nnz_jac; ++i)
+        out[i] = 0.0;
     subtract_jacobian(m, lane_view(const_cast<double*>(k), g, m.n_reactions, l),
-                      lane_view(const_cast<double*>(y), g, m.n_species, l), lane_view(jac, g, m.nnz_jac, l));
+                      lane_view(const_cast<double*>(y), g, m.n_species, l), out);
   }
 
   // ---- tiled single-phase kernels: one workgroup per TC cells of a group, the tile's
@@ -221,8 +232,10 @@ namespace
   }
 
   // f += f(y): rates of the tile's cells into LDS (in place over k), then per-species
-  // gathers; the old value of the next item is loaded while the current one is summed
-  template<int TC, int NT>
+  // gathers; the old value of the next item is loaded while the current one is summed.
+  // FROM_ZERO: f = f(y), the reference's Fill(0) and AddForcingTerms in one pass (f is only
+  // written).
+  template<int TC, int NT, bool FROM_ZERO = false>
   __global__ void __launch_bounds__(NT) k_add_forcing_tiled(DevMech m, DevCoop c, int64_t n_cells,
                                                             const double* __restrict__ k,
                                                             const double* __restrict__ y, double* __restrict__ f)
@@ -246,12 +259,12 @@ namespace
     const int ns = N * TC;
     double* out = f + g * static_cast<size_t>(N) * kGroup + q0 + cc;
     int i = threadIdx.x;
-    double v_next = i < ns ? out[(i / TC) * kGroup] : 0.0;
+    double v_next = !FROM_ZERO && i < ns ? out[(i / TC) * kGroup] : 0.0;
     for (; i < ns; i += NT)
     {
       const int sp = i / TC;
       double v = v_next;
-      if (i + NT < ns)
+      if (!FROM_ZERO && i + NT < ns)
         v_next = out[((i + NT) / TC) * kGroup];
       v = gather_sum<TC>(v, c.spec_rxn, c.spec_coef, c.spec_offsets[sp], c.spec_offsets[sp + 1], rt, cc);
       out[sp * kGroup] = v;
@@ -259,10 +272,12 @@ namespace
   }
 
   // jac -= J(y): per Jacobian element, its terms' products k * y... in the reference's
-  // order, four entries at a time with packed operands.  DUE: only the running cells
-  // whose control flags ask for a new Jacobian, each formed from zero (the per-cell
-  // forms' step start); the other cells' Jacobians are left as they are.
-  template<int TC, int NT, bool DUE>
+  // order, two entries at a time with packed operands (elements average one to two
+  // terms).  FROM_ZERO: jac = -J(y), the reference's Fill(0) and SubtractJacobianTerms in
+  // one pass (jac is only written).  DUE: only the running cells whose control flags ask
+  // for a new Jacobian, each formed from zero (the per-cell forms' step start); the other
+  // cells' Jacobians are left as they are.
+  template<int TC, int NT, bool DUE, bool FROM_ZERO = DUE>
   __device__ inline void subtract_jacobian_tile(const DevMech& m, const DevCoop& c, int64_t n_cells,
                                                 const double* __restrict__ k, const double* __restrict__ y,
                                                 double* __restrict__ jac, const CellControl* __restrict__ ctl)
@@ -292,34 +307,35 @@ namespace
     const int nnz = m.nnz_jac;
     const int ne = nnz * TC;
     double* out = jac + g * static_cast<size_t>(nnz) * kGroup + q0 + cc;
+    constexpr int B = 2;
     int i = threadIdx.x;
-    double v_next = !DUE && i < ne ? out[(i / TC) * kGroup] : 0.0;
+    double v_next = !FROM_ZERO && i < ne ? out[(i / TC) * kGroup] : 0.0;
     for (; i < ne; i += NT)
     {
       const int e = i / TC;
       double v = v_next;
-      if (!DUE && i + NT < ne)
+      if (!FROM_ZERO && i + NT < ne)
         v_next = out[((i + NT) / TC) * kGroup];
       const int qe = c.jel_offsets[e + 1];
-      for (int q = c.jel_offsets[e]; q < qe; q += 4)
+      for (int q = c.jel_offsets[e]; q < qe; q += B)
       {
-        uint64_t pk[4];
-        double cf[4], kk[4], y0[4], y1[4];
+        uint64_t pk[B];
+        double cf[B], kk[B], y0[B], y1[B];
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int r = 0; r < B; ++r)
         {
           pk[r] = c.jel_pack[q + r];
           cf[r] = c.jel_coef[q + r];
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int r = 0; r < B; ++r)
         {
           kk[r] = kt[(pk[r] & 0xffffu) * TC + cc];
           y0[r] = yt[((pk[r] >> 16) & 0xffffu) * TC + cc];
           y1[r] = yt[((pk[r] >> 32) & 0xffffu) * TC + cc];
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int r = 0; r < B; ++r)
         {
           if (q + r < qe)
           {
@@ -343,13 +359,13 @@ namespace
     }
   }
 
-  template<int TC, int NT>
+  template<int TC, int NT, bool FROM_ZERO = false>
   __global__ void __launch_bounds__(NT) k_subtract_jacobian_tiled(DevMech m, DevCoop c, int64_t n_cells,
                                                                   const double* __restrict__ k,
                                                                   const double* __restrict__ y,
                                                                   double* __restrict__ jac)
   {
-    subtract_jacobian_tile<TC, NT, false>(m, c, n_cells, k, y, jac, nullptr);
+    subtract_jacobian_tile<TC, NT, false, FROM_ZERO>(m, c, n_cells, k, y, jac, nullptr);
   }
 
   template<int TC, int NT>
@@ -788,9 +804,6 @@ namespace
     CellControl cc = ctl[c];
     if (cc.status != MICM_HIP_STATE_RUNNING)
       return;
-    const Lane f0 = lane_view(w.f0, c / kGroup, m.n_species, c % kGroup);
-    for (int i = 0; i < m.n_species; ++i)
-      f0[i] = 0.0;
     if (cc.flags & kNewStep)
     {
       if (begin_step(p, cc))
@@ -855,7 +868,6 @@ namespace
         for (int j = 0; j < s; ++j)
           v += p.a[sc + j] * K[static_cast<size_t>(j) * w.k_stride];
         w.ynew[f] = v;
-        *Ks = 0.0;
         if (first)
           ctl[c].stats[MICM_HIP_STAT_FUNCTION_CALLS] += 1;
       }
@@ -2173,6 +2185,15 @@ namespace
                        stream, __VA_ARGS__);                                                                        \
   } while (0)
 
+#define LAUNCH_GATHER_Z(kernel, tc, zero, lds, stream, ...)                                                         \
+  do                                                                                                                \
+  {                                                                                                                 \
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel<tc, kGatherThreads, zero>),                    \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));                \
+    hipLaunchKernelGGL((kernel<tc, kGatherThreads, zero>), dim3(tiled_blocks(tc, n_cells)), dim3(kGatherThreads),   \
+                       lds, stream, __VA_ARGS__);                                                                   \
+  } while (0)
+
 #define LAUNCH_TILED(kernel, blocks, lds, stream, ...)                                                               \
   do                                                                                                                \
   {                                                                                                                 \
@@ -2661,8 +2682,10 @@ extern "C"
     return MICM_HIP_OK;
   }
 
-  int micm_hip_add_forcing_terms(const micm_hip_mechanism* m, int64_t n_cells, const double* k, const double* y,
-                                 double* f, void* stream)
+  // (C++ linkage: templates; the entry points below are the C-ABI)
+  extern "C++" template<bool FROM_ZERO>
+  int add_forcing_entry(const micm_hip_mechanism* m, int64_t n_cells, const double* k, const double* y, double* f,
+                        void* stream)
   {
     KERNEL_PRELUDE();
     if (!k || !y || !f)
@@ -2673,22 +2696,24 @@ extern "C"
       const size_t lds = gather_tile_bytes(m, tc);
       switch (tc)
       {
-        case 16: LAUNCH_GATHER(k_add_forcing_tiled, 16, lds, st, m->dev, m->coop, n_cells, k, y, f); break;
-        case 8: LAUNCH_GATHER(k_add_forcing_tiled, 8, lds, st, m->dev, m->coop, n_cells, k, y, f); break;
-        case 4: LAUNCH_GATHER(k_add_forcing_tiled, 4, lds, st, m->dev, m->coop, n_cells, k, y, f); break;
-        default: LAUNCH_GATHER(k_add_forcing_tiled, 2, lds, st, m->dev, m->coop, n_cells, k, y, f); break;
+        case 16: LAUNCH_GATHER_Z(k_add_forcing_tiled, 16, FROM_ZERO, lds, st, m->dev, m->coop, n_cells, k, y, f); break;
+        case 8: LAUNCH_GATHER_Z(k_add_forcing_tiled, 8, FROM_ZERO, lds, st, m->dev, m->coop, n_cells, k, y, f); break;
+        case 4: LAUNCH_GATHER_Z(k_add_forcing_tiled, 4, FROM_ZERO, lds, st, m->dev, m->coop, n_cells, k, y, f); break;
+        default: LAUNCH_GATHER_Z(k_add_forcing_tiled, 2, FROM_ZERO, lds, st, m->dev, m->coop, n_cells, k, y, f); break;
       }
     }
     else
     {
-      hipLaunchKernelGGL(k_add_forcing, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, k, y, f);
+      hipLaunchKernelGGL(k_add_forcing<FROM_ZERO>, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, k, y,
+                         f);
     }
     HIP_TRY(hipGetLastError());
     return MICM_HIP_OK;
   }
 
-  int micm_hip_subtract_jacobian_terms(const micm_hip_mechanism* m, int64_t n_cells, const double* k, const double* y,
-                                       double* jac, void* stream)
+  extern "C++" template<bool FROM_ZERO>
+  int subtract_jacobian_entry(const micm_hip_mechanism* m, int64_t n_cells, const double* k, const double* y,
+                              double* jac, void* stream)
   {
     KERNEL_PRELUDE();
     if (!k || !y || !jac)
@@ -2699,18 +2724,51 @@ extern "C"
       const size_t lds = gather_tile_bytes(m, tc);
       switch (tc)
       {
-        case 16: LAUNCH_GATHER(k_subtract_jacobian_tiled, 16, lds, st, m->dev, m->coop, n_cells, k, y, jac); break;
-        case 8: LAUNCH_GATHER(k_subtract_jacobian_tiled, 8, lds, st, m->dev, m->coop, n_cells, k, y, jac); break;
-        case 4: LAUNCH_GATHER(k_subtract_jacobian_tiled, 4, lds, st, m->dev, m->coop, n_cells, k, y, jac); break;
-        default: LAUNCH_GATHER(k_subtract_jacobian_tiled, 2, lds, st, m->dev, m->coop, n_cells, k, y, jac); break;
+        case 16:
+          LAUNCH_GATHER_Z(k_subtract_jacobian_tiled, 16, FROM_ZERO, lds, st, m->dev, m->coop, n_cells, k, y, jac);
+          break;
+        case 8:
+          LAUNCH_GATHER_Z(k_subtract_jacobian_tiled, 8, FROM_ZERO, lds, st, m->dev, m->coop, n_cells, k, y, jac);
+          break;
+        case 4:
+          LAUNCH_GATHER_Z(k_subtract_jacobian_tiled, 4, FROM_ZERO, lds, st, m->dev, m->coop, n_cells, k, y, jac);
+          break;
+        default:
+          LAUNCH_GATHER_Z(k_subtract_jacobian_tiled, 2, FROM_ZERO, lds, st, m->dev, m->coop, n_cells, k, y, jac);
+          break;
       }
     }
     else
     {
-      hipLaunchKernelGGL(k_subtract_jacobian, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, k, y, jac);
+      hipLaunchKernelGGL(k_subtract_jacobian<FROM_ZERO>, dim3(grid_for(n_cells)), dim3(kBlock), 0, st, m->dev, n_cells, k,
+                         y, jac);
     }
     HIP_TRY(hipGetLastError());
     return MICM_HIP_OK;
+  }
+
+  int micm_hip_add_forcing_terms(const micm_hip_mechanism* m, int64_t n_cells, const double* k, const double* y,
+                                 double* f, void* stream)
+  {
+    return add_forcing_entry<false>(m, n_cells, k, y, f, stream);
+  }
+
+  int micm_hip_fill_and_add_forcing_terms(const micm_hip_mechanism* m, int64_t n_cells, const double* k,
+                                          const double* y, double* f, void* stream)
+  {
+    return add_forcing_entry<true>(m, n_cells, k, y, f, stream);
+  }
+
+  int micm_hip_subtract_jacobian_terms(const micm_hip_mechanism* m, int64_t n_cells, const double* k, const double* y,
+                                       double* jac, void* stream)
+  {
+    return subtract_jacobian_entry<false>(m, n_cells, k, y, jac, stream);
+  }
+
+  int micm_hip_fill_and_subtract_jacobian_terms(const micm_hip_mechanism* m, int64_t n_cells, const double* k,
+                                                const double* y, double* jac, void* stream)
+  {
+    return subtract_jacobian_entry<true>(m, n_cells, k, y, jac, stream);
   }
 
   int micm_hip_alpha_minus_jacobian(const micm_hip_mechanism* m, int64_t n_cells, double* jac, double alpha, void* stream)
@@ -3108,7 +3166,8 @@ extern "C"
           {
             hipLaunchKernelGGL(k_plain_prepare, dim3(grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl);
           }
-          if (int rc = micm_hip_add_forcing_terms(m, s->n_cells, w.k, w.y, w.f0, st))
+          // the initial forcing formed from zero: Fill(0) and AddForcingTerms (rosenbrock.inl:98-99)
+          if (int rc = micm_hip_fill_and_add_forcing_terms(m, s->n_cells, w.k, w.y, w.f0, st))
             return rc;
           if (int rc = micm_hip_lu_decompose(m, s->n_cells, w.jac, w.lower, w.upper, st))
             return rc;
@@ -3119,7 +3178,8 @@ extern "C"
             {
               hipLaunchKernelGGL(k_stage_rhs_elem<1>, dim3(pair_grid), dim3(kBlock), 0, st, d, p, s->n_cells, w,
                                  s->ctl, stage);
-              if (int rc = micm_hip_add_forcing_terms(m, s->n_cells, w.k, w.ynew, Ks, st))
+              // K[stage] = f(Y_stage) from zero (rosenbrock.inl:156-157)
+              if (int rc = micm_hip_fill_and_add_forcing_terms(m, s->n_cells, w.k, w.ynew, Ks, st))
                 return rc;
             }
             hipLaunchKernelGGL(k_stage_rhs_elem<2>, dim3(pair_grid), dim3(kBlock), 0, st, d, p, s->n_cells, w, s->ctl,

@@ -72,6 +72,8 @@ class _Base:
                 "rate_constants": [_vp, ctypes.c_int64, _dp, _dp, _dp],
                 "forcing": [_vp, ctypes.c_int64, _dp, _dp, _dp],
                 "jacobian": [_vp, ctypes.c_int64, _dp, _dp, _dp],
+                "add_forcing": [_vp, ctypes.c_int64, _dp, _dp, _dp],
+                "subtract_jacobian": [_vp, ctypes.c_int64, _dp, _dp, _dp],
                 "lu": [_vp, ctypes.c_int64, _dp, ctypes.c_double, _dp, _dp, _dp],
                 "lu_solve": [_vp, ctypes.c_int64, _dp, _dp, _dp],
                 "normalized_error": [_vp, ctypes.c_int64, _dp, _dp, _dp, ctypes.c_double, _dp, _dp],
@@ -139,18 +141,29 @@ class _Base:
         self._call("rate_constants", n, _d(conditions), _d(custom), _d(k))
         return k
 
-    def forcing(self, k, y):
+    def forcing(self, k, y, f0=None):
+        """f(y) (Fill(0) then AddForcingTerms); with f0, AddForcingTerms into a copy of f0."""
         k = np.ascontiguousarray(k, dtype=np.float64)
         y = np.ascontiguousarray(y, dtype=np.float64)
-        f = np.zeros_like(y)
-        self._call("forcing", y.shape[0], _d(k), _d(y), _d(f))
+        if f0 is None:
+            f = np.zeros_like(y)
+            self._call("forcing", y.shape[0], _d(k), _d(y), _d(f))
+        else:
+            f = np.array(f0, dtype=np.float64, copy=True, order="C")
+            self._call("add_forcing", y.shape[0], _d(k), _d(y), _d(f))
         return f
 
-    def jacobian(self, k, y):
+    def jacobian(self, k, y, jac0=None):
+        """-J(y) (Fill(0) then SubtractJacobianTerms); with jac0, SubtractJacobianTerms into
+        a copy of jac0."""
         k = np.ascontiguousarray(k, dtype=np.float64)
         y = np.ascontiguousarray(y, dtype=np.float64)
-        jac = np.zeros((y.shape[0], self.nnz_jacobian))
-        self._call("jacobian", y.shape[0], _d(k), _d(y), _d(jac))
+        if jac0 is None:
+            jac = np.zeros((y.shape[0], self.nnz_jacobian))
+            self._call("jacobian", y.shape[0], _d(k), _d(y), _d(jac))
+        else:
+            jac = np.array(jac0, dtype=np.float64, copy=True, order="C")
+            self._call("subtract_jacobian", y.shape[0], _d(k), _d(y), _d(jac))
         return jac
 
     def lu(self, jac, alpha):

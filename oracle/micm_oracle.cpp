@@ -1286,6 +1286,26 @@ extern "C"
     return 0;
   }
 
+  // AddForcingTerms / SubtractJacobianTerms accumulating into f / jac as they hold
+  // (process_set.hpp:524-670), without the Fill(0) the two functions above apply first
+  int oracle_add_forcing(void* h, int64_t n_cells, const double* k, const double* y, double* f)
+  {
+    const Mech& M = static_cast<OracleSolver*>(h)->M;
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < n_cells; ++c)
+      add_forcing(M, k + c * M.R, y + c * M.N, f + c * M.N);
+    return 0;
+  }
+
+  int oracle_subtract_jacobian(void* h, int64_t n_cells, const double* k, const double* y, double* jac)
+  {
+    const Mech& M = static_cast<OracleSolver*>(h)->M;
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < n_cells; ++c)
+      subtract_jacobian(M, k + c * M.R, y + c * M.N, jac + c * M.nnzA);
+    return 0;
+  }
+
   int oracle_lu(void* h, int64_t n_cells, const double* jac, double alpha, double* jac_out, double* lower, double* upper)
   {
     const Mech& M = static_cast<OracleSolver*>(h)->M;

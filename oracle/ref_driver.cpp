@@ -443,6 +443,50 @@ extern "C"
     return 0;
   }
 
+  // AddForcingTerms into f [cell][species] as it holds (no Fill(0))
+  int ref_add_forcing(void* h, int64_t n_cells, const double* k, const double* y, double* f)
+  {
+    auto* s = static_cast<RefSolver*>(h);
+    StateT st = s->solver->GetState(n_cells);
+    auto Y = st.variables_;
+    auto F = st.variables_;
+    for (int64_t c = 0; c < n_cells; ++c)
+    {
+      for (int64_t r = 0; r < s->n_reactions; ++r)
+        st.rate_constants_[c][r] = k[c * s->n_reactions + r];
+      for (int64_t i = 0; i < s->n_species; ++i)
+      {
+        Y[c][i] = y[c * s->n_species + i];
+        F[c][i] = f[c * s->n_species + i];
+      }
+    }
+    s->solver->solver_.rates_.AddForcingTerms(st, Y, F);
+    for (int64_t c = 0; c < n_cells; ++c)
+      for (int64_t i = 0; i < s->n_species; ++i)
+        f[c * s->n_species + i] = F[c][i];
+    return 0;
+  }
+
+  // SubtractJacobianTerms into jac [cell][nnz] as it holds (no Fill(0))
+  int ref_subtract_jacobian(void* h, int64_t n_cells, const double* k, const double* y, double* jac)
+  {
+    auto* s = static_cast<RefSolver*>(h);
+    StateT st = s->solver->GetState(n_cells);
+    auto Y = st.variables_;
+    for (int64_t c = 0; c < n_cells; ++c)
+    {
+      for (int64_t r = 0; r < s->n_reactions; ++r)
+        st.rate_constants_[c][r] = k[c * s->n_reactions + r];
+      for (int64_t i = 0; i < s->n_species; ++i)
+        Y[c][i] = y[c * s->n_species + i];
+    }
+    auto& v = st.jacobian_.AsVector();
+    std::memcpy(v.data(), jac, v.size() * sizeof(double));
+    s->solver->solver_.rates_.SubtractJacobianTerms(st, Y, st.jacobian_);
+    std::memcpy(jac, v.data(), v.size() * sizeof(double));
+    return 0;
+  }
+
   // -J [cell][nnz] = SubtractJacobianTerms on a zeroed matrix
   int ref_jacobian(void* h, int64_t n_cells, const double* k, const double* y, double* jac)
   {

@@ -75,12 +75,30 @@ def test_kernels_match_oracle(case, n_cells):
     f_ref = oracle.forcing(k_ref, y)
     assert bits_equal(host(f_dev, n_cells, N), f_ref)
 
-    # -J: bitwise
+    # accumulating into a forcing that holds values (AddForcingTerms without Fill(0)), and
+    # Fill(0) with AddForcingTerms in one pass over a buffer of NaNs: bitwise
+    rng = np.random.default_rng(n_cells)
+    f0 = rng.standard_normal((n_cells, N)) * 1e-3
+    fa_dev = dev(f0)
+    mech.add_forcing_terms(n_cells, kr_dev.data_ptr(), y_dev.data_ptr(), fa_dev.data_ptr(), s)
+    assert bits_equal(host(fa_dev, n_cells, N), oracle.forcing(k_ref, y, f0))
+    fz_dev = dev(np.full((n_cells, N), np.nan))
+    mech.fill_and_add_forcing_terms(n_cells, kr_dev.data_ptr(), y_dev.data_ptr(), fz_dev.data_ptr(), s)
+    assert bits_equal(host(fz_dev, n_cells, N), f_ref)
+
+    # -J: bitwise, from zero, accumulating, and Fill(0) with SubtractJacobianTerms in one pass
     nj = mech.nnz_jacobian
     j_dev = dev_zeros(n_cells, nj)
     mech.subtract_jacobian_terms(n_cells, kr_dev.data_ptr(), y_dev.data_ptr(), j_dev.data_ptr(), s)
     j_ref = oracle.jacobian(k_ref, y)
     assert bits_equal(host(j_dev, n_cells, nj), j_ref)
+    j0 = rng.standard_normal((n_cells, nj))
+    ja_dev = dev(j0)
+    mech.subtract_jacobian_terms(n_cells, kr_dev.data_ptr(), y_dev.data_ptr(), ja_dev.data_ptr(), s)
+    assert bits_equal(host(ja_dev, n_cells, nj), oracle.jacobian(k_ref, y, j0))
+    jz_dev = dev(np.full((n_cells, nj), np.nan))
+    mech.fill_and_subtract_jacobian_terms(n_cells, kr_dev.data_ptr(), y_dev.data_ptr(), jz_dev.data_ptr(), s)
+    assert bits_equal(host(jz_dev, n_cells, nj), j_ref)
 
     # alpha - J and Doolittle LU: bitwise
     alpha = 1.0 / (37.5 * 0.5728200000000000)

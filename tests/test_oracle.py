@@ -77,6 +77,12 @@ def test_oracle_matches_live_reference(name):
     assert bits_equal(o.rate_constants(cells.conditions, cells.custom), k)
     j = ref.jacobian(k, y)
     assert bits_equal(o.jacobian(k, y), j)
+    # AddForcingTerms / SubtractJacobianTerms accumulate into what the output holds
+    rng = np.random.default_rng(5)
+    f0 = rng.standard_normal(y.shape) * 1e-3
+    assert bits_equal(o.forcing(k, y, f0), ref.forcing(k, y, f0))
+    j0 = rng.standard_normal(j.shape)
+    assert bits_equal(o.jacobian(k, y, j0), ref.jacobian(k, y, j0))
     for tab in ([2, 3, 4, 41, 61] if name in ("readme", "all_types") else [tableau]):
         r2, o2 = Reference(desc, tab), Oracle(desc, tab)
         a = r2.solve(cells.conditions, cells.custom, y, 120.0, 2)
