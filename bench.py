@@ -322,6 +322,8 @@ def main():
                     help="cells (0: 100000 for 210_560, 10000 for 60_200 and ts1, 1000 for readme)")
     ap.add_argument("--form", default="autotune", choices=["autotune", "persistent", "plain", "cooperative"])
     ap.add_argument("--coop-threads", type=int, default=0, help="threads per cell of the cooperative form (0: library)")
+    ap.add_argument("--coop-stage", type=int, default=-1,
+                    help="table staging / layout of the cooperative form (micm_hip_state_set_cooperative_shape)")
     ap.add_argument("--handoff", type=int, default=-1, help="persistent form: hand-off step count (-1: library)")
     ap.add_argument("--compare-sample", type=int, default=8192,
                     help="cells on which the plain form and the chosen form are timed side by side (0: skip)")
@@ -401,7 +403,7 @@ def main():
     else:
         form = {"persistent": _lib.SOLVE_PERSISTENT, "plain": _lib.SOLVE_PLAIN,
                 "cooperative": _lib.SOLVE_COOPERATIVE}[args.form]
-        state.set_cooperative_shape(args.coop_threads, -1)
+        state.set_cooperative_shape(args.coop_threads, args.coop_stage)
         state.set_handoff(args.handoff)
         form_label = args.form + (f" ({args.coop_threads} threads per cell)" if args.coop_threads else "")
         coop_threads = args.coop_threads if form == _lib.SOLVE_COOPERATIVE else 0
