@@ -315,6 +315,13 @@ int micm_hip_state_set_handoff(micm_hip_state* state, int64_t steps);
  * knob in the role of the vector length L of the reference's GPU policies
  * (cuda_solver_builder.hpp:24-38); results do not depend on it. */
 int micm_hip_state_set_cooperative_shape(micm_hip_state* state, int32_t threads, int32_t stage);
+/* Blocking of the cooperative form's LU level schedule: the shortest run of products an
+ * element's ordered sum (lu_decomposition_doolittle.inl:325-403) is split into, except its
+ * last -- 1, 2, 4 or 8; -1 keeps every sum whole; 0: the library's default (4).  Every
+ * blocking subtracts the same products in the same order; it only moves where a sum is
+ * split between the levels of the schedule.  A tuning knob (the autotuner's "LU schedule
+ * blocking"); results do not depend on it. */
+int micm_hip_state_set_lu_blocking(micm_hip_state* state, int32_t min_segment);
 /* Whether the cooperative form fits this mechanism on the current device (its
  * per-cell working set must fit one workgroup's LDS); *out = 1 or 0. */
 int micm_hip_mechanism_cooperative_supported(const micm_hip_mechanism* mechanism, int32_t stages, int32_t* out);

@@ -90,6 +90,7 @@ _SIGNATURES = {
     "micm_hip_state_set_stream": [_vp, _vp],
     "micm_hip_state_set_handoff": [_vp, _i64],
     "micm_hip_state_set_cooperative_shape": [_vp, _i32, _i32],
+    "micm_hip_state_set_lu_blocking": [_vp, ctypes.c_int32],
     "micm_hip_packed_create": [_i32, ctypes.POINTER(_vp), _i32, ctypes.POINTER(_vp)],
     "micm_hip_packed_solve": [_vp, ctypes.c_double, ctypes.POINTER(RosenbrockParameters)],
     "micm_hip_packed_replayed": [_vp],
@@ -351,9 +352,14 @@ class DeviceState:
         check(lib().micm_hip_state_set_handoff(self.h, int(steps)))
 
     def set_cooperative_shape(self, threads=0, stage=-1):
-        """Threads per cell (64/128/256, 0 = library choice) and LDS staging of the
-        substitution (bit 0) and LU (bit 1) tables (-1 = library choice)."""
+        """Threads per cell (64 .. 1024, 0 = library choice) and LDS staging / layout
+        (micm_hip_state_set_cooperative_shape; -1 = library choice)."""
         check(lib().micm_hip_state_set_cooperative_shape(self.h, int(threads), int(stage)))
+
+    def set_lu_blocking(self, min_segment):
+        """Shortest segment of the cooperative form's LU schedule: 1, 2, 4, 8, -1 (whole
+        sums) or 0 (the library's default)."""
+        check(lib().micm_hip_state_set_lu_blocking(self.h, int(min_segment)))
 
     def results(self):
         states = np.zeros(self.n_cells, dtype=np.int32)

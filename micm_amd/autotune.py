@@ -38,6 +38,10 @@ CANDIDATES = (
     ("plain", _lib.SOLVE_PLAIN, -1, 0, -1),
 )
 
+# LU schedule blockings tried besides the library's default (shortest segment of an
+# element's ordered sum; -1 keeps sums whole): micm_hip_state_set_lu_blocking
+LU_BLOCKINGS = (1, 2, 8, -1)
+
 _cache = {}
 
 
@@ -51,15 +55,17 @@ def apply(state, choice):
     """Configure `state` with a choice returned by `autotune`."""
     state.set_handoff(choice["handoff"])
     state.set_cooperative_shape(choice["threads"], choice["stage"])
+    state.set_lu_blocking(choice.get("lu_blocking", 0))
     return choice["form"]
 
 
-def _time_candidates(mech, y, k, dt, params, candidates, repeats, verbose, give_up=None):
+def _time_candidates(mech, y, k, dt, params, candidates, repeats, verbose, give_up=None, lu_blocking=0):
     """Seconds per Solve of each candidate on these cells (best of `repeats` after a
     warm-up run); a candidate whose warm-up run takes more than `give_up` seconds is
     not repeated and keeps that time."""
     probe = _lib.DeviceState(mech, y.shape[0])
     probe.set_rate_constants(k)
+    probe.set_lu_blocking(lu_blocking)
     times = {}
     for label, form, handoff, threads, stage in candidates:
         probe.set_handoff(handoff)
@@ -74,7 +80,8 @@ def _time_candidates(mech, y, k, dt, params, candidates, repeats, verbose, give_
                 break
         times[label] = min(el[1:]) if len(el) > 1 else el[0]
         if verbose:
-            print(f"autotune: {label:32s} {y.shape[0]:6d} cells {times[label] * 1e3:9.2f} ms", file=sys.stderr,
+            tag = label + (f" (LU blocking {lu_blocking})" if lu_blocking else "")
+            print(f"autotune: {tag:44s} {y.shape[0]:6d} cells {times[label] * 1e3:9.2f} ms", file=sys.stderr,
                   flush=True)
     return times
 
@@ -86,14 +93,16 @@ def autotune(mech, state, dt, params, sample=65536, first_sample=2048, pilot=256
     those within twice the best (the plain form: four times) on `first_sample` cells,
     the `finalists` fastest of those, with the persistent forms that hand slow cells
     on, on `sample` cells; a persistent form hands its slow cells to the fastest
-    cooperative shape timed before it.  The last sample fills the chip for the per-lane forms (64
+    cooperative shape of that sample.  The last sample fills the chip for the per-lane forms (64
     lanes x 4 SIMDs x 256 CUs = 65,536 cells): on fewer cells a per-lane form waits for
     its slowest cell with most lanes idle, and the cooperative shapes' time is that of
     their slowest cells, which favours the shapes that run one cell fastest over those
     that run the most cells at once.  Returns a dict with keys label, form, handoff,
     threads, stage (for a persistent form: the hand-off's shape), sample, seconds (label -> seconds on the largest sample the
     candidate reached), handoff_to (the cooperative candidate a persistent choice hands its
-    slow cells to)."""
+    slow cells to), lu_blocking (micm_hip_state_set_lu_blocking, for the fastest
+    cooperative shape of the last sample; 0 keeps the default) and lu_blocking_seconds
+    (first-sample times of that shape with each blocking)."""
     n = state.n_cells
     m = min(sample, n)
     key = _key(mech, params.stages, dt, m)
@@ -124,6 +133,8 @@ def autotune(mech, state, dt, params, sample=65536, first_sample=2048, pilot=256
     m0 = min(pilot, m)
     times = {}
     hand_to = {}
+    lu_blocking = 0
+    blocking_times = {}
     best = None
     slow_handoff = False
     for c in cands:  # pilot: each candidate gives up once it is four times slower than the best so far
@@ -151,16 +162,40 @@ def autotune(mech, state, dt, params, sample=65536, first_sample=2048, pilot=256
         # the plain form gains the most from more cells: it stays in at twice the best
         cands = ranked[:finalists] + [c for c in ranked[finalists:]
                                       if c[1] == _lib.SOLVE_PLAIN and times[c[0]] <= 2.0 * best1]
-    # the hand-off goes to the fastest cooperative shape on the largest sample timed so far
-    lane_forms = [handing_to(c, cands) for c in lane_forms]
-    last = cands + lane_forms
+    first_times = dict(times)
     if m > m1:
-        times.update(_time_candidates(mech, *pick(m), dt, params, last, repeats, verbose))
-    elif m > m0 and lane_forms:  # the others were timed on these m cells already
-        times.update(_time_candidates(mech, *pick(m), dt, params, lane_forms, repeats, verbose))
+        times.update(_time_candidates(mech, *pick(m), dt, params, cands, repeats, verbose))
+    # the LU schedule's blocking, for the fastest cooperative shape of the last sample, timed
+    # on the first sample; a blocking that is 2 % faster there is timed on the last sample too
+    # and taken if it is faster there as well
+    coop = sorted((c for c in cands if c[1] == _lib.SOLVE_COOPERATIVE and c[0] in times), key=lambda c: times[c[0]])
+    if coop and m1 > m0 and coop[0][0] in first_times:
+        top = coop[0]
+        base = first_times[top[0]]
+        best_t, best_b = base, 0
+        for b in LU_BLOCKINGS:
+            t = _time_candidates(mech, *pick(m1), dt, params, [top], repeats, verbose, lu_blocking=b)[top[0]]
+            blocking_times[b] = t
+            if t < 0.98 * base and t < best_t:
+                best_t, best_b = t, b
+        if best_b != 0:
+            if m > m1:
+                t = _time_candidates(mech, *pick(m), dt, params, [top], repeats, verbose, lu_blocking=best_b)[top[0]]
+                if t < times[top[0]]:
+                    times[top[0]], lu_blocking = t, best_b
+            else:
+                times[top[0]], lu_blocking = best_t, best_b
+    # the hand-off goes to the fastest cooperative shape of the largest sample (with the LU
+    # blocking chosen for it)
+    lane_forms = [handing_to(c, cands) for c in lane_forms]
+    if m > m0 and lane_forms:
+        times.update(_time_candidates(mech, *pick(m), dt, params, lane_forms, repeats, verbose,
+                                      lu_blocking=lu_blocking))
+    last = cands + lane_forms
     cands = [c for c in last if c[0] in times]
     label, form, handoff, threads, stage = min(cands, key=lambda c: times[c[0]])
     choice = {"label": label, "form": form, "handoff": handoff, "threads": threads, "stage": stage,
-              "handoff_to": hand_to.get(label) if handoff > 0 else None, "sample": m, "seconds": times}
+              "handoff_to": hand_to.get(label) if handoff > 0 else None, "lu_blocking": lu_blocking,
+              "lu_blocking_seconds": blocking_times, "sample": m, "seconds": times}
     _cache[key] = dict(choice)
     return choice

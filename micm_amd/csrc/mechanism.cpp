@@ -190,11 +190,10 @@ namespace micm_hip
     // The last segment runs no earlier than `final_min` and carries the divisor; an
     // element without products gets one task that only divides.  Returns the level
     // of the last segment.
-    // Shortest segment (except an element's last) of the LU and substitution sums: the
-    // LU's many short segments cost more in per-task overhead than they save in chain
-    // length, the substitutions' do not (measured on MI355X for 60/200, TS1 and
-    // 210/560).
-    constexpr size_t kLuSegmentMin = 4;
+    // Shortest segment (except an element's last) of the substitution sums; the LU's is a
+    // choice (kLuBlockings): its many short segments can cost more in per-task overhead
+    // than they save in chain length, the substitutions' do not (measured on MI355X for
+    // 60/200, TS1 and 210/560).
     constexpr size_t kSolveSegmentMin = 1;
 
     int32_t add_segments(std::vector<LevelTask>& out, uint32_t pos, uint32_t diag, const std::vector<uint32_t>& terms,
@@ -267,13 +266,18 @@ namespace micm_hip
       // level schedules (16-bit positions; 0xffff marks "no divisor")
       t.coop_levels_ok = t.alu_col.size() < kUnitDivisor && n < 0xffff && t.n_reactions < 0xffff;
       t.lv_lu = LevelSchedule{};
+      t.lv_lu_blocked.clear();
       t.lv_fwd = LevelSchedule{};
       t.lv_bwd = LevelSchedule{};
       if (!t.coop_levels_ok)
         return;
       // LU: one task per Doolittle element that has products to subtract or a divisor;
-      // the level of an ALU position is one more than the highest level it reads
+      // the level of an ALU position is one more than the highest level it reads.  One
+      // schedule per blocking.
+      t.lv_lu_blocked.clear();
+      for (int b = 0; b < kNumLuBlockings; ++b)
       {
+        const size_t min_len = kLuBlockings[b] > 0 ? static_cast<size_t>(kLuBlockings[b]) : ~size_t(0);
         std::vector<int32_t> level(t.alu_col.size(), 0), ready;
         std::vector<uint32_t> terms;
         std::vector<LevelTask> tasks;
@@ -295,9 +299,11 @@ namespace micm_hip
             ready.push_back(std::max(level[l], level[u]));
           }
           level[out] = add_segments(tasks, static_cast<uint32_t>(out), diag, terms, ready, lower ? level[diag] : 0,
-                                    kLuSegmentMin) + 1;
+                                    min_len) + 1;
         }
-        t.lv_lu = pack_levels(tasks);
+        t.lv_lu_blocked.push_back(pack_levels(tasks));
+        if (kLuBlockings[b] == kLuBlockingDefault)
+          t.lv_lu = t.lv_lu_blocked.back();
       }
       // forward substitution: row i subtracts L[i][j] x[j], j ascending (unit diagonal)
       {

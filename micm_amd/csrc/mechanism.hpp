@@ -16,6 +16,14 @@ namespace micm_hip
   constexpr uint32_t kNoDivisor = 0xffff;    // no division
   constexpr uint32_t kUnitDivisor = 0xfffe;  // forward substitution: divide by the row's L[i][i]
 
+  // Blocking of the LU level schedule: the shortest segment (products) an element's
+  // ordered sum is split into, except its last; 0 keeps each sum whole.  Longer segments
+  // mean fewer tasks per level, shorter ones a sum that starts as soon as its first
+  // operands are final.  Results do not depend on it.
+  constexpr int kLuBlockings[] = { 1, 2, 4, 8, 0 };
+  constexpr int kNumLuBlockings = 5;
+  constexpr int kLuBlockingDefault = 4;
+
   // Independent tasks grouped by dependency level (see MechanismTables::lv_lu).
   struct LevelSchedule
   {
@@ -92,6 +100,9 @@ namespace micm_hip
     // the last segment divides.
     bool coop_levels_ok = false;                           // positions fit the 16-bit fields
     LevelSchedule lv_lu;   // out = ALU position, diag = U[i][i] position (L elements), terms (l | u << 16)
+    // the LU schedule for each shortest-segment length of kLuBlockings (lv_lu is the one
+    // for kLuBlockingDefault)
+    std::vector<LevelSchedule> lv_lu_blocked;
     LevelSchedule lv_fwd;  // out = row, diag = kUnitDivisor on a row's last segment, terms (ALU position | column << 16)
     LevelSchedule lv_bwd;  // out = row, diag = U[i][i] position, terms (ALU position | column << 16)
     // packed operands: reaction -> reactant ids, Jacobian term -> (process, dependent ids);

@@ -1690,6 +1690,7 @@ struct micm_hip_mechanism
   MechanismTables t;
   DevMech dev{};
   DevCoop coop{};
+  DevLevels lu_blocked[kNumLuBlockings] = {};  // the LU schedule of each blocking (kLuBlockings)
   DevAdjoint adj{};
   void* blob = nullptr;  // one device allocation for every table
   int device = 0;
@@ -1729,6 +1730,9 @@ struct micm_hip_state
   // cooperative launch shape (-1 / 0: chosen from the mechanism)
   int coop_stage = -1;
   int coop_threads = 0;
+  // shortest LU segment of the cooperative form's schedule (0: kLuBlockingDefault; -1:
+  // whole sums)
+  int lu_blocking = 0;
   // the tape a recorded Solve appends to (records == nullptr otherwise)
   Tape recording{ nullptr, nullptr, 0 };
   // vector length of the dense matrix policy whose summation order the lockstep form's
@@ -1809,6 +1813,14 @@ namespace
   size_t level_bytes(const LevelSchedule& l)
   {
     return padded_bytes(l.offsets) + padded_bytes(l.tasks) + padded_bytes(l.terms);
+  }
+
+  size_t blocked_level_bytes(const MechanismTables& t)
+  {
+    size_t bytes = 0;
+    for (const LevelSchedule& l : t.lv_lu_blocked)
+      bytes += level_bytes(l);
+    return bytes;
   }
 
   int check_mech(const micm_hip_mechanism* m)
@@ -1986,7 +1998,7 @@ namespace
              padded_bytes(t.lower_diag) + padded_bytes(t.upper_diag) + padded_bytes(t.fwd_offsets) +
              padded_bytes(t.fwd_pairs) + padded_bytes(t.bwd_offsets) + padded_bytes(t.bwd_pairs) +
              padded_bytes(t.jac_to_alu) + padded_bytes(t.alu_diag) + padded_bytes(t.alu_lower) +
-             padded_bytes(t.alu_upper) + level_bytes(t.lv_lu) + level_bytes(t.lv_fwd) +
+             padded_bytes(t.alu_upper) + level_bytes(t.lv_lu) + blocked_level_bytes(t) + level_bytes(t.lv_fwd) +
              level_bytes(t.lv_bwd) +
              padded_bytes(t.rxn_pack) + padded_bytes(t.term_pack) +
              padded_bytes(t.spec_offsets) + padded_bytes(t.spec_rxn) +
@@ -2042,6 +2054,8 @@ namespace
     upload(m->blob, off, t.alu_lower, c.alu_lower, staging);
     upload(m->blob, off, t.alu_upper, c.alu_upper, staging);
     upload_levels(m->blob, off, t.lv_lu, c.lu, staging);
+    for (size_t b = 0; b < t.lv_lu_blocked.size() && b < static_cast<size_t>(kNumLuBlockings); ++b)
+      upload_levels(m->blob, off, t.lv_lu_blocked[b], m->lu_blocked[b], staging);
     upload_levels(m->blob, off, t.lv_fwd, c.fwd, staging);
     upload_levels(m->blob, off, t.lv_bwd, c.bwd, staging);
     upload(m->blob, off, t.rxn_pack, c.rxn_pack, staging);
@@ -2202,18 +2216,43 @@ namespace
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kTileThreads), lds, stream, __VA_ARGS__);                        \
   } while (0)
 
-  CoopPlan plan_coop_full(const micm_hip_mechanism* m, int stages, int stage_request, int threads_request);
+  // Index into kLuBlockings of an LU blocking as micm_hip_state_set_lu_blocking takes it
+  // (0: the default, -1: whole sums), or -1 for a blocking that is not built.
+  int lu_blocking_index(int blocking)
+  {
+    const int want = blocking == 0 ? kLuBlockingDefault : blocking == -1 ? 0 : blocking;
+    if (want < 0)
+      return -1;
+    for (int b = 0; b < kNumLuBlockings; ++b)
+      if (kLuBlockings[b] == want)
+        return b;
+    return -1;
+  }
+
+  // The cooperative tables with the LU schedule of `blocking`.
+  DevCoop coop_with_blocking(const micm_hip_mechanism* m, int blocking)
+  {
+    DevCoop c = m->coop;
+    const int b = lu_blocking_index(blocking);
+    if (m->t.coop_levels_ok && b >= 0 && m->lu_blocked[b].offsets)
+      c.lu = m->lu_blocked[b];
+    return c;
+  }
+
+  CoopPlan plan_coop_full(const micm_hip_mechanism* m, int stages, int stage_request, int threads_request,
+                          int lu_blocking);
 
   // stage_request < 0: stage the substitution tables when they fit and the LU tables
   // when that costs no resident cell per CU, or take the compact layout when that fits
   // more cells per CU; 4: the compact layout; threads_request 0: one wavefront.
-  CoopPlan plan_coop(const micm_hip_mechanism* m, int stages, int stage_request = -1, int threads_request = 0)
+  CoopPlan plan_coop(const micm_hip_mechanism* m, int stages, int stage_request = -1, int threads_request = 0,
+                     int lu_blocking = 0)
   {
     // 4: the compact layout, its register budget chosen by occupancy; 5 / 6 / 7: the compact
     // layout built for three / two waves per SIMD / without a register bound (256 threads)
     const bool compact_request = stage_request >= 4;
-    CoopPlan plan = plan_coop_full(m, stages, compact_request ? -1 : stage_request, threads_request);
-    const DevCoop& c = m->coop;
+    CoopPlan plan = plan_coop_full(m, stages, compact_request ? -1 : stage_request, threads_request, lu_blocking);
+    const DevCoop c = coop_with_blocking(m, lu_blocking);
     if (!m->t.coop_levels_ok || (stage_request >= 0 && !compact_request))
       return plan;
     // the compact layout keeps one species per thread in registers and publishes the
@@ -2259,10 +2298,11 @@ namespace
     return plan;
   }
 
-  CoopPlan plan_coop_full(const micm_hip_mechanism* m, int stages, int stage_request, int threads_request)
+  CoopPlan plan_coop_full(const micm_hip_mechanism* m, int stages, int stage_request, int threads_request,
+                          int lu_blocking)
   {
     CoopPlan plan;
-    const DevCoop& c = m->coop;
+    const DevCoop c = coop_with_blocking(m, lu_blocking);
     if (!m->t.coop_levels_ok)
       return plan;
     constexpr size_t kLdsPerCu = 160 * 1024;
@@ -2341,7 +2381,8 @@ namespace
       blocks = std::max<int64_t>(1, std::min(blocks, n_fixed));
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, next);
     hipLaunchKernelGGL((k_coop<NT, STAGE, COMPACT, MINW, RECORD>), dim3(static_cast<unsigned>(blocks)), dim3(NT),
-                       plan.lds, st, m->dev, m->coop, p, plan.p_capacity, n_fixed, list, count, next, resume, s->y, s->k, s->atol,
+                       plan.lds, st, m->dev, coop_with_blocking(m, s->lu_blocking), p, plan.p_capacity, n_fixed, list, count,
+                       next, resume, s->y, s->k, s->atol,
                        s->status, s->stats, s->final_time, update_rates ? s->cond : nullptr,
                        update_rates ? s->custom : nullptr, k_rows, s->recording);
     HIP_TRY(hipGetLastError());
@@ -2384,7 +2425,7 @@ namespace
   int launch_coop(micm_hip_state* s, const DevParams& p, const int32_t* list, const int* count, int64_t n_fixed,
                   const CellControl* resume, bool update_rates, hipStream_t st)
   {
-    const CoopPlan plan = plan_coop(s->mech, p.stages, s->coop_stage, s->coop_threads);
+    const CoopPlan plan = plan_coop(s->mech, p.stages, s->coop_stage, s->coop_threads, s->lu_blocking);
     if (plan.lds == 0)
       return fail(MICM_HIP_ERR_UNSUPPORTED, "the cooperative form does not fit this mechanism in LDS");
     switch (plan.threads)
@@ -2417,7 +2458,7 @@ namespace
   {
     if (form == MICM_HIP_SOLVE_COOPERATIVE || form == MICM_HIP_SOLVE_PERSISTENT)
     {
-      const CoopPlan plan = plan_coop(s->mech, stages, s->coop_stage, s->coop_threads);
+      const CoopPlan plan = plan_coop(s->mech, stages, s->coop_stage, s->coop_threads, s->lu_blocking);
       if (plan.compact)
       {
         int64_t rows = 0;
@@ -3079,6 +3120,16 @@ extern "C"
     return MICM_HIP_OK;
   }
 
+  int micm_hip_state_set_lu_blocking(micm_hip_state* s, int32_t min_segment)
+  {
+    if (!s)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "null state");
+    if (min_segment != 0 && lu_blocking_index(min_segment) < 0)
+      return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "LU blocking must be 0, -1, 1, 2, 4 or 8");
+    s->lu_blocking = min_segment;
+    return MICM_HIP_OK;
+  }
+
   int micm_hip_mechanism_cooperative_supported(const micm_hip_mechanism* m, int32_t stages, int32_t* out)
   {
     if (int rc = check_mech(m))
@@ -3240,7 +3291,7 @@ extern "C"
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "flags: only MICM_HIP_SOLVE_UPDATE_RATES");
     if (params->stages < 1 || params->stages > 6)
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "stages must be 1..6");
-    const CoopPlan plan = plan_coop(m, static_cast<int>(params->stages), s->coop_stage, s->coop_threads);
+    const CoopPlan plan = plan_coop(m, static_cast<int>(params->stages), s->coop_stage, s->coop_threads, s->lu_blocking);
     if (plan.lds == 0)
       return fail(MICM_HIP_ERR_UNSUPPORTED, "the cooperative form does not fit this mechanism in LDS");
     int64_t rows = 0;
@@ -3370,7 +3421,7 @@ extern "C"
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "flags: only MICM_HIP_SOLVE_UPDATE_RATES");
     if (params->stages < 1 || params->stages > 6)
       return fail(MICM_HIP_ERR_INVALID_ARGUMENT, "stages must be 1..6");
-    if (plan_coop(m, static_cast<int>(params->stages), s->coop_stage, s->coop_threads).lds == 0)
+    if (plan_coop(m, static_cast<int>(params->stages), s->coop_stage, s->coop_threads, s->lu_blocking).lds == 0)
       return fail(MICM_HIP_ERR_UNSUPPORTED, "recorded solves run the cooperative form, which does not fit this mechanism");
     const DevParams p = make_params(*params, time_step, s->rtol);
     if (int rc = prepare_solve(s, MICM_HIP_SOLVE_COOPERATIVE, p.stages))
@@ -3497,6 +3548,7 @@ extern "C"
       shapes.push_back(s->handoff_steps);
       shapes.push_back(s->coop_threads);
       shapes.push_back(s->coop_stage);
+      shapes.push_back(s->lu_blocking);
       shapes.push_back(static_cast<int64_t>(s->n_slots));
       shapes.push_back(rtol_bits);
       shapes.push_back(static_cast<int64_t>(reinterpret_cast<intptr_t>(s->coop_k)));

@@ -84,6 +84,31 @@ extern "C"
 
 extern "C"
 {
+  // The factorisation of harness_factor_solve by the LU schedule of blocking `b`
+  // (index into kLuBlockings); returns the number of levels, or -1 for a bad index.
+  int harness_factor_blocked(const micm_hip_mechanism_desc* desc, int32_t b, const double* jac, double alpha,
+                             double* lower, double* upper)
+  {
+    const MechanismTables t = micm_hip::compile_mechanism(*desc);
+    if (b < 0 || static_cast<size_t>(b) >= t.lv_lu_blocked.size())
+      return -1;
+    const int32_t n = static_cast<int32_t>(t.n_species);
+    std::vector<double> alu(t.nnz_alu(), 0.0), none;
+    for (size_t e = 0; e < t.jac_to_alu.size(); ++e)
+      alu[t.jac_to_alu[e]] = jac[e];
+    for (int32_t i = 0; i < n; ++i)
+      alu[t.alu_diag[i]] += alpha;
+    run(t.lv_lu_blocked[b], nullptr, alu, none, true, true);
+    for (size_t p = 0; p < t.alu_lower.size(); ++p)
+      lower[p] = t.alu_lower[p] >= 0 ? alu[t.alu_lower[p]] : 1.0;
+    for (size_t p = 0; p < t.alu_upper.size(); ++p)
+      upper[p] = alu[t.alu_upper[p]];
+    return t.lv_lu_blocked[b].n_levels();
+  }
+}
+
+extern "C"
+{
   // Per level of schedule `which` (0 LU, 1 forward, 2 backward): tasks and the
   // largest number of terms of one task; returns the number of levels written.
   int harness_level_stats(const micm_hip_mechanism_desc* desc, int32_t which, int64_t* tasks, int64_t* max_terms,

@@ -36,10 +36,10 @@ def _run(monkeypatch, model, n_cells, tag, supported=True):
     """Runs autotune with seconds = model[label](cells); returns (choice, calls)."""
     calls = []
 
-    def fake(mech, y, k, dt, params, candidates, repeats, verbose, give_up=None):
+    def fake(mech, y, k, dt, params, candidates, repeats, verbose, give_up=None, lu_blocking=0):
         out = {}
         for c in candidates:
-            calls.append((c[0], y.shape[0]))
+            calls.append((c[0], y.shape[0]) if not lu_blocking else (f"{c[0]} lu{lu_blocking}", y.shape[0]))
             out[c[0]] = model[c[0]](y.shape[0])
         return out
 
@@ -125,7 +125,7 @@ def test_hand_off_goes_to_the_fastest_cooperative_shape(monkeypatch):
     model["cooperative256-compact-w2"] = _linear(0.3, 8.5e-5)
     model["persistent+handoff16"] = lambda n: 1.4 if n <= 2048 else 0.2 + 8.2e-5 * n
 
-    def fake(mech, y, k, dt, params, candidates, repeats, verbose, give_up=None):
+    def fake(mech, y, k, dt, params, candidates, repeats, verbose, give_up=None, lu_blocking=0):
         out = {}
         for c in candidates:
             seen.setdefault(c[0], []).append((y.shape[0], c[3], c[4]))
@@ -138,3 +138,28 @@ def test_hand_off_goes_to_the_fastest_cooperative_shape(monkeypatch):
     assert (65536, w2[3], w2[4]) in seen["persistent+handoff16"]
     assert choice["label"] == "persistent+handoff16" and choice["handoff_to"] == "cooperative256-compact-w2"
     assert (choice["threads"], choice["stage"]) == (w2[3], w2[4])
+
+
+def test_lu_blocking_is_chosen_for_the_fastest_cooperative_shape(monkeypatch):
+    """The LU schedule's blocking is timed on the first sample with the fastest cooperative
+    shape of the last sample, confirmed on the last sample when 2 % faster than the
+    default, and carried into the hand-off timings and the choice."""
+    calls = []
+
+    def fake(mech, y, k, dt, params, candidates, repeats, verbose, give_up=None, lu_blocking=0):
+        out = {}
+        for c in candidates:
+            calls.append((c[0], y.shape[0], lu_blocking))
+            t = 0.5 + 1e-4 * y.shape[0]
+            if c[0] == "cooperative256":
+                t = 0.3 + 8.0e-5 * y.shape[0] * {0: 1.0, 1: 1.05, 2: 0.99, 8: 0.9, -1: 1.3}[lu_blocking]
+            out[c[0]] = t
+        return out
+
+    monkeypatch.setattr(autotune, "_time_candidates", fake)
+    choice = autotune.autotune(_Mech(107), _State(100000), 30.0, types.SimpleNamespace(stages=4))
+    assert choice["lu_blocking"] == 8 and choice["label"] == "cooperative256"
+    assert {b for (label, n, b) in calls if label == "cooperative256" and n == 2048} == {0, 1, 2, 8, -1}
+    assert ("cooperative256", 65536, 8) in calls
+    assert all(b == 8 for (label, n, b) in calls if n == 65536 and label.startswith("persistent"))
+    assert set(choice["lu_blocking_seconds"]) == {1, 2, 8, -1}

@@ -132,13 +132,14 @@ def test_kernels_match_oracle(case, n_cells):
 
 
 def _solve(mech, cells, order, tableau, dt, form, n_solves=1, rate_constants=None, handoff=-1, coop_shape=(0, -1),
-           lockstep_order=1):
+           lockstep_order=1, lu_blocking=0):
     """n_solves x (UpdateStateParameters, Solve); with rate_constants given, those are
     loaded instead of being computed on the device."""
     state = _lib.DeviceState(mech, cells.n_cells)
     state.set_lockstep_order(lockstep_order)
     state.set_handoff(handoff)
     state.set_cooperative_shape(*coop_shape)
+    state.set_lu_blocking(lu_blocking)
     state.set_variables(cells.variables[:, order])
     state.set_conditions(cells.conditions)
     state.set_custom_parameters(cells.custom)
@@ -214,7 +215,7 @@ def test_per_cell_forms_agree_bitwise(name):
     """Every per-cell form runs the same per-cell arithmetic (the cooperative form
     with its work spread over a workgroup without reordering any sum), so plain,
     persistent (with and without hand-off) and cooperative results (every launch
-    shape and table staging) are identical."""
+    shape, table staging and LU schedule blocking) are identical."""
     desc, inputs, tableau, dt = CASES[name]
     mech = _lib.Mechanism(desc)
     cells = inputs(desc, 700 if name == "60_200" else 130, seed=21)
@@ -229,6 +230,12 @@ def test_per_cell_forms_agree_bitwise(name):
         for threads, stage in ((64, -1), (128, 0), (256, 3), (64, 1), (512, -1), (1024, 2), (128, 4), (256, 4),
                               (256, 5), (256, 6), (256, 7)):
             runs.append(_solve(mech, cells, order, tableau, dt, _lib.SOLVE_COOPERATIVE, coop_shape=(threads, stage)))
+        # every LU blocking, with the LU tables in global memory (compact) and staged in LDS
+        for blocking in (1, 2, 8, -1):
+            runs.append(_solve(mech, cells, order, tableau, dt, _lib.SOLVE_COOPERATIVE, coop_shape=(256, 6),
+                               lu_blocking=blocking))
+            runs.append(_solve(mech, cells, order, tableau, dt, _lib.SOLVE_COOPERATIVE, coop_shape=(128, 2),
+                               lu_blocking=blocking))
     for r in runs:
         assert bits_equal(r[0], ref[0])
         assert np.array_equal(r[1], ref[1]) and np.array_equal(r[2], ref[2])
@@ -280,6 +287,9 @@ def test_cooperative_shape_arguments_checked():
     for threads, stage in ((96, -1), (2048, -1), (256, 8), (256, -2)):
         with pytest.raises(_lib.MicmHipError):
             state.set_cooperative_shape(threads, stage)
+    for blocking in (3, 16, -2):
+        with pytest.raises(_lib.MicmHipError):
+            state.set_lu_blocking(blocking)
     ref = _solve(mech, cells, order, tableau, dt, _lib.SOLVE_PLAIN)
     run = _solve(mech, cells, order, tableau, dt, _lib.SOLVE_COOPERATIVE, coop_shape=(128, 4))
     assert bits_equal(run[0], ref[0]) and np.array_equal(run[2], ref[2])

@@ -29,6 +29,7 @@ def harness(tmp_path_factory):
     lib = ctypes.CDLL(out)
     lib.harness_sizes.argtypes = [ctypes.c_void_p, _ip]
     lib.harness_factor_solve.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp, _dp, _dp]
+    lib.harness_factor_blocked.argtypes = [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.c_double, _dp, _dp]
     return lib
 
 
@@ -74,3 +75,18 @@ def test_level_schedules_reproduce_doolittle(harness, name):
                                      lo.ctypes.data_as(_dp), up.ctypes.data_as(_dp), x.ctypes.data_as(_dp))
         assert bits_equal(lo, lo_ref[0]) and bits_equal(up, up_ref[0])
         assert bits_equal(x, x_ref[0])
+        # every LU blocking (shortest segment 1, 2, 4, 8, whole sums) gives the same factors;
+        # the blocking changes how a sum is split over the levels, not the level at which
+        # the element is final, so every blocking has as many levels
+        levels = []
+        for b in range(5):
+            lo_b, up_b = np.zeros(nnz_l), np.zeros(nnz_u)
+            nl = harness.harness_factor_blocked(ctypes.byref(cdesc), b,
+                                                np.ascontiguousarray(jac[c]).ctypes.data_as(_dp), alpha,
+                                                lo_b.ctypes.data_as(_dp), up_b.ctypes.data_as(_dp))
+            assert nl > 0
+            levels.append(nl)
+            assert bits_equal(lo_b, lo_ref[0]) and bits_equal(up_b, up_ref[0])
+        assert len(set(levels)) == 1
+        assert harness.harness_factor_blocked(ctypes.byref(cdesc), 5, np.ascontiguousarray(jac[c]).ctypes.data_as(_dp),
+                                              alpha, lo.ctypes.data_as(_dp), up.ctypes.data_as(_dp)) == -1
