@@ -133,7 +133,7 @@ def readme_mechanism():
     return MechanismDesc(3, reactions, species_names=["Foo", "Bar", "Baz"])
 
 
-def random_mechanism(n_species, n_reactions, mix, seed):
+def random_mechanism(n_species, n_reactions, mix, seed, products_from="above"):
     """Seeded random mechanism.
 
     mix: dict rate_type -> exact count (sums to n_reactions).  Every reaction
@@ -145,6 +145,8 @@ def random_mechanism(n_species, n_reactions, mix, seed):
     products drawn from all species, 60-80 % of cells of the 60- and
     210-species mechanisms end in NaN or exceed the step limit in the reference
     solver, because yields up to 2 make most reaction cycles net producers.)
+    products_from="all" draws them from every species instead: the configurations'
+    text taken literally.
 
       Arrhenius: A log-uniform in 1e-20..1e-16 m3 molecule-1 s-1 times
                  N_A^(n_reactants-1); C uniform in [-2000, 500] K; B=0, D=300, E=0.
@@ -167,7 +169,7 @@ def random_mechanism(n_species, n_reactions, mix, seed):
     for t in types:
         n_react = 1 if t == USER_DEFINED else 1 + rng.integer(2)
         reactants = [rng.integer(n_species - 1) for _ in range(n_react)]
-        lowest = max(reactants) + 1
+        lowest = max(reactants) + 1 if products_from == "above" else 0
         n_prod = 1 + rng.integer(3)
         products = [(lowest + rng.integer(n_species - lowest), rng.uniform(0.1, 2.0)) for _ in range(n_prod)]
         if t == ARRHENIUS:
@@ -199,9 +201,10 @@ def mechanism_60_200(seed=6020):
     return random_mechanism(60, 200, {ARRHENIUS: 160, TROE: 20, USER_DEFINED: 20}, seed)
 
 
-def mechanism_210_560(seed=21056):
+def mechanism_210_560(seed=21056, products_from="above"):
     """210 species / 560 reactions: 75 % Arrhenius, 10 % Troe, 5 % ternary, 10 % photolysis."""
-    return random_mechanism(210, 560, {ARRHENIUS: 420, TROE: 56, TERNARY: 28, USER_DEFINED: 56}, seed)
+    return random_mechanism(210, 560, {ARRHENIUS: 420, TROE: 56, TERNARY: 28, USER_DEFINED: 56}, seed,
+                            products_from)
 
 
 # ------------------------------------------------------------------ cell inputs
