@@ -94,9 +94,11 @@ def algorithmic_flops(per, stats):
                s[2] * per["error"])
 
 
-def cpu_baseline(desc, cells, order, dt, sample, threads, tableau=4, results=None):
-    """Cell-solves/s of the reference CPU solver (per-cell States) on `sample` cells.
-    With a dict `results`, its y, states and stats for those cells are stored there."""
+def cpu_baseline(desc, cells, order, dt, sample, threads, tableau=4, results=None, min_seconds=0.5):
+    """Cell-solves/s of the reference CPU solver (per-cell States) on `sample` cells,
+    the whole sample solved again until `min_seconds` have passed (a small sample is
+    otherwise dominated by starting the threads).  With a dict `results`, its y, states
+    and stats for those cells are stored there.  Returns (kind, rate, seconds, repeats)."""
     from oracle import Oracle, Reference, reference_available
     kind = "reference" if reference_available() else "port"
     idx = np.arange(sample)
@@ -105,12 +107,17 @@ def cpu_baseline(desc, cells, order, dt, sample, threads, tableau=4, results=Non
     if kind == "port":
         solver = Oracle(desc, tableau)
         solver.lib.oracle_set_threads(int(threads))
+        reps = 0
         t0 = time.perf_counter()
-        out = solver.solve(sub.conditions, sub.custom, y, dt, 1)
-        el = time.perf_counter() - t0
+        while True:
+            out = solver.solve(sub.conditions, sub.custom, y, dt, 1)
+            reps += 1
+            el = time.perf_counter() - t0
+            if el >= min_seconds or reps >= 10000:
+                break
         if results is not None:
             results.update(y=out[0], states=out[1], stats=out[2])
-        return kind, sample / el, el
+        return kind, sample * reps / el, el, reps
     solvers = [Reference(desc, tableau) for _ in range(threads)]
     chunks = np.array_split(idx, threads)
     outs = [None] * threads
@@ -120,18 +127,23 @@ def cpu_baseline(desc, cells, order, dt, sample, threads, tableau=4, results=Non
         if len(c):
             outs[k] = solvers[k].solve(sub.conditions[c], sub.custom[c], y[c], dt, 1)
 
+    reps = 0
     t0 = time.perf_counter()
-    ts = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
-    for t in ts:
-        t.start()
-    for t in ts:
-        t.join()
-    el = time.perf_counter() - t0
+    while True:
+        ts = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= min_seconds or reps >= 10000:
+            break
     if results is not None:
         done = [o for o in outs if o is not None]
         results.update(y=np.concatenate([o[0] for o in done]), states=np.concatenate([o[1] for o in done]),
                        stats=np.concatenate([o[2] for o in done]))
-    return kind, sample / el, el
+    return kind, sample * reps / el, el, reps
 
 
 def parity(y_gpu, st_gpu, stats_gpu, ref):
@@ -578,19 +590,19 @@ def main():
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
         # one core first; its rate sizes the all-core sample when none is given
         sample1 = min(n_cells, 16 if args.cpu_sample < 0 else max(8, args.cpu_sample // threads))
-        kind1, cps1, el1 = cpu_baseline(desc, cells, order, dt, sample1, 1, tableau)
+        kind1, cps1, el1, reps1 = cpu_baseline(desc, cells, order, dt, sample1, 1, tableau)
         sample = min(n_cells, args.cpu_sample if args.cpu_sample > 0 else max(256, int(15.0 * cps1)))
         ref = {}
-        kind, cps, el = cpu_baseline(desc, cells, order, dt, sample, threads, tableau, results=ref)
-        log(f"CPU baseline: {sample} cells on {threads} cores in {el:.1f} s")
+        kind, cps, el, reps = cpu_baseline(desc, cells, order, dt, sample, threads, tableau, results=ref)
+        log(f"CPU baseline: {sample} cells x {reps} on {threads} cores in {el:.1f} s")
         # the timed Solve's own results on those cells against the reference's
         result["parity_sample"] = parity(y_out, status, stats, ref)
         result["cpu_baseline"] = {"value": round(cps, 2), "unit": "cell-solves/s", "cores": threads, "kind": kind,
                                   "cpu": cpu_model(),
                                   "sample": f"first {sample} cells of the same workload, one 1-cell State per "
-                                            f"cell, {el:.1f} s wall on {threads} cores",
+                                            f"cell, solved {reps} time(s), {el:.1f} s wall on {threads} cores",
                                   "single_core_value": round(cps1, 2),
-                                  "single_core_sample": f"first {sample1} cells, {el1:.1f} s"}
+                                  "single_core_sample": f"first {sample1} cells, solved {reps1} time(s), {el1:.1f} s"}
         if args.workload == "readme" and lock_cells is not None:
             kind_l, cps_l, el_l = cpu_lockstep(desc, lock_cells, order, dt, tableau)
             result["cpu_baseline"]["lockstep"] = {
@@ -693,9 +705,9 @@ def main_packed(args):
         seconds = 0.0
         parts_cpu = []
         for q in parts:
-            kind1, cps1, _ = cpu_baseline(q["desc"], q["cells"], q["order"], dt, 16, 1, tableau)
+            kind1, cps1, _, _ = cpu_baseline(q["desc"], q["cells"], q["order"], dt, 16, 1, tableau)
             sample = min(q["state"].n_cells, max(256, int(5.0 * cps1)))
-            kind, cps, el = cpu_baseline(q["desc"], q["cells"], q["order"], dt, sample, threads, tableau)
+            kind, cps, el, _ = cpu_baseline(q["desc"], q["cells"], q["order"], dt, sample, threads, tableau)
             seconds += q["state"].n_cells / cps
             parts_cpu.append(f"{q['label']}: {sample} cells {el:.1f} s ({cps:.0f} cell-solves/s)")
         result["cpu_baseline"] = {"value": round(n_total / seconds, 2), "unit": "cell-solves/s", "cores": threads,
