@@ -532,6 +532,198 @@ namespace micm_hip
     }
   }
 
+  // One task from its record (LevelSchedule::records): out[o] = (out[o] - its products,
+  // subtracted in order) / A[diag].  The first kInlineTerms products come with the
+  // record; a longer task reads the rest from the term table.
+  __device__ __forceinline__ void run_record(const uint4& ra, const uint4& rb, const uint32_t* terms, const double* A,
+                                             const double* X, double* out)
+  {
+    const uint32_t o = ra.x & 0xffffu, d = ra.x >> 16, n = ra.z;
+    const uint32_t ts[kInlineTerms] = { ra.w, rb.x, rb.y, rb.z, rb.w };
+    double a[kInlineTerms], x[kInlineTerms];
+#pragma unroll
+    for (int r = 0; r < kInlineTerms; ++r)
+    {
+      a[r] = A[ts[r] & 0xffffu];
+      x[r] = X[ts[r] >> 16];
+    }
+    double v = out[o];
+    const double dv = d < kUnitDivisor ? A[d] : 1.0;
+#pragma unroll
+    for (int r = 0; r < kInlineTerms; ++r)
+      if (static_cast<uint32_t>(r) < n)
+        v -= a[r] * x[r];
+    if (n > static_cast<uint32_t>(kInlineTerms))
+      v = coop_dot(v, terms, ra.y + kInlineTerms, ra.y + n, A, X);
+    if (d < kUnitDivisor)
+      v = v / dv;
+    out[o] = v;
+  }
+
+  // A level schedule with its records in global memory, pipelined: each thread's first
+  // task of level l + 2 (one 32-byte record: header and first terms) is read while
+  // level l runs, into one of two record buffers that alternate between even and odd
+  // levels, so a level waits only for its operands in LDS.  A thread without a task at
+  // a level reads the sentinel record.  Level bounds are read four levels ahead.  Same
+  // operations in the same order as coop_levels_direct.
+  template<int NT>
+  __device__ inline void coop_levels_piped(const DevLevels& lv, const uint32_t* terms, const double* A,
+                                           const double* X, double* out)
+  {
+    const int tid = threadIdx.x;
+    const int nl = lv.n_levels;
+    if (nl == 0)
+      return;
+    const uint4* rec = lv.records;
+    auto bound = [&](int i) { return lv.offsets[i < nl ? i : nl]; };
+    auto first_record = [&](int l, int b, int e, uint4& ra, uint4& rb) {
+      const int t = (l < nl && b + tid < e) ? b + tid : lv.n_tasks;
+      ra = rec[2 * t];
+      rb = rec[2 * t + 1];
+    };
+    auto level = [&](int b, int e, const uint4& ra, const uint4& rb) {
+      if (b + tid < e)
+        run_record(ra, rb, terms, A, X, out);
+      for (int t = b + tid + NT; t < e; t += NT)
+      {
+        const uint4 a2 = rec[2 * t], b2 = rec[2 * t + 1];
+        run_record(a2, b2, terms, A, X, out);
+      }
+    };
+    int b0 = bound(0), b1 = bound(1), b2 = bound(2), b3 = bound(3);
+    uint4 pa, pb, qa, qb;  // records of the even and of the odd levels
+    first_record(0, b0, b1, pa, pb);
+    first_record(1, b1, b2, qa, qb);
+    for (int l = 0; l < nl; l += 2)
+    {
+      {
+        const int b4 = bound(l + 4);
+        level(b0, b1, pa, pb);
+        first_record(l + 2, b2, b3, pa, pb);
+        lds_barrier<NT>();
+        b0 = b1;
+        b1 = b2;
+        b2 = b3;
+        b3 = b4;
+      }
+      if (l + 1 >= nl)
+        break;
+      {
+        const int b4 = bound(l + 5);
+        level(b0, b1, qa, qb);
+        first_record(l + 3, b2, b3, qa, qb);
+        lds_barrier<NT>();
+        b0 = b1;
+        b1 = b2;
+        b2 = b3;
+        b3 = b4;
+      }
+    }
+  }
+
+  // The two-phase form of coop_levels with its records and terms in global memory,
+  // pipelined like coop_levels_piped: for level l + 2, each thread's first product term
+  // and first task record are read while level l runs, into one of two buffers that
+  // alternate between even and odd levels.  Needs every level's products to fit P.
+  template<int NT>
+  __device__ inline void coop_levels_piped2(const DevLevels& lv, const uint32_t* terms, const double* A,
+                                            const double* X, double* out, double* P)
+  {
+    const int tid = threadIdx.x;
+    const int nl = lv.n_levels;
+    if (nl == 0)
+      return;
+    const uint4* rec = lv.records;
+    auto bound = [&](int i) { return lv.offsets[i < nl ? i : nl]; };
+    // first term of level i (the end of the terms past the last level)
+    auto term_bound = [&](int i) { return static_cast<int>(lv.tasks[bound(i)].y); };
+    struct Ahead
+    {
+      uint4 ra, rb;  // first task's record
+      uint32_t pt;   // first product term
+    };
+    auto fetch = [&](int l, int b, int e, int qb, int qe, Ahead& h) {
+      const int t = (l < nl && b + tid < e) ? b + tid : lv.n_tasks;
+      h.ra = rec[2 * t];
+      h.rb = rec[2 * t + 1];
+      h.pt = terms[(l < nl && qb + tid < qe) ? qb + tid : 0];
+    };
+    auto level = [&](int b, int e, int qb, int qe, const Ahead& h) {
+      if (qb + tid < qe)
+        P[tid] = A[h.pt & 0xffffu] * X[h.pt >> 16];
+      for (int q = qb + tid + NT; q < qe; q += NT)
+      {
+        const uint32_t tr = terms[q];
+        P[q - qb] = A[tr & 0xffffu] * X[tr >> 16];
+      }
+      lds_barrier<NT>();
+      auto sum = [&](const uint4& ra) {
+        const uint32_t o = ra.x & 0xffffu, d = ra.x >> 16, n = ra.z;
+        double v = out[o];
+        const double dv = d < kUnitDivisor ? A[d] : 1.0;
+        const double* p = P + (static_cast<int>(ra.y) - qb);
+        uint32_t q = 0;
+        for (; q + 8 <= n; q += 8)
+        {
+          double s8[8];
+#pragma unroll
+          for (int r = 0; r < 8; ++r)
+            s8[r] = p[q + r];
+#pragma unroll
+          for (int r = 0; r < 8; ++r)
+            v -= s8[r];
+        }
+        for (; q < n; ++q)
+          v -= p[q];
+        if (d < kUnitDivisor)
+          v = v / dv;
+        out[o] = v;
+      };
+      if (b + tid < e)
+        sum(h.ra);
+      for (int t = b + tid + NT; t < e; t += NT)
+        sum(rec[2 * t]);
+    };
+    int b0 = bound(0), b1 = bound(1), b2 = bound(2), b3 = bound(3);
+    int q0 = term_bound(0), q1 = term_bound(1), q2 = term_bound(2), q3 = term_bound(3);
+    Ahead ev, od;  // the even and the odd levels' buffers
+    fetch(0, b0, b1, q0, q1, ev);
+    fetch(1, b1, b2, q1, q2, od);
+    for (int l = 0; l < nl; l += 2)
+    {
+      {
+        const int b4 = bound(l + 4), q4 = term_bound(l + 4);
+        level(b0, b1, q0, q1, ev);
+        fetch(l + 2, b2, b3, q2, q3, ev);
+        lds_barrier<NT>();
+        b0 = b1;
+        b1 = b2;
+        b2 = b3;
+        b3 = b4;
+        q0 = q1;
+        q1 = q2;
+        q2 = q3;
+        q3 = q4;
+      }
+      if (l + 1 >= nl)
+        break;
+      {
+        const int b4 = bound(l + 5), q4 = term_bound(l + 5);
+        level(b0, b1, q0, q1, od);
+        fetch(l + 3, b2, b3, q2, q3, od);
+        lds_barrier<NT>();
+        b0 = b1;
+        b1 = b2;
+        b2 = b3;
+        b3 = b4;
+        q0 = q1;
+        q1 = q2;
+        q2 = q3;
+        q3 = q4;
+      }
+    }
+  }
+
   // Runs a level schedule: task t computes out[o] = (out[o] - dot) / A[diag] (no
   // division when diag is 0xffff), reading A and X.
   //
@@ -700,15 +892,29 @@ namespace micm_hip
     for (int e = tid; e < m.nnz_jac; e += NT)
       L.alu[c.jac_to_alu[e]] = L.jac[e];
     coop_sync<NT>();
-    coop_levels_direct<NT, GT>(c.lu, T.lu_tasks, T.lu_terms, L.alu, L.alu, L.alu);
+    if constexpr (GT)
+      coop_levels_piped<NT>(c.lu, c.lu.terms, L.alu, L.alu, L.alu);
+    else
+      coop_levels_direct<NT, false>(c.lu, T.lu_tasks, T.lu_terms, L.alu, L.alu, L.alu);
   }
 
   // x <- U^-1 L^-1 x with the factors in L.alu (unit lower diagonal).
   template<int NT, bool GT = false>
   __device__ inline void coop_solve(const DevCoop& c, const CoopLds& L, const CoopTables& T, double* x)
   {
-    coop_levels_direct<NT, GT>(c.fwd, T.fwd_tasks, T.fwd_terms, L.alu, x, x);
-    coop_levels<NT>(c.bwd, T.bwd_tasks, T.bwd_terms, L.alu, x, x, L.products, L.p_capacity);
+    if constexpr (GT)
+    {
+      coop_levels_piped<NT>(c.fwd, c.fwd.terms, L.alu, x, x);
+      if (c.bwd.max_level_terms <= L.p_capacity)
+        coop_levels_piped2<NT>(c.bwd, c.bwd.terms, L.alu, x, x, L.products);
+      else
+        coop_levels<NT>(c.bwd, T.bwd_tasks, T.bwd_terms, L.alu, x, x, L.products, L.p_capacity);
+    }
+    else
+    {
+      coop_levels_direct<NT, false>(c.fwd, T.fwd_tasks, T.fwd_terms, L.alu, x, x);
+      coop_levels<NT>(c.bwd, T.bwd_tasks, T.bwd_terms, L.alu, x, x, L.products, L.p_capacity);
+    }
   }
 
   // 32-bit words of the tables a launch stages in LDS.

@@ -176,6 +176,23 @@ namespace micm_hip
       for (size_t l = 0; l + 1 < s.offsets.size(); ++l)
         s.max_level_terms = std::max<int32_t>(
             s.max_level_terms, static_cast<int32_t>(s.tasks[2 * s.offsets[l + 1] + 1] - s.tasks[2 * s.offsets[l] + 1]));
+      // records: the task's header and its first terms in one 32-byte read
+      s.records.clear();
+      for (size_t q = 0; q + 1 < s.tasks.size() / 2; ++q)
+      {
+        const uint32_t first = s.tasks[2 * q + 1], count = s.tasks[2 * q + 3] - first;
+        s.records.push_back(s.tasks[2 * q]);
+        s.records.push_back(first);
+        s.records.push_back(count);
+        for (uint32_t r = 0; r < static_cast<uint32_t>(kInlineTerms); ++r)
+          s.records.push_back(r < count ? s.terms[first + r] : 0u);
+      }
+      // the sentinel: no output, no terms
+      s.records.push_back(0u | (kNoDivisor << 16));
+      s.records.push_back(static_cast<uint32_t>(s.terms.size()));
+      s.records.push_back(0u);
+      for (int r = 0; r < kInlineTerms; ++r)
+        s.records.push_back(0u);
       // sixteen entries naming position 0 let the batches of eight read past a task's last term
       s.terms.insert(s.terms.end(), 16, 0u);
       return s;

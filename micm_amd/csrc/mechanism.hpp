@@ -24,12 +24,18 @@ namespace micm_hip
   constexpr int kNumLuBlockings = 5;
   constexpr int kLuBlockingDefault = 4;
 
+  // Terms carried in a task's record (LevelSchedule::records).
+  constexpr int kInlineTerms = 5;
+
   // Independent tasks grouped by dependency level (see MechanismTables::lv_lu).
   struct LevelSchedule
   {
     std::vector<int32_t> offsets;  // level -> first task, n_levels + 1
     std::vector<uint32_t> tasks;   // 2 per task + sentinel: out | diag << 16, first term
     std::vector<uint32_t> terms;   // two 16-bit fields, then sixteen zero entries of padding
+    // 8 per task + sentinel: out | diag << 16, first term, number of terms, then the task's
+    // first kInlineTerms terms (0 past its last), so that one 32-byte read brings a task
+    std::vector<uint32_t> records;
     int32_t max_terms = 0;
     int32_t max_level_terms = 0;  // largest number of terms of one level
     int32_t n_levels() const { return offsets.empty() ? 0 : static_cast<int32_t>(offsets.size()) - 1; }

@@ -1527,7 +1527,7 @@ namespace
             L.alu[d] = jdiag_r;
           }
           coop_sync<NT>();
-          coop_levels_direct<NT, true>(c.lu, T.lu_tasks, T.lu_terms, L.alu, L.alu, L.alu);
+          coop_levels_piped<NT>(c.lu, c.lu.terms, L.alu, L.alu, L.alu);
         }
         else
         {
@@ -1797,6 +1797,9 @@ namespace
     upload(blob, off, l.tasks, tasks, staging);
     d.tasks = reinterpret_cast<const uint2*>(tasks);
     upload(blob, off, l.terms, d.terms, staging);
+    const uint32_t* records = nullptr;
+    upload(blob, off, l.records, records, staging);
+    d.records = reinterpret_cast<const uint4*>(records);
     d.n_levels = l.n_levels();
     d.n_tasks = l.tasks.empty() ? 0 : l.n_tasks();
     d.n_terms = static_cast<int32_t>(l.terms.size());
@@ -1812,7 +1815,7 @@ namespace
 
   size_t level_bytes(const LevelSchedule& l)
   {
-    return padded_bytes(l.offsets) + padded_bytes(l.tasks) + padded_bytes(l.terms);
+    return padded_bytes(l.offsets) + padded_bytes(l.tasks) + padded_bytes(l.terms) + padded_bytes(l.records);
   }
 
   size_t blocked_level_bytes(const MechanismTables& t)
