@@ -54,6 +54,10 @@ namespace micm_hip
     int p_capacity;    // doubles in products
   };
 
+  // Doubles past the product buffer's capacity that the two-phase back substitution may
+  // read (one batch of eight ahead of a task's last product, never used).
+  constexpr int kProductSlack = 16;
+
   __host__ __device__ inline int coop_scratch_doubles(int n, int n_reactions, int nnz_alu, int n_terms)
   {
     int scratch = n_reactions > n ? n_reactions : n;
@@ -68,8 +72,8 @@ namespace micm_hip
   __host__ __device__ inline size_t coop_lds_doubles(int nnz_alu, int nnz_jac, int n, int stages, int n_reactions,
                                                       int n_terms, int p_capacity)
   {
-    return static_cast<size_t>(nnz_alu) + nnz_jac + static_cast<size_t>(4 + stages) * n + n_reactions +
-           coop_scratch_doubles(n, n_reactions, nnz_alu, n_terms) + p_capacity;
+    return static_cast<size_t>(nnz_alu) + 1 + nnz_jac + static_cast<size_t>(4 + stages) * n + n_reactions +
+           coop_scratch_doubles(n, n_reactions, nnz_alu, n_terms) + p_capacity + kProductSlack;
   }
 
   __device__ inline CoopLds coop_layout(double* lds, const DevMech& m, const DevCoop& c, int stages, int p_capacity)
@@ -77,7 +81,7 @@ namespace micm_hip
     CoopLds L;
     const int n = m.n_species;
     L.alu = lds;
-    L.jac = L.alu + c.nnz_alu;
+    L.jac = L.alu + c.nnz_alu + 1;  // alu[nnz_alu] = 0 (see coop_decompose)
     L.y = L.jac + m.nnz_jac;
     L.ynew = L.y + n;
     L.f0 = L.ynew + n;
@@ -104,8 +108,8 @@ namespace micm_hip
                                                                  int p_capacity)
   {
     int scratch = n_reactions > n ? n_reactions : n;
-    if (p_capacity > scratch)
-      scratch = p_capacity;
+    if (p_capacity + kProductSlack > scratch)
+      scratch = p_capacity + kProductSlack;
     if (n_terms > stages * n && n_terms > scratch)
       scratch = n_terms;
     return static_cast<size_t>(scratch);
@@ -113,7 +117,7 @@ namespace micm_hip
   __host__ __device__ inline size_t coop_compact_lds_doubles(int nnz_alu, int n, int stages, int n_reactions,
                                                              int n_terms, int p_capacity)
   {
-    return static_cast<size_t>(nnz_alu) + static_cast<size_t>(1 + stages) * n +
+    return static_cast<size_t>(nnz_alu) + 1 + static_cast<size_t>(1 + stages) * n +
            coop_compact_scratch_doubles(n, n_reactions, stages, n_terms, p_capacity);
   }
   __device__ inline CoopLds coop_layout_compact(double* lds, const DevMech& m, const DevCoop& c, int stages,
@@ -122,7 +126,7 @@ namespace micm_hip
     CoopLds L;
     const int n = m.n_species;
     L.alu = lds;
-    L.y = L.alu + c.nnz_alu;
+    L.y = L.alu + c.nnz_alu + 1;  // alu[nnz_alu] = 0
     L.K = L.y + n;
     L.scratch = L.K + static_cast<size_t>(stages) * n;
     L.jac = nullptr;
@@ -532,41 +536,143 @@ namespace micm_hip
     }
   }
 
-  // One task from its record (LevelSchedule::records): out[o] = (out[o] - its products,
-  // subtracted in order) / A[diag].  The first kInlineTerms products come with the
-  // record; a longer task reads the rest from the term table.
-  __device__ __forceinline__ void run_record(const uint4& ra, const uint4& rb, const uint32_t* terms, const double* A,
+  // A table that no thread writes during the kernel, read through the constant address
+  // space: a wave-uniform index then becomes a scalar load (waited for with the LDS
+  // counter at the next barrier, not with the vector-memory counter that the prefetched
+  // records use).
+  template<class T>
+  __device__ __forceinline__ T const_load(const T* p, int i)
+  {
+    using CT = const __attribute__((address_space(4))) T;
+    return ((CT*)(p))[i];
+  }
+
+  // A task's record (LevelSchedule::records): header and first kInlineTerms terms.
+  struct TaskRecord
+  {
+    uint4 h, t1, t2, t3;  // h: out | diag << 16, first term, count, term 0; t1..t3: terms 1..12
+  };
+  static_assert(kRecordWords == 16, "a record is four uint4");
+
+  // Makes every lane wait here for the reads that fill r, whether or not it then uses r:
+  // a read left pending on a path that skips its use would make the wait before the next
+  // write of r's registers (a whole level later) wait for every read issued since.
+  __device__ __forceinline__ void settle(const uint4& r)
+  {
+    asm volatile("" ::"v"(r.x), "v"(r.y), "v"(r.z), "v"(r.w));
+  }
+
+  __device__ __forceinline__ TaskRecord load_record(const uint4* rec, int t)
+  {
+    TaskRecord r;
+    r.h = rec[4 * t];
+    r.t1 = rec[4 * t + 1];
+    r.t2 = rec[4 * t + 2];
+    r.t3 = rec[4 * t + 3];
+    return r;
+  }
+
+  // One task from its record: out[o] = (out[o] - its products, subtracted in order) /
+  // A[diag].  The first kInlineTerms products come with the record (five, then eight
+  // more if the task has them); a longer task reads the rest from the term table.
+  // PADDED: the record's slots past the task's last term name a zero in both A and X
+  // (the LU's records), so every inline slot subtracts without a test (0 * 0 changes
+  // nothing); otherwise each slot is tested, and slots 1 to 4 are skipped as a block
+  // by a wavefront whose tasks have one term at most.
+  template<bool PADDED>
+  __device__ __forceinline__ void run_record(const TaskRecord& r, const uint32_t* terms, const double* A,
                                              const double* X, double* out)
   {
-    const uint32_t o = ra.x & 0xffffu, d = ra.x >> 16, n = ra.z;
-    const uint32_t ts[kInlineTerms] = { ra.w, rb.x, rb.y, rb.z, rb.w };
-    double a[kInlineTerms], x[kInlineTerms];
-#pragma unroll
-    for (int r = 0; r < kInlineTerms; ++r)
-    {
-      a[r] = A[ts[r] & 0xffffu];
-      x[r] = X[ts[r] >> 16];
-    }
+    const uint32_t o = r.h.x & 0xffffu, d = r.h.x >> 16, n = r.h.z;
     double v = out[o];
     const double dv = d < kUnitDivisor ? A[d] : 1.0;
+    if constexpr (PADDED)
+    {
+      const uint32_t ts[5] = { r.h.w, r.t1.x, r.t1.y, r.t1.z, r.t1.w };
+      double a[5], x[5];
 #pragma unroll
-    for (int r = 0; r < kInlineTerms; ++r)
-      if (static_cast<uint32_t>(r) < n)
-        v -= a[r] * x[r];
-    if (n > static_cast<uint32_t>(kInlineTerms))
-      v = coop_dot(v, terms, ra.y + kInlineTerms, ra.y + n, A, X);
+      for (int q = 0; q < 5; ++q)
+      {
+        a[q] = A[ts[q] & 0xffffu];
+        x[q] = X[ts[q] >> 16];
+      }
+#pragma unroll
+      for (int q = 0; q < 5; ++q)
+        v -= a[q] * x[q];
+    }
+    else
+    {
+      {
+        const double a = A[r.h.w & 0xffffu], x = X[r.h.w >> 16];
+        if (n > 0u)
+          v -= a * x;
+      }
+      if (n > 1u)
+      {
+        const uint32_t ts[4] = { r.t1.x, r.t1.y, r.t1.z, r.t1.w };
+        double a[4], x[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+          a[q] = A[ts[q] & 0xffffu];
+          x[q] = X[ts[q] >> 16];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (static_cast<uint32_t>(q) + 1u < n)
+            v -= a[q] * x[q];
+      }
+    }
+    if (n > 5u)
+    {
+      const uint32_t ts[8] = { r.t2.x, r.t2.y, r.t2.z, r.t2.w, r.t3.x, r.t3.y, r.t3.z, r.t3.w };
+      double a[8], x[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+      {
+        a[q] = A[ts[q] & 0xffffu];
+        x[q] = X[ts[q] >> 16];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (PADDED || static_cast<uint32_t>(q) + 5u < n)
+          v -= a[q] * x[q];
+    }
+    // the rest in batches of eight, each batch's reads consumed before the next batch's
+    // (reads past the task's last term stay inside the term table's padding), so that
+    // no read is left in flight when the task ends
+    for (uint32_t q = r.h.y + kInlineTerms, e = r.h.y + n; q < e; q += 8)
+    {
+      uint32_t ts[8];
+      double a[8], x[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        ts[k] = terms[q + k];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+      {
+        a[k] = A[ts[k] & 0xffffu];
+        x[k] = X[ts[k] >> 16];
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (q + k < e)
+          v -= a[k] * x[k];
+    }
     if (d < kUnitDivisor)
       v = v / dv;
     out[o] = v;
   }
 
-  // A level schedule with its records in global memory, pipelined: each thread's first
-  // task of level l + 2 (one 32-byte record: header and first terms) is read while
-  // level l runs, into one of two record buffers that alternate between even and odd
-  // levels, so a level waits only for its operands in LDS.  A thread without a task at
-  // a level reads the sentinel record.  Level bounds are read four levels ahead.  Same
-  // operations in the same order as coop_levels_direct.
-  template<int NT>
+  // A level schedule with its records in global memory, pipelined: at the start of level
+  // l each thread reads the record of its first task at level l + 2, then runs level l
+  // from the record read two levels earlier, so every read has a whole level to arrive.
+  // Three buffers rotate over levels l, l + 1 and l + 2, unrolled by three so that each
+  // keeps its registers (a rotation by register moves would wait for the read just
+  // issued).  A thread without a task at a level reads the sentinel record.  Level
+  // bounds are scalar reads (constant address space).  Same operations in the same order
+  // as coop_levels_direct.
+  template<int NT, bool PADDED>
   __device__ inline void coop_levels_piped(const DevLevels& lv, const uint32_t* terms, const double* A,
                                            const double* X, double* out)
   {
@@ -575,56 +681,44 @@ namespace micm_hip
     if (nl == 0)
       return;
     const uint4* rec = lv.records;
-    auto bound = [&](int i) { return lv.offsets[i < nl ? i : nl]; };
-    auto first_record = [&](int l, int b, int e, uint4& ra, uint4& rb) {
-      const int t = (l < nl && b + tid < e) ? b + tid : lv.n_tasks;
-      ra = rec[2 * t];
-      rb = rec[2 * t + 1];
-    };
-    auto level = [&](int b, int e, const uint4& ra, const uint4& rb) {
-      if (b + tid < e)
-        run_record(ra, rb, terms, A, X, out);
-      for (int t = b + tid + NT; t < e; t += NT)
-      {
-        const uint4 a2 = rec[2 * t], b2 = rec[2 * t + 1];
-        run_record(a2, b2, terms, A, X, out);
-      }
+    auto bound = [&](int i) { return const_load(lv.offsets, i < nl ? i : nl); };
+    auto first_record = [&](int l, int b, int e) {
+      return load_record(rec, (l < nl && b + tid < e) ? b + tid : lv.n_tasks);
     };
     int b0 = bound(0), b1 = bound(1), b2 = bound(2), b3 = bound(3);
-    uint4 pa, pb, qa, qb;  // records of the even and of the odd levels
-    first_record(0, b0, b1, pa, pb);
-    first_record(1, b1, b2, qa, qb);
-    for (int l = 0; l < nl; l += 2)
+    TaskRecord r0 = first_record(0, b0, b1), r1 = first_record(1, b1, b2), r2;
+    auto step = [&](int l, const TaskRecord& cur, TaskRecord& ahead) {
+      const int b4 = bound(l + 4);
+      ahead = first_record(l + 2, b2, b3);
+      settle(cur.h);
+      settle(cur.t1);
+      settle(cur.t2);
+      settle(cur.t3);
+      if (b0 + tid < b1)
+        run_record<PADDED>(cur, terms, A, X, out);
+      for (int t = b0 + tid + NT; t < b1; t += NT)
+        run_record<PADDED>(load_record(rec, t), terms, A, X, out);
+      if (l < nl)
+        lds_barrier<NT>();
+      b0 = b1;
+      b1 = b2;
+      b2 = b3;
+      b3 = b4;
+    };
+    // steps past the last level have no tasks and no barrier (a break between the steps
+    // would join the paths in a way that makes the next wait cover every read)
+    for (int l = 0; l < nl; l += 3)
     {
-      {
-        const int b4 = bound(l + 4);
-        level(b0, b1, pa, pb);
-        first_record(l + 2, b2, b3, pa, pb);
-        lds_barrier<NT>();
-        b0 = b1;
-        b1 = b2;
-        b2 = b3;
-        b3 = b4;
-      }
-      if (l + 1 >= nl)
-        break;
-      {
-        const int b4 = bound(l + 5);
-        level(b0, b1, qa, qb);
-        first_record(l + 3, b2, b3, qa, qb);
-        lds_barrier<NT>();
-        b0 = b1;
-        b1 = b2;
-        b2 = b3;
-        b3 = b4;
-      }
+      step(l, r0, r2);
+      step(l + 1, r1, r0);
+      step(l + 2, r2, r1);
     }
   }
 
   // The two-phase form of coop_levels with its records and terms in global memory,
-  // pipelined like coop_levels_piped: for level l + 2, each thread's first product term
-  // and first task record are read while level l runs, into one of two buffers that
-  // alternate between even and odd levels.  Needs every level's products to fit P.
+  // pipelined like coop_levels_piped: the first product term and the first task record
+  // of level l + 2 are read at the start of level l.  Needs every level's products to
+  // fit P.
   template<int NT>
   __device__ inline void coop_levels_piped2(const DevLevels& lv, const uint32_t* terms, const double* A,
                                             const double* X, double* out, double* P)
@@ -634,93 +728,113 @@ namespace micm_hip
     if (nl == 0)
       return;
     const uint4* rec = lv.records;
-    auto bound = [&](int i) { return lv.offsets[i < nl ? i : nl]; };
+    auto bound = [&](int i) { return const_load(lv.offsets, i < nl ? i : nl); };
     // first term of level i (the end of the terms past the last level)
-    auto term_bound = [&](int i) { return static_cast<int>(lv.tasks[bound(i)].y); };
+    auto term_bound = [&](int i) {
+      return static_cast<int>(const_load(reinterpret_cast<const uint32_t*>(lv.tasks), 2 * bound(i) + 1));
+    };
     struct Ahead
     {
-      uint4 ra, rb;  // first task's record
-      uint32_t pt;   // first product term
+      uint4 h;      // first task's record header (its terms are not needed: the sums read P)
+      uint32_t pt;  // first product term
     };
-    auto fetch = [&](int l, int b, int e, int qb, int qe, Ahead& h) {
-      const int t = (l < nl && b + tid < e) ? b + tid : lv.n_tasks;
-      h.ra = rec[2 * t];
-      h.rb = rec[2 * t + 1];
-      h.pt = terms[(l < nl && qb + tid < qe) ? qb + tid : 0];
-    };
-    auto level = [&](int b, int e, int qb, int qe, const Ahead& h) {
-      if (qb + tid < qe)
-        P[tid] = A[h.pt & 0xffffu] * X[h.pt >> 16];
-      for (int q = qb + tid + NT; q < qe; q += NT)
-      {
-        const uint32_t tr = terms[q];
-        P[q - qb] = A[tr & 0xffffu] * X[tr >> 16];
-      }
-      lds_barrier<NT>();
-      auto sum = [&](const uint4& ra) {
-        const uint32_t o = ra.x & 0xffffu, d = ra.x >> 16, n = ra.z;
-        double v = out[o];
-        const double dv = d < kUnitDivisor ? A[d] : 1.0;
-        const double* p = P + (static_cast<int>(ra.y) - qb);
-        uint32_t q = 0;
-        for (; q + 8 <= n; q += 8)
-        {
-          double s8[8];
-#pragma unroll
-          for (int r = 0; r < 8; ++r)
-            s8[r] = p[q + r];
-#pragma unroll
-          for (int r = 0; r < 8; ++r)
-            v -= s8[r];
-        }
-        for (; q < n; ++q)
-          v -= p[q];
-        if (d < kUnitDivisor)
-          v = v / dv;
-        out[o] = v;
-      };
-      if (b + tid < e)
-        sum(h.ra);
-      for (int t = b + tid + NT; t < e; t += NT)
-        sum(rec[2 * t]);
+    auto fetch = [&](int l, int b, int e, int qb, int qe) {
+      Ahead a;
+      a.h = rec[4 * ((l < nl && b + tid < e) ? b + tid : lv.n_tasks)];
+      a.pt = terms[(l < nl && qb + tid < qe) ? qb + tid : 0];
+      return a;
     };
     int b0 = bound(0), b1 = bound(1), b2 = bound(2), b3 = bound(3);
     int q0 = term_bound(0), q1 = term_bound(1), q2 = term_bound(2), q3 = term_bound(3);
-    Ahead ev, od;  // the even and the odd levels' buffers
-    fetch(0, b0, b1, q0, q1, ev);
-    fetch(1, b1, b2, q1, q2, od);
-    for (int l = 0; l < nl; l += 2)
+    Ahead r0 = fetch(0, b0, b1, q0, q1), r1 = fetch(1, b1, b2, q1, q2), r2;
+    // a task's ordered sum of its products in batches of eight, the reads of each batch
+    // issued before the previous batch is subtracted; it reads up to fifteen products
+    // past its last one (inside the buffer's slack), unused
+    auto sum = [&](const uint4& h) {
+      const uint32_t o = h.x & 0xffffu, d = h.x >> 16, n = h.z;
+      double v = out[o];
+      const double dv = d < kUnitDivisor ? A[d] : 1.0;
+      const double* p = P + (static_cast<int>(h.y) - q0);
+      double a[8], b[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        a[r] = p[r];
+      uint32_t q = 0;
+      for (; q + 16 <= n; q += 16)
+      {
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          b[r] = p[q + 8 + r];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          v -= a[r];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          a[r] = p[q + 16 + r];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          v -= b[r];
+      }
+      if (q + 8 <= n)
+      {
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          b[r] = p[q + 8 + r];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          v -= a[r];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          a[r] = b[r];
+        q += 8;
+      }
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        if (q + r < n)
+          v -= a[r];
+      if (d < kUnitDivisor)
+        v = v / dv;
+      out[o] = v;
+    };
+    auto step = [&](int l, const Ahead& cur, Ahead& ahead) {
+      const int b4 = bound(l + 4), q4 = term_bound(l + 4);
+      ahead = fetch(l + 2, b2, b3, q2, q3);
+      settle(cur.h);
+      settle(uint4{ cur.pt, 0u, 0u, 0u });
+      if (q0 + tid < q1)
+        P[tid] = A[cur.pt & 0xffffu] * X[cur.pt >> 16];
+      for (int q = q0 + tid + NT; q < q1; q += NT)
+      {
+        const uint32_t tr = terms[q];
+        P[q - q0] = A[tr & 0xffffu] * X[tr >> 16];
+      }
+      if (l < nl)
+        lds_barrier<NT>();
+      if (b0 + tid < b1)
+        sum(cur.h);
+      for (int t = b0 + tid + NT; t < b1; t += NT)
+        sum(rec[4 * t]);
+      if (l < nl)
+        lds_barrier<NT>();
+      b0 = b1;
+      b1 = b2;
+      b2 = b3;
+      b3 = b4;
+      q0 = q1;
+      q1 = q2;
+      q2 = q3;
+      q3 = q4;
+    };
+    // steps past the last level have no tasks and no barrier (a break between the steps
+    // would join the paths in a way that makes the next wait cover every read)
+    for (int l = 0; l < nl; l += 3)
     {
-      {
-        const int b4 = bound(l + 4), q4 = term_bound(l + 4);
-        level(b0, b1, q0, q1, ev);
-        fetch(l + 2, b2, b3, q2, q3, ev);
-        lds_barrier<NT>();
-        b0 = b1;
-        b1 = b2;
-        b2 = b3;
-        b3 = b4;
-        q0 = q1;
-        q1 = q2;
-        q2 = q3;
-        q3 = q4;
-      }
-      if (l + 1 >= nl)
-        break;
-      {
-        const int b4 = bound(l + 5), q4 = term_bound(l + 5);
-        level(b0, b1, q0, q1, od);
-        fetch(l + 3, b2, b3, q2, q3, od);
-        lds_barrier<NT>();
-        b0 = b1;
-        b1 = b2;
-        b2 = b3;
-        b3 = b4;
-        q0 = q1;
-        q1 = q2;
-        q2 = q3;
-        q3 = q4;
-      }
+      step(l, r0, r2);
+      step(l + 1, r1, r0);
+      step(l + 2, r2, r1);
     }
   }
 
@@ -886,14 +1000,15 @@ namespace micm_hip
   __device__ inline void coop_decompose(const DevMech& m, const DevCoop& c, const CoopLds& L, const CoopTables& T)
   {
     const int tid = threadIdx.x;
-    for (int q = tid; q < c.nnz_alu; q += NT)
+    // the factor and the zero one past it (that the LU records' padded slots read)
+    for (int q = tid; q <= c.nnz_alu; q += NT)
       L.alu[q] = 0.0;
     coop_sync<NT>();
     for (int e = tid; e < m.nnz_jac; e += NT)
       L.alu[c.jac_to_alu[e]] = L.jac[e];
     coop_sync<NT>();
     if constexpr (GT)
-      coop_levels_piped<NT>(c.lu, c.lu.terms, L.alu, L.alu, L.alu);
+      coop_levels_piped<NT, true>(c.lu, c.lu.terms, L.alu, L.alu, L.alu);
     else
       coop_levels_direct<NT, false>(c.lu, T.lu_tasks, T.lu_terms, L.alu, L.alu, L.alu);
   }
@@ -904,7 +1019,7 @@ namespace micm_hip
   {
     if constexpr (GT)
     {
-      coop_levels_piped<NT>(c.fwd, c.fwd.terms, L.alu, x, x);
+      coop_levels_piped<NT, false>(c.fwd, c.fwd.terms, L.alu, x, x);
       if (c.bwd.max_level_terms <= L.p_capacity)
         coop_levels_piped2<NT>(c.bwd, c.bwd.terms, L.alu, x, x, L.products);
       else

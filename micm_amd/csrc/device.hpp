@@ -62,7 +62,7 @@ namespace micm_hip
     const int32_t* offsets;  // n_levels + 1
     const uint2* tasks;      // n_tasks + 1
     const uint32_t* terms;   // two 16-bit positions
-    const uint4* records;    // 2 per task + 2 for the sentinel (LevelSchedule::records)
+    const uint4* records;    // 4 per task + 4 for the sentinel (LevelSchedule::records)
     int32_t n_levels, n_tasks, n_terms;
     int32_t max_level_terms;  // largest number of terms of one level
   };

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <queue>
 #include <stdexcept>
 
 namespace micm_hip
@@ -147,27 +148,35 @@ namespace micm_hip
     // Tasks grouped by level; within a level the longest first, so that a lane's
     // round-robin share is balanced.  tasks[2 t] = out | diag << 16, tasks[2 t + 1] =
     // first term; a sentinel task closes the last term range.
-    LevelSchedule pack_levels(std::vector<LevelTask>& v)
+    // `level` (one per task) replaces the tasks' own levels when given; `pad` fills the
+    // record slots past a task's last term.
+    LevelSchedule pack_levels(const std::vector<LevelTask>& v, const std::vector<int32_t>* level = nullptr,
+                              uint32_t pad = 0)
     {
-      std::stable_sort(v.begin(), v.end(), [](const LevelTask& a, const LevelTask& b) {
-        if (a.level != b.level)
-          return a.level < b.level;
-        return a.terms.size() > b.terms.size();
+      auto level_of = [&](size_t i) { return level ? (*level)[i] : v[i].level; };
+      std::vector<uint32_t> idx(v.size());
+      for (size_t i = 0; i < v.size(); ++i)
+        idx[i] = static_cast<uint32_t>(i);
+      std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+        if (level_of(a) != level_of(b))
+          return level_of(a) < level_of(b);
+        return v[a].terms.size() > v[b].terms.size();
       });
       LevelSchedule s;
       s.offsets.push_back(0);
       int32_t lv = 0;
       for (size_t q = 0; q < v.size(); ++q)
       {
-        while (v[q].level > lv)
+        const LevelTask& x = v[idx[q]];
+        while (level_of(idx[q]) > lv)
         {
           s.offsets.push_back(static_cast<int32_t>(q));
           ++lv;
         }
-        s.tasks.push_back(v[q].out | (v[q].diag << 16));
+        s.tasks.push_back(x.out | (x.diag << 16));
         s.tasks.push_back(static_cast<uint32_t>(s.terms.size()));
-        s.terms.insert(s.terms.end(), v[q].terms.begin(), v[q].terms.end());
-        s.max_terms = std::max<int32_t>(s.max_terms, static_cast<int32_t>(v[q].terms.size()));
+        s.terms.insert(s.terms.end(), x.terms.begin(), x.terms.end());
+        s.max_terms = std::max<int32_t>(s.max_terms, static_cast<int32_t>(x.terms.size()));
       }
       if (!v.empty())
         s.offsets.push_back(static_cast<int32_t>(v.size()));
@@ -176,7 +185,7 @@ namespace micm_hip
       for (size_t l = 0; l + 1 < s.offsets.size(); ++l)
         s.max_level_terms = std::max<int32_t>(
             s.max_level_terms, static_cast<int32_t>(s.tasks[2 * s.offsets[l + 1] + 1] - s.tasks[2 * s.offsets[l] + 1]));
-      // records: the task's header and its first terms in one 32-byte read
+      // records: the task's header and its first terms in one 64-byte read
       s.records.clear();
       for (size_t q = 0; q + 1 < s.tasks.size() / 2; ++q)
       {
@@ -185,7 +194,7 @@ namespace micm_hip
         s.records.push_back(first);
         s.records.push_back(count);
         for (uint32_t r = 0; r < static_cast<uint32_t>(kInlineTerms); ++r)
-          s.records.push_back(r < count ? s.terms[first + r] : 0u);
+          s.records.push_back(r < count ? s.terms[first + r] : pad);
       }
       // the sentinel: no output, no terms
       s.records.push_back(0u | (kNoDivisor << 16));
@@ -248,6 +257,146 @@ namespace micm_hip
       return last.level;
     }
 
+    // Order constraints of a schedule whose outputs, divisors and both operands of every
+    // term index one array (the in-place factorisation): a task comes after the writer of
+    // every value it reads (the value it read in the original schedule), after the
+    // previous writer of its output, and after every reader of the value its output
+    // replaces.  succ in CSR form; height = the longest path (in tasks) to the end.
+    struct TaskOrder
+    {
+      std::vector<uint32_t> succ_ptr, succ, n_deps;
+      std::vector<int32_t> height;
+      size_t widest = 0;  // most tasks of one level of the original schedule
+    };
+
+    TaskOrder task_order(const std::vector<LevelTask>& v)
+    {
+      TaskOrder o;
+      const size_t nt = v.size();
+      int32_t n_levels = 0;
+      for (const LevelTask& x : v)
+        n_levels = std::max(n_levels, x.level + 1);
+      std::vector<uint32_t> per_level(static_cast<size_t>(n_levels) + 1, 0);
+      for (const LevelTask& x : v)
+        ++per_level[static_cast<size_t>(x.level) + 1];
+      for (uint32_t c : per_level)
+        o.widest = std::max<size_t>(o.widest, c);
+      for (size_t l = 1; l < per_level.size(); ++l)
+        per_level[l] += per_level[l - 1];
+      std::vector<uint32_t> order(nt);  // tasks in level order
+      {
+        std::vector<uint32_t> fill(per_level.begin(), per_level.end() - 1);
+        for (size_t t = 0; t < nt; ++t)
+          order[fill[static_cast<size_t>(v[t].level)]++] = static_cast<uint32_t>(t);
+      }
+      auto reads = [&](const LevelTask& x, auto&& f) {
+        f(x.out);
+        if (x.diag < kUnitDivisor)
+          f(x.diag);
+        for (uint32_t tr : x.terms)
+        {
+          f(tr & 0xffffu);
+          f(tr >> 16);
+        }
+      };
+      uint32_t n_pos = 0;
+      for (const LevelTask& x : v)
+        reads(x, [&](uint32_t i) { n_pos = std::max(n_pos, i + 1); });
+      std::vector<int64_t> writer(n_pos, -1);
+      std::vector<std::vector<uint32_t>> readers(n_pos);
+      std::vector<std::pair<uint32_t, uint32_t>> edges;  // (before, after)
+      std::vector<uint32_t> d;
+      for (int32_t l = 0; l < n_levels; ++l)
+      {
+        const uint32_t b = per_level[static_cast<size_t>(l)], e = per_level[static_cast<size_t>(l) + 1];
+        for (uint32_t q = b; q < e; ++q)
+        {
+          const uint32_t t = order[q];
+          d.clear();
+          reads(v[t], [&](uint32_t i) {
+            if (writer[i] >= 0)
+              d.push_back(static_cast<uint32_t>(writer[i]));
+          });
+          for (uint32_t r : readers[v[t].out])
+            if (r != t)
+              d.push_back(r);
+          std::sort(d.begin(), d.end());
+          d.erase(std::unique(d.begin(), d.end()), d.end());
+          for (uint32_t x : d)
+            edges.emplace_back(x, t);
+        }
+        for (uint32_t q = b; q < e; ++q)
+          reads(v[order[q]], [&](uint32_t i) { readers[i].push_back(order[q]); });
+        for (uint32_t q = b; q < e; ++q)
+        {
+          writer[v[order[q]].out] = order[q];
+          readers[v[order[q]].out].clear();
+        }
+      }
+      o.succ_ptr.assign(nt + 1, 0);
+      o.n_deps.assign(nt, 0);
+      for (const auto& ed : edges)
+      {
+        ++o.succ_ptr[ed.first + 1];
+        ++o.n_deps[ed.second];
+      }
+      for (size_t t = 0; t < nt; ++t)
+        o.succ_ptr[t + 1] += o.succ_ptr[t];
+      o.succ.resize(edges.size());
+      {
+        std::vector<uint32_t> fill(o.succ_ptr.begin(), o.succ_ptr.end() - 1);
+        for (const auto& ed : edges)
+          o.succ[fill[ed.first]++] = ed.second;
+      }
+      o.height.assign(nt, 0);
+      for (size_t q = nt; q-- > 0;)
+      {
+        const uint32_t t = order[q];
+        for (uint32_t k = o.succ_ptr[t]; k < o.succ_ptr[t + 1]; ++k)
+          o.height[t] = std::max(o.height[t], o.height[o.succ[k]] + 1);
+      }
+      return o;
+    }
+
+    // New levels for the tasks such that no level holds more than `cap`: a list schedule
+    // that fills each level with the ready tasks on the longest remaining paths first.
+    // The tasks and their terms are unchanged, so every result is the same bit for bit.
+    std::vector<int32_t> cap_levels(const std::vector<LevelTask>& v, const TaskOrder& o, size_t cap)
+    {
+      const size_t nt = v.size();
+      std::vector<int32_t> level(nt, 0);
+      auto later = [&](uint32_t a, uint32_t b) {
+        if (o.height[a] != o.height[b])
+          return o.height[a] < o.height[b];
+        if (v[a].level != v[b].level)
+          return v[a].level > v[b].level;
+        return a > b;
+      };
+      std::priority_queue<uint32_t, std::vector<uint32_t>, decltype(later)> ready(later);
+      std::vector<uint32_t> pending(o.n_deps), next;
+      for (size_t t = 0; t < nt; ++t)
+        if (pending[t] == 0)
+          ready.push(static_cast<uint32_t>(t));
+      size_t placed = 0;
+      for (int32_t lv = 0; placed < nt; ++lv)
+      {
+        next.clear();
+        for (size_t k = 0; k < cap && !ready.empty(); ++k)
+        {
+          const uint32_t t = ready.top();
+          ready.pop();
+          level[t] = lv;
+          ++placed;
+          for (uint32_t q = o.succ_ptr[t]; q < o.succ_ptr[t + 1]; ++q)
+            if (--pending[o.succ[q]] == 0)
+              next.push_back(o.succ[q]);
+        }
+        for (uint32_t x : next)
+          ready.push(x);
+      }
+      return level;
+    }
+
     void build_cooperative_schedule(MechanismTables& t)
     {
       const int32_t n = static_cast<int32_t>(t.n_species);
@@ -284,6 +433,7 @@ namespace micm_hip
       t.coop_levels_ok = t.alu_col.size() < kUnitDivisor && n < 0xffff && t.n_reactions < 0xffff;
       t.lv_lu = LevelSchedule{};
       t.lv_lu_blocked.clear();
+      t.lv_lu_capped.clear();
       t.lv_fwd = LevelSchedule{};
       t.lv_bwd = LevelSchedule{};
       if (!t.coop_levels_ok)
@@ -318,7 +468,21 @@ namespace micm_hip
           level[out] = add_segments(tasks, static_cast<uint32_t>(out), diag, terms, ready, lower ? level[diag] : 0,
                                     min_len) + 1;
         }
-        t.lv_lu_blocked.push_back(pack_levels(tasks));
+        // record slots past a task's last term name the zero kept one past the factor
+        // (both operands), so the LU's records subtract 0 * 0 there
+        const uint32_t pad = static_cast<uint32_t>(t.alu_col.size()) * 0x10001u;
+        const TaskOrder order = task_order(tasks);
+        for (int c = 0; c < kNumLuCaps; ++c)
+        {
+          if (order.widest <= static_cast<size_t>(kLuCaps[c]))
+          {
+            t.lv_lu_capped.emplace_back();
+            continue;
+          }
+          const std::vector<int32_t> level_capped = cap_levels(tasks, order, static_cast<size_t>(kLuCaps[c]));
+          t.lv_lu_capped.push_back(pack_levels(tasks, &level_capped, pad));
+        }
+        t.lv_lu_blocked.push_back(pack_levels(tasks, nullptr, pad));
         if (kLuBlockings[b] == kLuBlockingDefault)
           t.lv_lu = t.lv_lu_blocked.back();
       }

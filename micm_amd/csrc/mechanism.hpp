@@ -24,8 +24,16 @@ namespace micm_hip
   constexpr int kNumLuBlockings = 5;
   constexpr int kLuBlockingDefault = 4;
 
-  // Terms carried in a task's record (LevelSchedule::records).
-  constexpr int kInlineTerms = 5;
+  // Widths of the LU schedules with capped levels: a workgroup of W threads runs the
+  // schedule whose levels hold at most W tasks, so that every thread has at most one task
+  // per level (the one whose record it read ahead); wider workgroups run the uncapped
+  // schedule.  Index c of lv_lu_capped.
+  constexpr int kLuCaps[] = { 64, 128, 256, 512 };
+  constexpr int kNumLuCaps = 4;
+
+  // Terms carried in a task's record (LevelSchedule::records: 16 words per task).
+  constexpr int kInlineTerms = 13;
+  constexpr int kRecordWords = 3 + kInlineTerms;
 
   // Independent tasks grouped by dependency level (see MechanismTables::lv_lu).
   struct LevelSchedule
@@ -33,8 +41,11 @@ namespace micm_hip
     std::vector<int32_t> offsets;  // level -> first task, n_levels + 1
     std::vector<uint32_t> tasks;   // 2 per task + sentinel: out | diag << 16, first term
     std::vector<uint32_t> terms;   // two 16-bit fields, then sixteen zero entries of padding
-    // 8 per task + sentinel: out | diag << 16, first term, number of terms, then the task's
-    // first kInlineTerms terms (0 past its last), so that one 32-byte read brings a task
+    // kRecordWords per task + sentinel: out | diag << 16, first term, number of terms, then
+    // the task's first kInlineTerms terms, so that one 64-byte read brings a task.  Slots
+    // past a task's last term hold 0, or in the LU schedules the position one past the
+    // factor in both fields: the device keeps a zero there, so that a padded slot subtracts
+    // 0 * 0 and needs no test.
     std::vector<uint32_t> records;
     int32_t max_terms = 0;
     int32_t max_level_terms = 0;  // largest number of terms of one level
@@ -109,6 +120,9 @@ namespace micm_hip
     // the LU schedule for each shortest-segment length of kLuBlockings (lv_lu is the one
     // for kLuBlockingDefault)
     std::vector<LevelSchedule> lv_lu_blocked;
+    // lv_lu_capped[b * kNumLuCaps + c]: lv_lu_blocked[b] re-levelled so that no level
+    // holds more than kLuCaps[c] tasks (empty where no level of lv_lu_blocked[b] does)
+    std::vector<LevelSchedule> lv_lu_capped;
     LevelSchedule lv_fwd;  // out = row, diag = kUnitDivisor on a row's last segment, terms (ALU position | column << 16)
     LevelSchedule lv_bwd;  // out = row, diag = U[i][i] position, terms (ALU position | column << 16)
     // packed operands: reaction -> reactant ids, Jacobian term -> (process, dependent ids);

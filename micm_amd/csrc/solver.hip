@@ -1514,7 +1514,8 @@ namespace
         {
           // -J(y) again (y and k are those of the step), its diagonal replaced by the
           // accumulated one: the same values as keeping -J and adding alpha in place
-          for (int q = lane; q < c.nnz_alu; q += NT)
+          // (and the zero one past the factor that the LU records' padded slots read)
+          for (int q = lane; q <= c.nnz_alu; q += NT)
             L.alu[q] = 0.0;
           coop_sync<NT>();
           coop_jacobian<NT, true>(m, c, L);
@@ -1527,7 +1528,7 @@ namespace
             L.alu[d] = jdiag_r;
           }
           coop_sync<NT>();
-          coop_levels_piped<NT>(c.lu, c.lu.terms, L.alu, L.alu, L.alu);
+          coop_levels_piped<NT, true>(c.lu, c.lu.terms, L.alu, L.alu, L.alu);
         }
         else
         {
@@ -1691,6 +1692,9 @@ struct micm_hip_mechanism
   DevMech dev{};
   DevCoop coop{};
   DevLevels lu_blocked[kNumLuBlockings] = {};  // the LU schedule of each blocking (kLuBlockings)
+  // lu_blocked[b] with levels capped at kLuCaps[c] tasks, at b * kNumLuCaps + c (offsets
+  // null where no level of lu_blocked[b] exceeds the cap)
+  DevLevels lu_capped[kNumLuBlockings * kNumLuCaps] = {};
   DevAdjoint adj{};
   void* blob = nullptr;  // one device allocation for every table
   int device = 0;
@@ -1823,6 +1827,9 @@ namespace
     size_t bytes = 0;
     for (const LevelSchedule& l : t.lv_lu_blocked)
       bytes += level_bytes(l);
+    for (const LevelSchedule& l : t.lv_lu_capped)
+      if (l.n_levels() > 0)
+        bytes += level_bytes(l);
     return bytes;
   }
 
@@ -2059,6 +2066,9 @@ namespace
     upload_levels(m->blob, off, t.lv_lu, c.lu, staging);
     for (size_t b = 0; b < t.lv_lu_blocked.size() && b < static_cast<size_t>(kNumLuBlockings); ++b)
       upload_levels(m->blob, off, t.lv_lu_blocked[b], m->lu_blocked[b], staging);
+    for (size_t k = 0; k < t.lv_lu_capped.size() && k < static_cast<size_t>(kNumLuBlockings * kNumLuCaps); ++k)
+      if (t.lv_lu_capped[k].n_levels() > 0)
+        upload_levels(m->blob, off, t.lv_lu_capped[k], m->lu_capped[k], staging);
     upload_levels(m->blob, off, t.lv_fwd, c.fwd, staging);
     upload_levels(m->blob, off, t.lv_bwd, c.bwd, staging);
     upload(m->blob, off, t.rxn_pack, c.rxn_pack, staging);
@@ -2232,14 +2242,29 @@ namespace
     return -1;
   }
 
-  // The cooperative tables with the LU schedule of `blocking`.
-  DevCoop coop_with_blocking(const micm_hip_mechanism* m, int blocking)
+  // The cooperative tables with the LU schedule of `blocking` for a workgroup of `threads`:
+  // its levels capped at `threads` tasks where that schedule exists.
+  DevCoop coop_with_blocking(const micm_hip_mechanism* m, int blocking, int threads)
   {
     DevCoop c = m->coop;
     const int b = lu_blocking_index(blocking);
-    if (m->t.coop_levels_ok && b >= 0 && m->lu_blocked[b].offsets)
+    if (!m->t.coop_levels_ok || b < 0)
+      return c;
+    if (m->lu_blocked[b].offsets)
       c.lu = m->lu_blocked[b];
+    for (int k = 0; k < kNumLuCaps; ++k)
+      if (kLuCaps[k] == threads && m->lu_capped[b * kNumLuCaps + k].offsets)
+        c.lu = m->lu_capped[b * kNumLuCaps + k];
     return c;
+  }
+
+  // Threads of a cooperative workgroup as plan_coop takes them: 128 ... 1024, else one
+  // wavefront.
+  int coop_threads_of(int threads_request)
+  {
+    return (threads_request == 128 || threads_request == 256 || threads_request == 512 || threads_request == 1024)
+               ? threads_request
+               : kWave;
   }
 
   CoopPlan plan_coop_full(const micm_hip_mechanism* m, int stages, int stage_request, int threads_request,
@@ -2255,15 +2280,12 @@ namespace
     // layout built for three / two waves per SIMD / without a register bound (256 threads)
     const bool compact_request = stage_request >= 4;
     CoopPlan plan = plan_coop_full(m, stages, compact_request ? -1 : stage_request, threads_request, lu_blocking);
-    const DevCoop c = coop_with_blocking(m, lu_blocking);
     if (!m->t.coop_levels_ok || (stage_request >= 0 && !compact_request))
       return plan;
     // the compact layout keeps one species per thread in registers and publishes the
     // rate constants through global memory, so it needs a workgroup barrier
-    const int threads = (threads_request == 128 || threads_request == 256 || threads_request == 512 ||
-                         threads_request == 1024)
-                            ? threads_request
-                            : kWave;
+    const int threads = coop_threads_of(threads_request);
+    const DevCoop c = coop_with_blocking(m, lu_blocking, threads);
     if (threads <= kWave || threads < m->dev.n_species)
       return plan;
     const size_t compact = coop_compact_lds_doubles(c.nnz_alu, m->dev.n_species, stages, m->dev.n_reactions,
@@ -2305,7 +2327,8 @@ namespace
                           int lu_blocking)
   {
     CoopPlan plan;
-    const DevCoop c = coop_with_blocking(m, lu_blocking);
+    const int threads = coop_threads_of(threads_request);
+    const DevCoop c = coop_with_blocking(m, lu_blocking, threads);
     if (!m->t.coop_levels_ok)
       return plan;
     constexpr size_t kLdsPerCu = 160 * 1024;
@@ -2340,10 +2363,7 @@ namespace
     plan.lds = base + ((stage & 1) ? solve : 0) + ((stage & 2) ? lu : 0);
     plan.stage = stage;
     plan.p_capacity = p_solve;
-    plan.threads = (threads_request == 128 || threads_request == 256 || threads_request == 512 ||
-                    threads_request == 1024)
-                       ? threads_request
-                       : kWave;
+    plan.threads = threads;
     return plan;
   }
 
@@ -2384,7 +2404,8 @@ namespace
       blocks = std::max<int64_t>(1, std::min(blocks, n_fixed));
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, next);
     hipLaunchKernelGGL((k_coop<NT, STAGE, COMPACT, MINW, RECORD>), dim3(static_cast<unsigned>(blocks)), dim3(NT),
-                       plan.lds, st, m->dev, coop_with_blocking(m, s->lu_blocking), p, plan.p_capacity, n_fixed, list, count,
+                       plan.lds, st, m->dev, coop_with_blocking(m, s->lu_blocking, NT), p, plan.p_capacity, n_fixed, list,
+                       count,
                        next, resume, s->y, s->k, s->atol,
                        s->status, s->stats, s->final_time, update_rates ? s->cond : nullptr,
                        update_rates ? s->custom : nullptr, k_rows, s->recording);

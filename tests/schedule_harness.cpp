@@ -107,6 +107,92 @@ extern "C"
   }
 }
 
+namespace
+{
+  // True when a task of some level writes a position that another task of the same level
+  // reads or writes (a race on the device, invisible to run()'s level snapshots).
+  bool same_level_conflict(const LevelSchedule& s)
+  {
+    for (int32_t l = 0; l < s.n_levels(); ++l)
+    {
+      std::vector<std::pair<uint32_t, int32_t>> written;  // position, task
+      for (int32_t t = s.offsets[l]; t < s.offsets[l + 1]; ++t)
+        written.emplace_back(s.tasks[2 * t] & 0xffffu, t);
+      std::sort(written.begin(), written.end());
+      for (size_t i = 1; i < written.size(); ++i)
+        if (written[i].first == written[i - 1].first)
+          return true;
+      auto writer = [&](uint32_t pos) {
+        auto it = std::lower_bound(written.begin(), written.end(), std::make_pair(pos, INT32_MIN));
+        return it != written.end() && it->first == pos ? it->second : -1;
+      };
+      for (int32_t t = s.offsets[l]; t < s.offsets[l + 1]; ++t)
+      {
+        const uint32_t d = s.tasks[2 * t] >> 16;
+        if (d < micm_hip::kUnitDivisor && writer(d) >= 0 && writer(d) != t)
+          return true;
+        for (uint32_t q = s.tasks[2 * t + 1]; q < s.tasks[2 * t + 3]; ++q)
+          for (uint32_t pos : { s.terms[q] & 0xffffu, s.terms[q] >> 16 })
+            if (writer(pos) >= 0 && writer(pos) != t)
+              return true;
+      }
+    }
+    return false;
+  }
+}  // namespace
+
+extern "C"
+{
+  // The factorisation of harness_factor_solve by every LU schedule with capped levels,
+  // blocking b and cap c at slot s = b * kNumLuCaps + c of lower / upper (nnz_lower /
+  // nnz_upper values each) and levels: the number of levels, 0 where there is no capped
+  // schedule (no level exceeds the cap), -2 where a level holds more tasks than the cap or
+  // two tasks of one level touch one position.  Returns the number of slots.
+  int harness_factor_capped(const micm_hip_mechanism_desc* desc, const double* jac, double alpha, double* lower,
+                            double* upper, int32_t* levels)
+  {
+    const MechanismTables t = micm_hip::compile_mechanism(*desc);
+    const int32_t n = static_cast<int32_t>(t.n_species);
+    const size_t slots = t.lv_lu_capped.size();
+    for (size_t k = 0; k < slots; ++k)
+    {
+      const LevelSchedule& s = t.lv_lu_capped[k];
+      const int cap = micm_hip::kLuCaps[k % micm_hip::kNumLuCaps];
+      levels[k] = s.n_levels();
+      if (s.n_levels() == 0)
+        continue;
+      for (int32_t l = 0; l < s.n_levels(); ++l)
+        if (s.offsets[l + 1] - s.offsets[l] > cap)
+          levels[k] = -2;
+      if (same_level_conflict(s))
+        levels[k] = -2;
+      std::vector<double> alu(t.nnz_alu(), 0.0), none;
+      for (size_t e = 0; e < t.jac_to_alu.size(); ++e)
+        alu[t.jac_to_alu[e]] = jac[e];
+      for (int32_t i = 0; i < n; ++i)
+        alu[t.alu_diag[i]] += alpha;
+      run(s, nullptr, alu, none, true, true);
+      double* lo = lower + k * t.alu_lower.size();
+      double* up = upper + k * t.alu_upper.size();
+      for (size_t p = 0; p < t.alu_lower.size(); ++p)
+        lo[p] = t.alu_lower[p] >= 0 ? alu[t.alu_lower[p]] : 1.0;
+      for (size_t p = 0; p < t.alu_upper.size(); ++p)
+        up[p] = alu[t.alu_upper[p]];
+    }
+    return static_cast<int>(slots);
+  }
+
+  // 1 when some level of the (uncapped) LU schedule of blocking `b` has two tasks
+  // touching one position, 0 when none has, -1 for a bad index.
+  int harness_blocked_conflict(const micm_hip_mechanism_desc* desc, int32_t b)
+  {
+    const MechanismTables t = micm_hip::compile_mechanism(*desc);
+    if (b < 0 || static_cast<size_t>(b) >= t.lv_lu_blocked.size())
+      return -1;
+    return same_level_conflict(t.lv_lu_blocked[b]) ? 1 : 0;
+  }
+}
+
 extern "C"
 {
   // Per level of schedule `which` (0 LU, 1 forward, 2 backward): tasks and the

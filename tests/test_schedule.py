@@ -30,6 +30,9 @@ def harness(tmp_path_factory):
     lib.harness_sizes.argtypes = [ctypes.c_void_p, _ip]
     lib.harness_factor_solve.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp, _dp, _dp]
     lib.harness_factor_blocked.argtypes = [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.c_double, _dp, _dp]
+    lib.harness_factor_capped.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp, _dp,
+                                          ctypes.POINTER(ctypes.c_int32)]
+    lib.harness_blocked_conflict.argtypes = [ctypes.c_void_p, ctypes.c_int32]
     return lib
 
 
@@ -88,5 +91,23 @@ def test_level_schedules_reproduce_doolittle(harness, name):
             levels.append(nl)
             assert bits_equal(lo_b, lo_ref[0]) and bits_equal(up_b, up_ref[0])
         assert len(set(levels)) == 1
+        # the schedules re-levelled to at most 64 ... 512 tasks per level: same factors,
+        # no level above its cap, no two tasks of a level touching one position, and at
+        # least as many levels as the uncapped schedule
+        if c == 0:
+            for b in range(5):
+                assert harness.harness_blocked_conflict(ctypes.byref(cdesc), b) == 0
+        slots = 5 * 4
+        lo_c, up_c = np.zeros((slots, nnz_l)), np.zeros((slots, nnz_u))
+        nl_c = np.zeros(slots, dtype=np.int32)
+        assert harness.harness_factor_capped(ctypes.byref(cdesc), np.ascontiguousarray(jac[c]).ctypes.data_as(_dp),
+                                             alpha, lo_c.ctypes.data_as(_dp), up_c.ctypes.data_as(_dp),
+                                             nl_c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))) == slots
+        for k in range(slots):
+            assert nl_c[k] == 0 or nl_c[k] >= levels[0], (k, nl_c[k])
+            if nl_c[k] > 0:
+                assert bits_equal(lo_c[k], lo_ref[0]) and bits_equal(up_c[k], up_ref[0]), k
+        if name == "210_560":
+            assert (nl_c > 0).all()  # every cap binds somewhere for this mechanism
         assert harness.harness_factor_blocked(ctypes.byref(cdesc), 5, np.ascontiguousarray(jac[c]).ctypes.data_as(_dp),
                                               alpha, lo.ctypes.data_as(_dp), up.ctypes.data_as(_dp)) == -1
