@@ -664,6 +664,63 @@ namespace micm_hip
     out[o] = v;
   }
 
+  __device__ __forceinline__ double readlane_f64(double v, uint32_t lane)
+  {
+    const uint64_t b = static_cast<uint64_t>(__double_as_longlong(v));
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(b), lane));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane));
+    return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
+  }
+
+  // The forward substitution's column sweep (ColumnSweep) by the first wavefront: lane r
+  // keeps its row's partial sum in a register; at column c the column's value comes from
+  // the lane that finished it (or from x, final before the sweep) through a lane read,
+  // and every lane with a term there subtracts L * x[c] (the others keep their sum by a
+  // select, so no value of x enters a row it is not in).  A lane's swept terms sit at
+  // consecutive ALU positions, its term at column c at start + (its columns below c);
+  // each factor value is read two columns before its use, over three rotating registers
+  // (unrolled by three so that none moves), the reads for columns a lane has no term at
+  // staying inside the factor and its zero.  `me` is this lane's entry of the table,
+  // read before the levels that precede the sweep.
+  template<int NT>
+  __device__ inline void coop_sweep(const DevSweep& sw, const uint4& me, const double* A, double* x)
+  {
+    if (threadIdx.x < kWave)
+    {
+      const uint32_t lane = threadIdx.x;
+      const int n_cols = sw.n_cols;
+      const uint32_t row = me.x & 0xffffu, start = me.x >> 16;
+      const uint64_t mask = me.y | (static_cast<uint64_t>(me.z) << 32);
+      const uint32_t src = me.w;
+      double v = row != 0xffffu ? x[row] : 0.0;
+      const double xv = lane < static_cast<uint32_t>(n_cols) && !(src & kSweepFromLane) ? x[src] : 0.0;
+      auto factor = [&](int c) {
+        const int cc = c < 63 ? c : 63;
+        return A[start + static_cast<uint32_t>(__popcll(mask & ((uint64_t(1) << cc) - 1u)))];
+      };
+      auto step = [&](int c, double a) {
+        const uint32_t cl = static_cast<uint32_t>(c) & 63u;
+        const uint32_t s = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(src), cl));
+        const double xc = (s & kSweepFromLane) ? readlane_f64(v, s & 63u) : readlane_f64(xv, cl);
+        const double nv = v - a * xc;
+        v = (c < n_cols && ((mask >> cl) & 1u)) ? nv : v;
+      };
+      double a0 = factor(0), a1 = factor(1), a2;
+      for (int c = 0; c < n_cols; c += 3)
+      {
+        a2 = factor(c + 2);
+        step(c, a0);
+        a0 = factor(c + 3);
+        step(c + 1, a1);
+        a1 = factor(c + 4);
+        step(c + 2, a2);
+      }
+      if (row != 0xffffu)
+        x[row] = v;
+    }
+    lds_barrier<NT>();
+  }
+
   // A level schedule with its records in global memory, pipelined: at the start of level
   // l each thread reads the record of its first task at level l + 2, then runs level l
   // from the record read two levels earlier, so every read has a whole level to arrive.
@@ -1017,9 +1074,17 @@ namespace micm_hip
   template<int NT, bool GT = false>
   __device__ inline void coop_solve(const DevCoop& c, const CoopLds& L, const CoopTables& T, double* x)
   {
+    // the forward substitution's levels before its column sweep, then the sweep
+    const bool sweep = c.fwd_sweep.first_level >= 0;
+    const uint4 me = sweep && threadIdx.x < kWave ? c.fwd_sweep.lanes[threadIdx.x] : make_uint4(0u, 0u, 0u, 0u);
+    DevLevels fwd = c.fwd;
+    if (sweep)
+      fwd.n_levels = c.fwd_sweep.first_level;
     if constexpr (GT)
     {
-      coop_levels_piped<NT, false>(c.fwd, c.fwd.terms, L.alu, x, x);
+      coop_levels_piped<NT, false>(fwd, c.fwd.terms, L.alu, x, x);
+      if (sweep)
+        coop_sweep<NT>(c.fwd_sweep, me, L.alu, x);
       if (c.bwd.max_level_terms <= L.p_capacity)
         coop_levels_piped2<NT>(c.bwd, c.bwd.terms, L.alu, x, x, L.products);
       else
@@ -1027,7 +1092,9 @@ namespace micm_hip
     }
     else
     {
-      coop_levels_direct<NT, false>(c.fwd, T.fwd_tasks, T.fwd_terms, L.alu, x, x);
+      coop_levels_direct<NT, false>(fwd, T.fwd_tasks, T.fwd_terms, L.alu, x, x);
+      if (sweep)
+        coop_sweep<NT>(c.fwd_sweep, me, L.alu, x);
       coop_levels<NT>(c.bwd, T.bwd_tasks, T.bwd_terms, L.alu, x, x, L.products, L.p_capacity);
     }
   }

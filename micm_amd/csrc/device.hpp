@@ -67,6 +67,16 @@ namespace micm_hip
     int32_t max_level_terms;  // largest number of terms of one level
   };
 
+  // The forward substitution's column sweep (ColumnSweep): its levels from first_level on
+  // run as n_cols column steps of one wavefront; lanes[r] = row | start << 16, the row's
+  // column mask (low, high word), and the source of column r.
+  struct DevSweep
+  {
+    const uint4* lanes;  // kSweepLanes
+    int32_t first_level;  // -1: none
+    int32_t n_cols;
+  };
+
   // Tables of the cooperative (one workgroup per cell) form; see MechanismTables.
   struct DevCoop
   {
@@ -77,6 +87,7 @@ namespace micm_hip
     const int32_t* alu_lower;  // L storage index -> ALU position (-1: the unit diagonal)
     const int32_t* alu_upper;  // U storage index -> ALU position
     DevLevels lu, fwd, bwd;
+    DevSweep fwd_sweep;
     const uint64_t* rxn_pack;   // reactant ids of each reaction (see MechanismTables)
     const uint64_t* term_pack;  // process and dependent ids of each Jacobian term
     const int32_t* spec_offsets;

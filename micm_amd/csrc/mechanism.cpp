@@ -397,6 +397,97 @@ namespace micm_hip
       return level;
     }
 
+    // The run of final levels of a forward substitution schedule that one wavefront
+    // sweeps column by column (see ColumnSweep) with the largest gain: at most kSweepLanes
+    // rows and 64 columns, every row's swept terms at consecutive ALU positions in
+    // ascending columns, each below the row, and no divisions.  A level costs about as
+    // much as eight columns of a sweep (a barrier and two LDS round trips against a few
+    // instructions), so the run kept is the one with the most levels per column; none
+    // when no run gains.
+    ColumnSweep column_sweep(const LevelSchedule& s)
+    {
+      constexpr int32_t kColumnsPerLevel = 8;
+      ColumnSweep best;
+      int32_t best_gain = 0;
+      const int32_t nl = s.n_levels();
+      struct Row
+      {
+        uint32_t row;
+        std::vector<uint32_t> terms;
+      };
+      for (int32_t first = nl - 1; first >= 0; --first)
+      {
+        // rows and their terms in level order
+        std::vector<Row> rows;
+        bool ok = true;
+        for (int32_t l = first; l < nl && ok; ++l)
+          for (int32_t q = s.offsets[l]; q < s.offsets[l + 1] && ok; ++q)
+          {
+            const uint32_t o = s.tasks[2 * q] & 0xffffu, d = s.tasks[2 * q] >> 16;
+            if (d < kUnitDivisor)
+              ok = false;
+            Row* r = nullptr;
+            for (Row& x : rows)
+              if (x.row == o)
+                r = &x;
+            if (!r)
+            {
+              rows.push_back(Row{ o, {} });
+              r = &rows.back();
+            }
+            r->terms.insert(r->terms.end(), s.terms.begin() + s.tasks[2 * q + 1], s.terms.begin() + s.tasks[2 * q + 3]);
+          }
+        if (!ok || rows.size() > static_cast<size_t>(kSweepLanes))
+          break;
+        std::vector<uint32_t> cols;
+        for (const Row& r : rows)
+          for (uint32_t tr : r.terms)
+            cols.push_back(tr >> 16);
+        std::sort(cols.begin(), cols.end());
+        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+        if (cols.size() > 64)
+          break;
+        for (const Row& r : rows)
+          for (size_t k = 0; k < r.terms.size() && ok; ++k)
+          {
+            const uint32_t pos = r.terms[k] & 0xffffu, col = r.terms[k] >> 16;
+            ok = col < r.row && (k == 0 || (pos == (r.terms[0] & 0xffffu) + k && col > (r.terms[k - 1] >> 16)));
+          }
+        if (!ok)
+          break;
+        const int32_t gain = kColumnsPerLevel * (nl - first) - static_cast<int32_t>(cols.size());
+        if (gain <= best_gain)
+          continue;
+        best_gain = gain;
+        ColumnSweep sw;
+        sw.first_level = first;
+        sw.n_cols = static_cast<int32_t>(cols.size());
+        sw.lanes.assign(4 * kSweepLanes, 0u);
+        for (int r = 0; r < kSweepLanes; ++r)
+          sw.lanes[4 * r] = 0xffffu;
+        for (size_t r = 0; r < rows.size(); ++r)
+        {
+          uint64_t mask = 0;
+          for (uint32_t tr : rows[r].terms)
+            mask |= uint64_t(1) << (std::lower_bound(cols.begin(), cols.end(), tr >> 16) - cols.begin());
+          const uint32_t start = rows[r].terms.empty() ? 0u : (rows[r].terms[0] & 0xffffu);
+          sw.lanes[4 * r] = rows[r].row | (start << 16);
+          sw.lanes[4 * r + 1] = static_cast<uint32_t>(mask);
+          sw.lanes[4 * r + 2] = static_cast<uint32_t>(mask >> 32);
+        }
+        for (size_t c = 0; c < cols.size(); ++c)
+        {
+          uint32_t src = cols[c];
+          for (size_t r = 0; r < rows.size(); ++r)
+            if (rows[r].row == cols[c])
+              src = static_cast<uint32_t>(r) | kSweepFromLane;
+          sw.lanes[4 * c + 3] = src;
+        }
+        best = std::move(sw);
+      }
+      return best;
+    }
+
     void build_cooperative_schedule(MechanismTables& t)
     {
       const int32_t n = static_cast<int32_t>(t.n_species);
@@ -435,6 +526,7 @@ namespace micm_hip
       t.lv_lu_blocked.clear();
       t.lv_lu_capped.clear();
       t.lv_fwd = LevelSchedule{};
+      t.fwd_sweep = ColumnSweep{};
       t.lv_bwd = LevelSchedule{};
       if (!t.coop_levels_ok)
         return;
@@ -507,6 +599,7 @@ namespace micm_hip
           level[i] = add_segments(tasks, static_cast<uint32_t>(i), kUnitDivisor, terms, ready, 0, kSolveSegmentMin) + 1;
         }
         t.lv_fwd = pack_levels(tasks);
+        t.fwd_sweep = column_sweep(t.lv_fwd);
       }
       // back substitution: row i subtracts U[i][j] x[j], j ascending, then divides by U[i][i]
       {

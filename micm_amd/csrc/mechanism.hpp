@@ -53,6 +53,26 @@ namespace micm_hip
     int32_t n_tasks() const { return static_cast<int32_t>(tasks.size() / 2) - 1; }
   };
 
+  // The last levels of the forward substitution run as one column sweep by a single
+  // wavefront: lane r holds the partial sum of row rows[r]; the columns (x positions the
+  // swept terms read) are taken in ascending order, and at column c every lane whose row
+  // has a term there subtracts L * x[c], x[c] being another lane's finished row or a value
+  // final before the sweep.  Each row's terms stay in their order, so the results are
+  // bit for bit those of the level schedule; the chain of one-term levels costs a few
+  // instructions per column instead of a workgroup barrier and a round trip per level.
+  struct ColumnSweep
+  {
+    int32_t first_level = -1;  // levels from here on are swept (-1: no sweep)
+    int32_t n_cols = 0;
+    // kSweepLanes entries of 4 words: row | start << 16 (0xffff: no row; start = ALU
+    // position of the row's first swept term, the others following it), the 64-bit mask
+    // of the columns the row has terms at, and this lane's column source: the lane whose
+    // row is that column | kSweepFromLane, else the x position
+    std::vector<uint32_t> lanes;
+  };
+  constexpr int kSweepLanes = 64;
+  constexpr uint32_t kSweepFromLane = 0x80000000u;
+
   // Rate parameters as consumed by the device (per sorted reaction, stride
   // MICM_HIP_RATE_PARAM_STRIDE).  Derived values live in spare slots:
   //   BRANCHED  slot 5 = k0, slot 6 = z   (reaction_rate_store.hpp:286-293)
@@ -124,6 +144,7 @@ namespace micm_hip
     // holds more than kLuCaps[c] tasks (empty where no level of lv_lu_blocked[b] does)
     std::vector<LevelSchedule> lv_lu_capped;
     LevelSchedule lv_fwd;  // out = row, diag = kUnitDivisor on a row's last segment, terms (ALU position | column << 16)
+    ColumnSweep fwd_sweep;  // lv_fwd's last levels as a column sweep
     LevelSchedule lv_bwd;  // out = row, diag = U[i][i] position, terms (ALU position | column << 16)
     // packed operands: reaction -> reactant ids, Jacobian term -> (process, dependent ids);
     // three / two 16-bit ids and the count in bits 48-63 (0xffff: more, use the lists)

@@ -2009,7 +2009,7 @@ namespace
              padded_bytes(t.fwd_pairs) + padded_bytes(t.bwd_offsets) + padded_bytes(t.bwd_pairs) +
              padded_bytes(t.jac_to_alu) + padded_bytes(t.alu_diag) + padded_bytes(t.alu_lower) +
              padded_bytes(t.alu_upper) + level_bytes(t.lv_lu) + blocked_level_bytes(t) + level_bytes(t.lv_fwd) +
-             level_bytes(t.lv_bwd) +
+             padded_bytes(t.fwd_sweep.lanes) + level_bytes(t.lv_bwd) +
              padded_bytes(t.rxn_pack) + padded_bytes(t.term_pack) +
              padded_bytes(t.spec_offsets) + padded_bytes(t.spec_rxn) +
              padded_bytes(t.spec_coef) + padded_bytes(t.jel_offsets) + padded_bytes(t.jel_term) +
@@ -2070,6 +2070,13 @@ namespace
       if (t.lv_lu_capped[k].n_levels() > 0)
         upload_levels(m->blob, off, t.lv_lu_capped[k], m->lu_capped[k], staging);
     upload_levels(m->blob, off, t.lv_fwd, c.fwd, staging);
+    c.fwd_sweep = DevSweep{ nullptr, -1, 0 };
+    if (t.fwd_sweep.first_level >= 0)
+    {
+      const uint32_t* lanes = nullptr;
+      upload(m->blob, off, t.fwd_sweep.lanes, lanes, staging);
+      c.fwd_sweep = DevSweep{ reinterpret_cast<const uint4*>(lanes), t.fwd_sweep.first_level, t.fwd_sweep.n_cols };
+    }
     upload_levels(m->blob, off, t.lv_bwd, c.bwd, staging);
     upload(m->blob, off, t.rxn_pack, c.rxn_pack, staging);
     upload(m->blob, off, t.term_pack, c.term_pack, staging);

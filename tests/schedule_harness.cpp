@@ -193,8 +193,70 @@ extern "C"
   }
 }
 
+namespace
+{
+  // The forward substitution as the device runs it with a column sweep: the levels before
+  // the sweep, then the sweep lane by lane and column by column (micm_amd/csrc/coop.hpp,
+  // coop_sweep).
+  void forward_swept(const MechanismTables& t, const std::vector<double>& alu, std::vector<double>& x)
+  {
+    const micm_hip::ColumnSweep& sw = t.fwd_sweep;
+    LevelSchedule head = t.lv_fwd;
+    head.offsets.resize(static_cast<size_t>(sw.first_level) + 1);
+    std::vector<double> none;
+    run(head, alu.data(), none, x, false, false);
+    const int lanes = micm_hip::kSweepLanes;
+    std::vector<double> v(lanes, 0.0), xv(lanes, 0.0);
+    for (int r = 0; r < lanes; ++r)
+    {
+      const uint32_t row = sw.lanes[4 * r] & 0xffffu, src = sw.lanes[4 * r + 3];
+      v[r] = row != 0xffffu ? x[row] : 0.0;
+      xv[r] = r < sw.n_cols && !(src & micm_hip::kSweepFromLane) ? x[src] : 0.0;
+    }
+    for (int c = 0; c < sw.n_cols; ++c)
+    {
+      const uint32_t s = sw.lanes[4 * c + 3];
+      const double xc = (s & micm_hip::kSweepFromLane) ? v[s & 63u] : xv[c];
+      for (int r = 0; r < lanes; ++r)
+      {
+        const uint64_t mask = sw.lanes[4 * r + 1] | (static_cast<uint64_t>(sw.lanes[4 * r + 2]) << 32);
+        if (!((mask >> c) & 1u))
+          continue;
+        const uint32_t start = sw.lanes[4 * r] >> 16;
+        const uint64_t below = mask & ((uint64_t(1) << c) - 1u);
+        v[r] = v[r] - alu[start + static_cast<uint32_t>(__builtin_popcountll(below))] * xc;
+      }
+    }
+    for (int r = 0; r < lanes; ++r)
+      if ((sw.lanes[4 * r] & 0xffffu) != 0xffffu)
+        x[sw.lanes[4 * r] & 0xffffu] = v[r];
+  }
+}  // namespace
+
 extern "C"
 {
+  // harness_factor_solve with the forward substitution's column sweep; returns the first
+  // swept level (-1: no sweep, and b untouched).
+  int harness_solve_swept(const micm_hip_mechanism_desc* desc, const double* jac, double alpha, double* b)
+  {
+    const MechanismTables t = micm_hip::compile_mechanism(*desc);
+    if (t.fwd_sweep.first_level < 0)
+      return -1;
+    const int32_t n = static_cast<int32_t>(t.n_species);
+    std::vector<double> alu(t.nnz_alu(), 0.0), none;
+    for (size_t e = 0; e < t.jac_to_alu.size(); ++e)
+      alu[t.jac_to_alu[e]] = jac[e];
+    for (int32_t i = 0; i < n; ++i)
+      alu[t.alu_diag[i]] += alpha;
+    run(t.lv_lu, nullptr, alu, none, true, true);
+    alu.push_back(0.0);  // the zero one past the factor
+    std::vector<double> x(b, b + n);
+    forward_swept(t, alu, x);
+    run(t.lv_bwd, alu.data(), none, x, false, false);
+    std::memcpy(b, x.data(), sizeof(double) * n);
+    return t.fwd_sweep.first_level;
+  }
+
   // Per level of schedule `which` (0 LU, 1 forward, 2 backward): tasks and the
   // largest number of terms of one task; returns the number of levels written.
   int harness_level_stats(const micm_hip_mechanism_desc* desc, int32_t which, int64_t* tasks, int64_t* max_terms,

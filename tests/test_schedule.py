@@ -33,6 +33,7 @@ def harness(tmp_path_factory):
     lib.harness_factor_capped.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp, _dp,
                                           ctypes.POINTER(ctypes.c_int32)]
     lib.harness_blocked_conflict.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+    lib.harness_solve_swept.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp]
     return lib
 
 
@@ -78,6 +79,14 @@ def test_level_schedules_reproduce_doolittle(harness, name):
                                      lo.ctypes.data_as(_dp), up.ctypes.data_as(_dp), x.ctypes.data_as(_dp))
         assert bits_equal(lo, lo_ref[0]) and bits_equal(up, up_ref[0])
         assert bits_equal(x, x_ref[0])
+        # the forward substitution's last levels as the single-wavefront column sweep
+        xs = f[c].copy()
+        first = harness.harness_solve_swept(ctypes.byref(cdesc), np.ascontiguousarray(jac[c]).ctypes.data_as(_dp),
+                                            alpha, xs.ctypes.data_as(_dp))
+        if name in ("210_560", "60_200", "ts1"):
+            assert first > 0, name  # these mechanisms sweep the forward tail
+        if first >= 0:
+            assert bits_equal(xs, x_ref[0])
         # every LU blocking (shortest segment 1, 2, 4, 8, whole sums) gives the same factors;
         # the blocking changes how a sum is split over the levels, not the level at which
         # the element is final, so every blocking has as many levels
