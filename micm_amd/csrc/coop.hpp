@@ -672,6 +672,203 @@ namespace micm_hip
     return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
   }
 
+  // v - p[0] - p[1] - ... - p[n-1], subtracted in that order, in batches of eight whose
+  // reads are issued before the previous batch is subtracted; reads up to fifteen
+  // entries past p[n-1] (unused), which must lie in LDS the caller owns.
+  __device__ __forceinline__ double ordered_sub(double v, const double* p, uint32_t n)
+  {
+    double a[8], b[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+      a[r] = p[r];
+    uint32_t q = 0;
+    for (; q + 16 <= n; q += 16)
+    {
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        b[r] = p[q + 8 + r];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        v -= a[r];
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        a[r] = p[q + 16 + r];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        v -= b[r];
+    }
+    if (q + 8 <= n)
+    {
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        b[r] = p[q + 8 + r];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        v -= a[r];
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        a[r] = b[r];
+      q += 8;
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+      if (q + r < n)
+        v -= a[r];
+    return v;
+  }
+
+  // v - p[0] - ... - p[n-1] in that order for n <= 64, straight-line code: batch k + 1's
+  // reads are issued before batch k is subtracted, and the batch that holds the last
+  // products subtracts only those (reads up to fifteen entries past p[n-1], unused, in
+  // LDS the caller owns).
+  __device__ __forceinline__ double ordered_sub_short(double v, const double* p, uint32_t n)
+  {
+    const uint32_t nb = n >> 3, rem = n & 7u;
+    double cur[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+      cur[r] = p[r];
+#pragma unroll
+    for (uint32_t k = 0; k <= 8; ++k)
+    {
+      if (k == nb)
+      {
+#pragma unroll
+        for (uint32_t r = 0; r < 8; ++r)
+          if (r < rem)
+            v -= cur[r];
+        break;
+      }
+      double nxt[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        nxt[r] = p[8 * k + 8 + r];
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        v -= cur[r];
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        cur[r] = nxt[r];
+    }
+    return v;
+  }
+
+  // What the back substitution's task chain needs before it starts, read at the start of
+  // the solve: this lane's header words of tasks `lane` and lane + 64, and its term words
+  // of tasks 0 to 4.
+  struct ChainAhead
+  {
+    uint2 h0, h1;
+    uint32_t t0;
+    uint32_t terms[4];
+  };
+
+  __device__ __forceinline__ ChainAhead chain_ahead(const DevChain& ch, bool on)
+  {
+    ChainAhead a{};
+    if (on && threadIdx.x < kWave)
+    {
+      const uint32_t lane = threadIdx.x;
+      const uint2* h = reinterpret_cast<const uint2*>(ch.headers);
+      a.h0 = h[lane];
+      a.h1 = h[kWave + lane];
+      const int last = ch.n_tasks - 1;
+      a.t0 = ch.lane_terms[lane];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        a.terms[q] = ch.lane_terms[static_cast<size_t>(1 + q < last ? 1 + q : last) * kChainLanes + lane];
+    }
+    return a;
+  }
+
+  // The back substitution's task chain (TaskChain) by the first wavefront, tasks in
+  // order.  Lane k reads the operands of product k of task t + 1 before task t is summed
+  // (all lanes alike, from the products buffer of its parity) and writes the product to
+  // the other buffer after it, except the late one (reading task t's result), whose
+  // factor value it keeps; the sum of task t + 1 forms that product from the result in a
+  // register.  Tasks run in blocks of four; the lane term words of the next block are
+  // read when a block starts.  P holds two buffers of kChainLanes doubles and the
+  // ordered sums' read slack.
+  template<int NT>
+  __device__ inline void coop_chain(const DevChain& ch, const ChainAhead& ahead, const double* A, double* x,
+                                    double* P)
+  {
+    if (threadIdx.x < kWave)
+    {
+      const uint32_t lane = threadIdx.x;
+      const int nt = ch.n_tasks;
+      auto term_of = [&](int t) { return ch.lane_terms[static_cast<size_t>(t < nt ? t : nt - 1) * kChainLanes + lane]; };
+      // header words of task t (uniform)
+      auto head = [&](int t, bool second) {
+        const uint32_t w0 = second ? ahead.h0.y : ahead.h0.x, w1 = second ? ahead.h1.y : ahead.h1.x;
+        return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(t < kWave ? w0 : w1), t & (kWave - 1)));
+      };
+      // products of task 0 (nothing before it in the chain)
+      {
+        const double a = A[ahead.t0 & 0xffffu], xx = x[ahead.t0 >> 16];
+        if (lane < (head(0, true) & 0xffffu))
+          P[lane] = a * xx;
+      }
+      double x_prev = 0.0, late_now = 0.0;
+      auto step = [&](int t, uint32_t term_next) {
+        if (t >= nt)
+          return;
+        // the operands of task t + 1's products (none past the last task): read first,
+        // multiplied after task t, so that their reads overlap its sum
+        const uint32_t w1 = t + 1 < nt ? head(t + 1, true) : 0u;
+        const uint32_t n1 = w1 & 0xffffu, late1 = w1 >> 16;
+        const double a1 = A[term_next & 0xffffu], x1 = x[term_next >> 16];
+        // task t
+        const uint32_t od = head(t, false), w = head(t, true);
+        const uint32_t o = od & 0xffffu, d = od >> 16, n = w & 0xffffu, late = w >> 16;
+        const double* p = P + (t & 1) * kChainLanes;
+        const double dv = d < kUnitDivisor ? A[d] : 1.0;
+        double v = x[o];
+        __builtin_amdgcn_sched_barrier(0);
+        if (late < n)
+        {
+          if (late > 0)
+            v = ordered_sub_short(v, p, late);
+          v -= readlane_f64(late_now, late) * x_prev;
+          v = ordered_sub_short(v, p + late + 1, n - late - 1);
+        }
+        else
+        {
+          v = ordered_sub_short(v, p, n);
+        }
+        if (d < kUnitDivisor)
+          v = v / dv;
+        if (lane == 0)
+          x[o] = v;
+        x_prev = v;
+        __builtin_amdgcn_sched_barrier(0);
+        // task t + 1's products that do not read task t's result; the late one's factor
+        late_now = a1;
+        if (lane < n1 && lane != late1)
+          P[((t + 1) & 1) * kChainLanes + lane] = a1 * x1;
+      };
+      uint32_t cur[4] = { ahead.terms[0], ahead.terms[1], ahead.terms[2], ahead.terms[3] };
+      for (int t = 0; t < nt; t += 4)
+      {
+        uint32_t nxt[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          nxt[q] = term_of(t + 5 + q);
+        step(t, cur[0]);
+        step(t + 1, cur[1]);
+        step(t + 2, cur[2]);
+        step(t + 3, cur[3]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          cur[q] = nxt[q];
+      }
+    }
+    lds_barrier<NT>();
+  }
+
   // The forward substitution's column sweep (ColumnSweep) by the first wavefront: lane r
   // keeps its row's partial sum in a register; at column c the column's value comes from
   // the lane that finished it (or from x, final before the sweep) through a lane read,
@@ -804,54 +1001,11 @@ namespace micm_hip
     int b0 = bound(0), b1 = bound(1), b2 = bound(2), b3 = bound(3);
     int q0 = term_bound(0), q1 = term_bound(1), q2 = term_bound(2), q3 = term_bound(3);
     Ahead r0 = fetch(0, b0, b1, q0, q1), r1 = fetch(1, b1, b2, q1, q2), r2;
-    // a task's ordered sum of its products in batches of eight, the reads of each batch
-    // issued before the previous batch is subtracted; it reads up to fifteen products
-    // past its last one (inside the buffer's slack), unused
+    // a task's ordered sum of its products
     auto sum = [&](const uint4& h) {
       const uint32_t o = h.x & 0xffffu, d = h.x >> 16, n = h.z;
-      double v = out[o];
       const double dv = d < kUnitDivisor ? A[d] : 1.0;
-      const double* p = P + (static_cast<int>(h.y) - q0);
-      double a[8], b[8];
-#pragma unroll
-      for (int r = 0; r < 8; ++r)
-        a[r] = p[r];
-      uint32_t q = 0;
-      for (; q + 16 <= n; q += 16)
-      {
-#pragma unroll
-        for (int r = 0; r < 8; ++r)
-          b[r] = p[q + 8 + r];
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int r = 0; r < 8; ++r)
-          v -= a[r];
-#pragma unroll
-        for (int r = 0; r < 8; ++r)
-          a[r] = p[q + 16 + r];
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int r = 0; r < 8; ++r)
-          v -= b[r];
-      }
-      if (q + 8 <= n)
-      {
-#pragma unroll
-        for (int r = 0; r < 8; ++r)
-          b[r] = p[q + 8 + r];
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int r = 0; r < 8; ++r)
-          v -= a[r];
-#pragma unroll
-        for (int r = 0; r < 8; ++r)
-          a[r] = b[r];
-        q += 8;
-      }
-#pragma unroll
-      for (int r = 0; r < 8; ++r)
-        if (q + r < n)
-          v -= a[r];
+      double v = ordered_sub(out[o], P + (static_cast<int>(h.y) - q0), n);
       if (d < kUnitDivisor)
         v = v / dv;
       out[o] = v;
@@ -1077,6 +1231,9 @@ namespace micm_hip
     // the forward substitution's levels before its column sweep, then the sweep
     const bool sweep = c.fwd_sweep.first_level >= 0;
     const uint4 me = sweep && threadIdx.x < kWave ? c.fwd_sweep.lanes[threadIdx.x] : make_uint4(0u, 0u, 0u, 0u);
+    // what the back substitution's task chain reads first
+    const bool chain = GT && c.bwd_chain.levels > 0 && L.p_capacity >= 2 * kChainLanes;
+    const ChainAhead ahead = chain_ahead(c.bwd_chain, chain);
     DevLevels fwd = c.fwd;
     if (sweep)
       fwd.n_levels = c.fwd_sweep.first_level;
@@ -1085,10 +1242,18 @@ namespace micm_hip
       coop_levels_piped<NT, false>(fwd, c.fwd.terms, L.alu, x, x);
       if (sweep)
         coop_sweep<NT>(c.fwd_sweep, me, L.alu, x);
+      // the back substitution's first levels as a task chain, then the rest by levels
+      DevLevels bwd = c.bwd;
+      if (chain)
+      {
+        coop_chain<NT>(c.bwd_chain, ahead, L.alu, x, L.products);
+        bwd.offsets += c.bwd_chain.levels;
+        bwd.n_levels -= c.bwd_chain.levels;
+      }
       if (c.bwd.max_level_terms <= L.p_capacity)
-        coop_levels_piped2<NT>(c.bwd, c.bwd.terms, L.alu, x, x, L.products);
+        coop_levels_piped2<NT>(bwd, c.bwd.terms, L.alu, x, x, L.products);
       else
-        coop_levels<NT>(c.bwd, T.bwd_tasks, T.bwd_terms, L.alu, x, x, L.products, L.p_capacity);
+        coop_levels<NT>(bwd, T.bwd_tasks, T.bwd_terms, L.alu, x, x, L.products, L.p_capacity);
     }
     else
     {

@@ -77,6 +77,16 @@ namespace micm_hip
     int32_t n_cols;
   };
 
+  // The back substitution's task chain (TaskChain): its first `levels` levels as n_tasks
+  // tasks run in order by one wavefront.
+  struct DevChain
+  {
+    const uint32_t* headers;     // 2 per task, kChainTasks tasks
+    const uint32_t* lane_terms;  // kChainLanes per task
+    int32_t levels;              // 0: none
+    int32_t n_tasks;
+  };
+
   // Tables of the cooperative (one workgroup per cell) form; see MechanismTables.
   struct DevCoop
   {
@@ -88,6 +98,7 @@ namespace micm_hip
     const int32_t* alu_upper;  // U storage index -> ALU position
     DevLevels lu, fwd, bwd;
     DevSweep fwd_sweep;
+    DevChain bwd_chain;
     const uint64_t* rxn_pack;   // reactant ids of each reaction (see MechanismTables)
     const uint64_t* term_pack;  // process and dependent ids of each Jacobian term
     const int32_t* spec_offsets;

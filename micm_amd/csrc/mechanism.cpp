@@ -488,6 +488,57 @@ namespace micm_hip
       return best;
     }
 
+    // The prefix of a back substitution schedule's levels that runs fastest as a task
+    // chain (see TaskChain): a level costs about a barrier pair and three LDS round trips
+    // plus its longest sum, a chained task its division plus its sum, so the prefix kept
+    // is the one that saves the most by that count.  Tasks of at most kChainLanes terms,
+    // at most kChainTasks tasks.
+    TaskChain task_chain(const LevelSchedule& s)
+    {
+      constexpr int64_t kLevelCost = 1000, kTaskCost = 200, kTermCost = 8;
+      TaskChain best;
+      const int32_t nl = s.n_levels();
+      int64_t saved = 0, best_saved = 0;
+      for (int32_t l = 0; l < nl && s.offsets[l + 1] <= kChainTasks; ++l)
+      {
+        int64_t longest = 0, tasks_cost = 0;
+        bool ok = true;
+        for (int32_t q = s.offsets[l]; q < s.offsets[l + 1]; ++q)
+        {
+          const int64_t n = s.tasks[2 * q + 3] - s.tasks[2 * q + 1];
+          ok = ok && n <= kChainLanes;
+          longest = std::max(longest, n);
+          tasks_cost += kTaskCost + kTermCost * n;
+        }
+        if (!ok)
+          break;
+        saved += kLevelCost + kTermCost * longest - tasks_cost;
+        if (saved > best_saved)
+        {
+          best_saved = saved;
+          best.levels = l + 1;
+        }
+      }
+      if (best.levels == 0)
+        return best;
+      uint32_t prev_out = 0xffffffffu;
+      for (int32_t q = 0; q < s.offsets[best.levels]; ++q)
+      {
+        const uint32_t first = s.tasks[2 * q + 1], n = s.tasks[2 * q + 3] - first;
+        uint32_t late = 0xffffffffu;
+        for (uint32_t k = 0; k < n; ++k)
+          if ((s.terms[first + k] >> 16) == prev_out)
+            late = k;
+        best.headers.push_back(s.tasks[2 * q]);
+        best.headers.push_back(n | ((late & 0xffffu) << 16));
+        for (uint32_t k = 0; k < static_cast<uint32_t>(kChainLanes); ++k)
+          best.lane_terms.push_back(k < n ? s.terms[first + k] : 0u);
+        prev_out = s.tasks[2 * q] & 0xffffu;
+      }
+      best.headers.resize(2 * kChainTasks, 0u);
+      return best;
+    }
+
     void build_cooperative_schedule(MechanismTables& t)
     {
       const int32_t n = static_cast<int32_t>(t.n_species);
@@ -527,6 +578,7 @@ namespace micm_hip
       t.lv_lu_capped.clear();
       t.lv_fwd = LevelSchedule{};
       t.fwd_sweep = ColumnSweep{};
+      t.bwd_chain = TaskChain{};
       t.lv_bwd = LevelSchedule{};
       if (!t.coop_levels_ok)
         return;
@@ -620,6 +672,7 @@ namespace micm_hip
                                 kSolveSegmentMin) + 1;
         }
         t.lv_bwd = pack_levels(tasks);
+        t.bwd_chain = task_chain(t.lv_bwd);
       }
       // forcing gather, in the scatter order of the reference (reactions ascending;
       // within a reaction its reactant slots, then its product slots)

@@ -73,6 +73,24 @@ namespace micm_hip
   constexpr int kSweepLanes = 64;
   constexpr uint32_t kSweepFromLane = 0x80000000u;
 
+  // The first levels of the back substitution as one chain of tasks run in order by a
+  // single wavefront: lane k forms product k of a task, the wavefront subtracts them in
+  // order.  While task t is summed, the lanes already form the products of task t + 1,
+  // except the one (`late`) that reads task t's own result, which the sum of task t + 1
+  // forms in its place; so the chain costs about one multiply, the task's subtractions
+  // and its division per task instead of two workgroup barriers and three LDS round trips
+  // per level.  The same operations in the same order as the level schedule.
+  struct TaskChain
+  {
+    int32_t levels = 0;  // levels [0, levels) of the schedule run as the chain (0: none)
+    // kChainTasks entries of 2 words (0 past the last task): out | diag << 16, number of
+    // terms | index of the late term << 16 (0xffff: none)
+    std::vector<uint32_t> headers;
+    std::vector<uint32_t> lane_terms;  // kChainLanes per task: term k of the task (0 past its last)
+  };
+  constexpr int kChainLanes = 64;
+  constexpr int kChainTasks = 128;  // the headers sit in two registers of each lane
+
   // Rate parameters as consumed by the device (per sorted reaction, stride
   // MICM_HIP_RATE_PARAM_STRIDE).  Derived values live in spare slots:
   //   BRANCHED  slot 5 = k0, slot 6 = z   (reaction_rate_store.hpp:286-293)
@@ -145,6 +163,7 @@ namespace micm_hip
     std::vector<LevelSchedule> lv_lu_capped;
     LevelSchedule lv_fwd;  // out = row, diag = kUnitDivisor on a row's last segment, terms (ALU position | column << 16)
     ColumnSweep fwd_sweep;  // lv_fwd's last levels as a column sweep
+    TaskChain bwd_chain;    // lv_bwd's first levels as a task chain
     LevelSchedule lv_bwd;  // out = row, diag = U[i][i] position, terms (ALU position | column << 16)
     // packed operands: reaction -> reactant ids, Jacobian term -> (process, dependent ids);
     // three / two 16-bit ids and the count in bits 48-63 (0xffff: more, use the lists)

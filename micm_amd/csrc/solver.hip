@@ -2009,7 +2009,8 @@ namespace
              padded_bytes(t.fwd_pairs) + padded_bytes(t.bwd_offsets) + padded_bytes(t.bwd_pairs) +
              padded_bytes(t.jac_to_alu) + padded_bytes(t.alu_diag) + padded_bytes(t.alu_lower) +
              padded_bytes(t.alu_upper) + level_bytes(t.lv_lu) + blocked_level_bytes(t) + level_bytes(t.lv_fwd) +
-             padded_bytes(t.fwd_sweep.lanes) + level_bytes(t.lv_bwd) +
+             padded_bytes(t.fwd_sweep.lanes) + level_bytes(t.lv_bwd) + padded_bytes(t.bwd_chain.headers) +
+             padded_bytes(t.bwd_chain.lane_terms) +
              padded_bytes(t.rxn_pack) + padded_bytes(t.term_pack) +
              padded_bytes(t.spec_offsets) + padded_bytes(t.spec_rxn) +
              padded_bytes(t.spec_coef) + padded_bytes(t.jel_offsets) + padded_bytes(t.jel_term) +
@@ -2078,6 +2079,14 @@ namespace
       c.fwd_sweep = DevSweep{ reinterpret_cast<const uint4*>(lanes), t.fwd_sweep.first_level, t.fwd_sweep.n_cols };
     }
     upload_levels(m->blob, off, t.lv_bwd, c.bwd, staging);
+    c.bwd_chain = DevChain{ nullptr, nullptr, 0, 0 };
+    if (t.bwd_chain.levels > 0)
+    {
+      upload(m->blob, off, t.bwd_chain.headers, c.bwd_chain.headers, staging);
+      upload(m->blob, off, t.bwd_chain.lane_terms, c.bwd_chain.lane_terms, staging);
+      c.bwd_chain.levels = t.bwd_chain.levels;
+      c.bwd_chain.n_tasks = static_cast<int32_t>(t.bwd_chain.lane_terms.size() / kChainLanes);
+    }
     upload(m->blob, off, t.rxn_pack, c.rxn_pack, staging);
     upload(m->blob, off, t.term_pack, c.term_pack, staging);
     upload(m->blob, off, t.spec_offsets, c.spec_offsets, staging);

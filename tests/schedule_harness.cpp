@@ -235,8 +235,9 @@ namespace
 
 extern "C"
 {
-  // harness_factor_solve with the forward substitution's column sweep; returns the first
-  // swept level (-1: no sweep, and b untouched).
+  // harness_factor_solve with the forward substitution's column sweep and the back
+  // substitution's task chain; returns the first swept level (-1: no sweep, and b
+  // untouched).
   int harness_solve_swept(const micm_hip_mechanism_desc* desc, const double* jac, double alpha, double* b)
   {
     const MechanismTables t = micm_hip::compile_mechanism(*desc);
@@ -252,9 +253,69 @@ extern "C"
     alu.push_back(0.0);  // the zero one past the factor
     std::vector<double> x(b, b + n);
     forward_swept(t, alu, x);
-    run(t.lv_bwd, alu.data(), none, x, false, false);
+    // the back substitution as the device runs it: the task chain in its order, each
+    // task's late product formed from the previous task's result, then the other levels
+    const micm_hip::TaskChain& ch = t.bwd_chain;
+    double x_prev = 0.0;
+    for (size_t q = 0; q < ch.lane_terms.size() / micm_hip::kChainLanes; ++q)
+    {
+      const uint32_t o = ch.headers[2 * q] & 0xffffu, d = ch.headers[2 * q] >> 16,
+                     nt = ch.headers[2 * q + 1] & 0xffffu, late = ch.headers[2 * q + 1] >> 16;
+      double v = x[o];
+      for (uint32_t k = 0; k < nt; ++k)
+      {
+        const uint32_t tr = ch.lane_terms[q * micm_hip::kChainLanes + k];
+        v -= alu[tr & 0xffffu] * (k == late ? x_prev : x[tr >> 16]);
+      }
+      if (d < micm_hip::kUnitDivisor)
+        v = v / alu[d];
+      x[o] = v;
+      x_prev = v;
+    }
+    LevelSchedule tail = t.lv_bwd;
+    tail.offsets.erase(tail.offsets.begin(), tail.offsets.begin() + ch.levels);
+    run(tail, alu.data(), none, x, false, false);
     std::memcpy(b, x.data(), sizeof(double) * n);
     return t.fwd_sweep.first_level;
+  }
+
+  // The back substitution's task chain against its level schedule: the chain holds the
+  // tasks of its levels in level order with their terms; a task's late term is the one
+  // reading the previous task's output, and every other term reads a value whose last
+  // writer comes at least two tasks earlier (its product is formed a task ahead).
+  // Returns 0, or 1 + the index of the first task that breaks this.
+  int harness_chain_check(const micm_hip_mechanism_desc* desc)
+  {
+    const MechanismTables t = micm_hip::compile_mechanism(*desc);
+    const micm_hip::TaskChain& ch = t.bwd_chain;
+    const LevelSchedule& s = t.lv_bwd;
+    if (ch.levels == 0)
+      return ch.headers.empty() && ch.lane_terms.empty() ? 0 : 1;
+    const size_t nt = ch.lane_terms.size() / micm_hip::kChainLanes;
+    if (nt != static_cast<size_t>(s.offsets[ch.levels]) || nt > static_cast<size_t>(micm_hip::kChainTasks) ||
+        ch.headers.size() != 2 * static_cast<size_t>(micm_hip::kChainTasks))
+      return 1;
+    std::vector<int64_t> last_writer(t.n_species, -1);
+    for (size_t q = 0; q < nt; ++q)
+    {
+      const uint32_t first = s.tasks[2 * q + 1], n = s.tasks[2 * q + 3] - first;
+      if (ch.headers[2 * q] != s.tasks[2 * q] || (ch.headers[2 * q + 1] & 0xffffu) != n || n > micm_hip::kChainLanes)
+        return static_cast<int>(q) + 1;
+      const uint32_t prev_out = q > 0 ? (ch.headers[2 * (q - 1)] & 0xffffu) : 0xffffffffu;
+      for (uint32_t k = 0; k < n; ++k)
+      {
+        const uint32_t tr = ch.lane_terms[q * micm_hip::kChainLanes + k], col = tr >> 16;
+        if (tr != s.terms[first + k])
+          return static_cast<int>(q) + 1;
+        const bool late = col == prev_out;
+        if (late != ((ch.headers[2 * q + 1] >> 16) == k))
+          return static_cast<int>(q) + 1;
+        if (!late && last_writer[col] >= 0 && last_writer[col] + 2 > static_cast<int64_t>(q))
+          return static_cast<int>(q) + 1;
+      }
+      last_writer[ch.headers[2 * q] & 0xffffu] = static_cast<int64_t>(q);
+    }
+    return 0;
   }
 
   // Per level of schedule `which` (0 LU, 1 forward, 2 backward): tasks and the

@@ -34,6 +34,7 @@ def harness(tmp_path_factory):
                                           ctypes.POINTER(ctypes.c_int32)]
     lib.harness_blocked_conflict.argtypes = [ctypes.c_void_p, ctypes.c_int32]
     lib.harness_solve_swept.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp]
+    lib.harness_chain_check.argtypes = [ctypes.c_void_p]
     return lib
 
 
@@ -87,6 +88,10 @@ def test_level_schedules_reproduce_doolittle(harness, name):
             assert first > 0, name  # these mechanisms sweep the forward tail
         if first >= 0:
             assert bits_equal(xs, x_ref[0])
+        if c == 0:
+            # the back substitution's task chain reproduces its levels' tasks, and no
+            # product formed a task ahead reads a value the previous task writes
+            assert harness.harness_chain_check(ctypes.byref(cdesc)) == 0, name
         # every LU blocking (shortest segment 1, 2, 4, 8, whole sums) gives the same factors;
         # the blocking changes how a sum is split over the levels, not the level at which
         # the element is final, so every blocking has as many levels
