@@ -262,18 +262,19 @@ def kernel_rooflines(mech, state, stream, n_cells, repeats=5):
     return out
 
 
-def measured_traffic(workload_name, n_cells, threads):
+def measured_traffic(workload_name, n_cells, threads, stage):
     """HBM bytes per Solve launch from a committed rocprofv3 PMC summary
     (profiles/*_hbm_traffic.md, FETCH_SIZE and WRITE_SIZE runs scaled by the same
     counters on a 1 GiB copy) when one covers this workload, cell count and
-    cooperative shape; None otherwise."""
+    cooperative shape (threads and the requested stage, which fixes the build's
+    register bound); None otherwise."""
     import glob
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_hbm_traffic.md"))):
         text = open(path).read()
         for block in text.split("```json")[1:]:
             rec = json.loads(block.split("```")[0])
-            if (rec.get("workload"), rec.get("cells"), rec.get("kernel")) == \
-                    (workload_name, n_cells, f"k_coop<{threads}>"):
+            if (rec.get("workload"), rec.get("cells"), rec.get("kernel"), rec.get("stage")) == \
+                    (workload_name, n_cells, f"k_coop<{threads}>", stage):
                 return rec["traffic_bytes"], os.path.relpath(path, ROOT)
     return None, None
 
@@ -420,6 +421,7 @@ def main():
         form_label = f"autotuned: {choice['label']}" + (f" (hand-off to {choice['handoff_to']})"
                                                          if choice.get("handoff_to") else "")
         coop_threads = choice["threads"] if form == _lib.SOLVE_COOPERATIVE else 0
+        coop_stage = choice["stage"]
     else:
         form = {"persistent": _lib.SOLVE_PERSISTENT, "plain": _lib.SOLVE_PLAIN,
                 "cooperative": _lib.SOLVE_COOPERATIVE}[args.form]
@@ -427,6 +429,7 @@ def main():
         state.set_handoff(args.handoff)
         form_label = args.form + (f" ({args.coop_threads} threads per cell)" if args.coop_threads else "")
         coop_threads = args.coop_threads if form == _lib.SOLVE_COOPERATIVE else 0
+        coop_stage = args.coop_stage
         choice = None
 
     ptr = state.pointers()
@@ -477,7 +480,8 @@ def main():
     achieved = bytes_per_step / (kernel_ms * 1e-3) / 1e9
     flops_per_step = algorithmic_flops(flops_per_invocation(desc, mech), stats)
     tflops = flops_per_step / (kernel_ms * 1e-3) / 1e12
-    traffic, traffic_source = measured_traffic(args.workload, n_cells, coop_threads) if coop_threads else (None, None)
+    traffic, traffic_source = (measured_traffic(args.workload, n_cells, coop_threads, coop_stage) if coop_threads
+                               else (None, None))
     conv = status == 2
     result = {
         "metric": {"210_560": "cell-solves/s, 210-species x 100k cells, batched Rosenbrock (FP64)",
