@@ -173,7 +173,9 @@ namespace micm_hip
         for (int i = tid; i < n; i += NT)
           alu[c.alu_diag[i]] += alpha * 1.0;
         coop_sync<NT>();
-        coop_levels_direct<NT, true>(c.lu, c.lu.tasks, c.lu.terms, alu, alu, alu);
+        // (the LU's records name the zero past the factor in their padded slots, which
+        // this layout does not keep: the unpadded executor tests every slot)
+        coop_levels_piped<NT, false>(c.lu, c.lu.terms, alu, alu, alu);
         for (int s = 0; s < S; ++s)
         {
           const int sc = s * (s - 1) / 2;
@@ -204,8 +206,8 @@ namespace micm_hip
             Ks[i] = v;
           }
           coop_sync<NT>();
-          coop_levels_direct<NT, true>(c.fwd, c.fwd.tasks, c.fwd.terms, alu, Ks, Ks);
-          coop_levels_direct<NT, true>(c.bwd, c.bwd.tasks, c.bwd.terms, alu, Ks, Ks);
+          coop_levels_piped<NT, false>(c.fwd, c.fwd.terms, alu, Ks, Ks);
+          coop_levels_piped<NT, false>(c.bwd, c.bwd.terms, alu, Ks, Ks);
         }
         // ---- reverse over the stages
         for (int i = tid; i < n; i += NT)
@@ -222,8 +224,8 @@ namespace micm_hip
           for (int i = tid; i < n; i += NT)
             rb[i] = Kb[static_cast<size_t>(s) * n + i];
           coop_sync<NT>();
-          coop_levels_direct<NT, true>(a.ut, a.ut.tasks, a.ut.terms, alu, rb, rb);
-          coop_levels_direct<NT, true>(a.lt, a.lt.tasks, a.lt.terms, alu, rb, rb);
+          coop_levels_piped<NT, false>(a.ut, a.ut.terms, alu, rb, rb);
+          coop_levels_piped<NT, false>(a.lt, a.lt.terms, alu, rb, rb);
           for (int r = tid; r < R; r += NT)
             w[r] = reaction_weight(m, r, rb);
           coop_sync<NT>();
