@@ -21,6 +21,9 @@ from . import _lib
 
 # (label, form, handoff steps, cooperative threads, staging); the shapes that are
 # usually fast come first, so that the pilot can give up on slow ones early
+# the same shape built with the larger register budget
+WIDER_BUILD = {"cooperative256-compact-w3": "cooperative256-compact-w2"}
+
 CANDIDATES = (
     ("cooperative128", _lib.SOLVE_COOPERATIVE, -1, 128, -1),
     ("cooperative256", _lib.SOLVE_COOPERATIVE, -1, 256, -1),
@@ -193,7 +196,13 @@ def autotune(mech, state, dt, params, sample=65536, first_sample=2048, pilot=256
                                       lu_blocking=lu_blocking))
     last = cands + lane_forms
     cands = [c for c in last if c[0] in times]
-    label, form, handoff, threads, stage = min(cands, key=lambda c: times[c[0]])
+    win = min(cands, key=lambda c: times[c[0]])
+    # a build bounded to fewer registers wins only by more than 1 %: its extra waves per
+    # SIMD help only where LDS holds that many cells per CU, and its spills cost HBM traffic
+    wider = {c[0]: c for c in cands}.get(WIDER_BUILD.get(win[0]))
+    if wider is not None and times[wider[0]] <= 1.01 * times[win[0]]:
+        win = wider
+    label, form, handoff, threads, stage = win
     choice = {"label": label, "form": form, "handoff": handoff, "threads": threads, "stage": stage,
               "handoff_to": hand_to.get(label) if handoff > 0 else None, "lu_blocking": lu_blocking,
               "lu_blocking_seconds": blocking_times, "sample": m, "seconds": times}

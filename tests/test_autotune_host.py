@@ -163,3 +163,14 @@ def test_lu_blocking_is_chosen_for_the_fastest_cooperative_shape(monkeypatch):
     assert ("cooperative256", 65536, 8) in calls
     assert all(b == 8 for (label, n, b) in calls if n == 65536 and label.startswith("persistent"))
     assert set(choice["lu_blocking_seconds"]) == {1, 2, 8, -1}
+
+
+def test_wider_register_budget_wins_a_near_tie(monkeypatch):
+    # the three-wave compact build 0.5 % faster than the two-wave one: the two-wave build is
+    # taken (no spills); 2 % faster: the three-wave build is taken
+    for margin, expected in ((0.995, "cooperative256-compact-w2"), (0.98, "cooperative256-compact-w3")):
+        model = {label: _linear(0.5, 1e-3) for label, *_ in autotune.CANDIDATES}
+        model["cooperative256-compact-w2"] = _linear(0.01, 1e-5)
+        model["cooperative256-compact-w3"] = lambda n, m=margin: m * (0.01 + 1e-5 * n)
+        choice, _ = _run(monkeypatch, model, 100000, tag=110 + int(margin * 1000))
+        assert choice["label"] == expected, (margin, choice["label"])
