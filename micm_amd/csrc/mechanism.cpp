@@ -489,13 +489,15 @@ namespace micm_hip
     }
 
     // The prefix of a back substitution schedule's levels that runs fastest as a task
-    // chain (see TaskChain): a level costs about a barrier pair and three LDS round trips
-    // plus its longest sum, a chained task its division plus its sum, so the prefix kept
-    // is the one that saves the most by that count.  Tasks of at most kChainLanes terms,
-    // at most kChainTasks tasks.
+    // chain (see TaskChain): a level costs about a barrier pair, three LDS round trips
+    // and its instructions plus its longest sum, a chained task its instructions, its
+    // division and its sum (about 1,700 and 1,000 clocks plus 8 a product, measured on
+    // MI355X with 210/560), so the prefix kept is the one that saves the most by that
+    // count: levels of one task gain, levels of several lose.  Tasks of at most
+    // kChainLanes terms, at most kChainTasks tasks.
     TaskChain task_chain(const LevelSchedule& s)
     {
-      constexpr int64_t kLevelCost = 1000, kTaskCost = 200, kTermCost = 8;
+      constexpr int64_t kLevelCost = 1700, kTaskCost = 1000, kTermCost = 8;
       TaskChain best;
       const int32_t nl = s.n_levels();
       int64_t saved = 0, best_saved = 0;
