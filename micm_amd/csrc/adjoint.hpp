@@ -120,6 +120,13 @@ namespace micm_hip
     L.scratch = tmp;
     L.jterm = jterm;
     const size_t rec_len = static_cast<size_t>(n) + 1;
+    // the forward substitution's levels before its column sweep, then the sweep (whose
+    // reads past a row's last factor entry stay inside this layout's LDS, unused)
+    const bool sweep = c.fwd_sweep.first_level >= 0;
+    const uint4 me = sweep && tid < kWave ? c.fwd_sweep.lanes[tid] : make_uint4(0u, 0u, 0u, 0u);
+    DevLevels fwd = c.fwd;
+    if (sweep)
+      fwd.n_levels = c.fwd_sweep.first_level;
     for (int64_t cell = blockIdx.x; cell < n_cells; cell += gridDim.x)
     {
       const size_t g = cell / kGroup;
@@ -206,7 +213,9 @@ namespace micm_hip
             Ks[i] = v;
           }
           coop_sync<NT>();
-          coop_levels_piped<NT, false>(c.fwd, c.fwd.terms, alu, Ks, Ks);
+          coop_levels_piped<NT, false>(fwd, c.fwd.terms, alu, Ks, Ks);
+          if (sweep)
+            coop_sweep<NT>(c.fwd_sweep, me, alu, Ks);
           coop_levels_piped<NT, false>(c.bwd, c.bwd.terms, alu, Ks, Ks);
         }
         // ---- reverse over the stages
