@@ -162,9 +162,13 @@ def autotune(mech, state, dt, params, sample=65536, first_sample=2048, pilot=256
         times.update(_time_candidates(mech, *pick(m1), dt, params, cands, repeats, verbose))
         ranked = sorted(cands, key=lambda c: times[c[0]])
         best1 = times[ranked[0][0]]
-        # the plain form gains the most from more cells: it stays in at twice the best
-        cands = ranked[:finalists] + [c for c in ranked[finalists:]
-                                      if c[1] == _lib.SOLVE_PLAIN and times[c[0]] <= 2.0 * best1]
+        # the plain form gains the most from more cells: it stays in at twice the best; a
+        # finalist's build with the larger register budget stays in beside it (the last pick
+        # prefers it on a near tie)
+        top = ranked[:finalists]
+        wider = [c for c in ranked[finalists:] if c[0] in {WIDER_BUILD.get(f[0]) for f in top}]
+        cands = top + wider + [c for c in ranked[finalists:]
+                               if c[1] == _lib.SOLVE_PLAIN and times[c[0]] <= 2.0 * best1 and c not in wider]
     first_times = dict(times)
     if m > m1:
         times.update(_time_candidates(mech, *pick(m), dt, params, cands, repeats, verbose))

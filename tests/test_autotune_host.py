@@ -174,3 +174,18 @@ def test_wider_register_budget_wins_a_near_tie(monkeypatch):
         model["cooperative256-compact-w3"] = lambda n, m=margin: m * (0.01 + 1e-5 * n)
         choice, _ = _run(monkeypatch, model, 100000, tag=110 + int(margin * 1000))
         assert choice["label"] == expected, (margin, choice["label"])
+
+
+def test_wider_build_reaches_the_last_sample_beside_its_twin(monkeypatch):
+    # the two-wave build ranks outside the three finalists on the first sample, its
+    # three-wave twin inside: both reach the last sample, where the three-wave build is
+    # fastest by 0.5 % and the near tie goes to the two-wave build
+    base = _linear(0.0099, 0.99e-5)
+    model = {label: _linear(0.5, 1e-3) for label, *_ in autotune.CANDIDATES}
+    model["cooperative256"] = lambda n: (1.02 if n > 2048 else 0.95) * base(n)
+    model["cooperative512"] = lambda n: (1.03 if n > 2048 else 0.97) * base(n)
+    model["cooperative256-compact-w3"] = base
+    model["cooperative256-compact-w2"] = lambda n: (1.005 if n > 2048 else 1.5) * base(n)
+    choice, calls = _run(monkeypatch, model, 100000, tag=130)
+    assert ("cooperative256-compact-w2", 65536) in calls
+    assert choice["label"] == "cooperative256-compact-w2"
