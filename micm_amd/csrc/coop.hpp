@@ -898,7 +898,9 @@ namespace micm_hip
       auto step = [&](int c, double a) {
         const uint32_t cl = static_cast<uint32_t>(c) & 63u;
         const uint32_t s = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(src), cl));
-        const double xc = (s & kSweepFromLane) ? readlane_f64(v, s & 63u) : readlane_f64(xv, cl);
+        // both lane reads, then a scalar select (no branch per column)
+        const double from_lane = readlane_f64(v, s & 63u), from_x = readlane_f64(xv, cl);
+        const double xc = (s & kSweepFromLane) ? from_lane : from_x;
         const double nv = v - a * xc;
         v = (c < n_cols && ((mask >> cl) & 1u)) ? nv : v;
       };
